@@ -1,0 +1,513 @@
+// ORACLE -- test infrastructure only.
+//
+// Drives the reference Velox CPU library (built from /root/reference sources by
+// oracle/ref.mk into oracle/_ref/) for two jobs:
+//
+//   ref_fixtures <outdir>        (strict build) writes the golden vectors under
+//                                 tests/golden/: reference World trajectories,
+//                                 reference narrow-phase contacts per pair,
+//                                 reference solver sweeps, reference integration.
+//   ref_bench <scene> <steps> <workers> [warmup]
+//                                (relaxed build) times World(BackendType::Cpu)
+//                                 on a scene and prints one JSON line: the CPU
+//                                 baseline beside the HIP device step.
+//
+// The reference's device path (src/backend_cuda.cu) cannot be compiled here, so
+// CudaSemantics below re-expresses its host-visible semantics (exact-AABB pair
+// set, (a<<32|b, point) contact order, greedy coloring, 2*I colored sweeps per
+// substep, backend_cuda.cu:174-761) on top of the reference's own header math
+// (collidePair, warmStartContact, solveContact, Body::advanceTransform).  The
+// reference World then runs every other stage of its step (warm-start matching,
+// CCD net, positional correction) with its own code.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+#include <array>
+#include <atomic>
+#include <condition_variable>
+#include <unordered_map>
+#include <sys/stat.h>
+#define private public
+#define protected public
+#include "velox/velox.h"
+#undef private
+#undef protected
+#include "narrowphase.h"
+#include "../velox_amd/csrc/scene_gen.hpp"
+
+using namespace velox;
+
+namespace {
+
+// ------------------------------------------------------------ npy writer --
+void writeNpy(const std::string& path, const char* descr, const std::vector<size_t>& shape, const void* data,
+              size_t bytes) {
+    std::string dict = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': (";
+    for (size_t i = 0; i < shape.size(); ++i) dict += std::to_string(shape[i]) + (shape.size() == 1 ? "," : (i + 1 < shape.size() ? ", " : ""));
+    dict += "), }";
+    size_t total = 10 + dict.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    dict += std::string(pad, ' ') + "\n";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); std::exit(2); }
+    const unsigned char magic[8] = {0x93, 'N', 'U', 'M', 'P', 'Y', 1, 0};
+    std::fwrite(magic, 1, 8, f);
+    uint16_t hl = (uint16_t)dict.size();
+    std::fwrite(&hl, 2, 1, f);
+    std::fwrite(dict.data(), 1, dict.size(), f);
+    if (bytes) std::fwrite(data, 1, bytes, f);
+    std::fclose(f);
+}
+template <class T> void writeRecords(const std::string& path, const std::vector<T>& v) {
+    writeNpy(path, "|u1", {v.size(), sizeof(T)}, v.data(), v.size() * sizeof(T));
+}
+
+// ------------------------------------------------------------ scene build --
+std::vector<BodyId> build(World& w, const vxscene::Scene& s) {
+    w.gravity = {s.gravity[0], s.gravity[1], s.gravity[2]};
+    w.substeps = s.substeps;
+    std::vector<BodyId> ids;
+    for (const vxscene::BodySpec& b : s.bodies) {
+        Vec3 p{b.pos[0], b.pos[1], b.pos[2]};
+        BodyId id;
+        switch (b.kind) {
+        case vxscene::Sphere: id = w.addSphere(p, b.radius, b.mass); break;
+        case vxscene::Box: id = w.addBox(p, {b.half[0], b.half[1], b.half[2]}, b.mass); break;
+        case vxscene::Capsule: id = w.addCapsule(p, b.radius, b.halfHeight, b.mass); break;
+        case vxscene::Hull: {
+            std::vector<Vec3> pts;
+            for (uint32_t k = 0; k < b.hullCount; ++k) {
+                const float* q = &s.hullPoints[3 * (b.hullFirst + k)];
+                pts.push_back({q[0], q[1], q[2]});
+            }
+            id = w.addConvexHull(p, pts, b.mass);
+            break;
+        }
+        case vxscene::Plane: id = w.addStaticPlane({b.normal[0], b.normal[1], b.normal[2]}, b.offset); break;
+        default: {
+            const vxscene::HeightfieldSpec& hf = s.heightfields[b.hfIndex];
+            id = w.addStaticHeightfield(hf.width, hf.depth, hf.cell, hf.heights,
+                                        {hf.origin[0], hf.origin[1], hf.origin[2]});
+            break;
+        }
+        }
+        Body& body = w.body(id);
+        if (b.kind != vxscene::Plane && b.kind != vxscene::Heightfield) {
+            body.orientation = {b.quat[0], b.quat[1], b.quat[2], b.quat[3]};
+            body.velocity = {b.vel[0], b.vel[1], b.vel[2]};
+            body.angularVelocity = {b.angVel[0], b.angVel[1], b.angVel[2]};
+        }
+        body.friction = b.friction;
+        body.restitution = b.restitution;
+        body.enableSleep = 0;  // the device step has no sleep manager
+        ids.push_back(id);
+    }
+    return ids;
+}
+
+// ------------------------------------------- CUDA-backend semantics on CPU --
+struct Aabb { Vec3 lo, hi; };
+
+void exactPairs(const std::vector<Body>& bodies, const MeshSoup& meshes, float dt, std::vector<uint64_t>& pairs) {
+    const size_t n = bodies.size();
+    std::vector<Aabb> box(n);
+    for (size_t i = 0; i < n; ++i) {
+        const Body& b = bodies[i];
+        if (b.shape == ShapeType::Plane) box[i] = {{-1e30f, -1e30f, -1e30f}, {1e30f, 1e30f, 1e30f}};
+        else if (b.shape == ShapeType::Mesh) box[i] = {meshes.meshes[b.meshIndex].aabbMin, meshes.meshes[b.meshIndex].aabbMax};
+        else bodyAabb(b, dt, box[i].lo, box[i].hi);
+    }
+    pairs.clear();
+    // Sweep on x to keep the exact test affordable; the pair set is that of an
+    // all-pairs exact AABB test (backend_cuda.cu:174-250).
+    std::vector<uint32_t> order(n);
+    for (uint32_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return box[a].lo.x < box[b].lo.x; });
+    for (size_t s = 0; s < n; ++s) {
+        uint32_t i = order[s];
+        for (size_t t = s + 1; t < n; ++t) {
+            uint32_t j = order[t];
+            if (box[j].lo.x > box[i].hi.x) break;
+            if (!aabbOverlap(box[i].lo, box[i].hi, box[j].lo, box[j].hi)) continue;
+            const Body& a = bodies[i];
+            const Body& b = bodies[j];
+            if ((a.isStatic() || isFullyAsleep(a.asleep)) && (b.isStatic() || isFullyAsleep(b.asleep))) continue;
+            if (!a.canCollideWith(b)) continue;
+            uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+            pairs.push_back((uint64_t)lo << 32 | hi);
+        }
+    }
+    std::sort(pairs.begin(), pairs.end());
+}
+
+bool canonicalLess(const Contact& x, const Contact& y) {  // backend_cuda.cu:587-595
+    uint64_t kx = (uint64_t)x.a << 32 | x.b, ky = (uint64_t)y.a << 32 | y.b;
+    if (kx != ky) return kx < ky;
+    if (x.point.x != y.point.x) return x.point.x < y.point.x;
+    if (x.point.y != y.point.y) return x.point.y < y.point.y;
+    return x.point.z < y.point.z;
+}
+
+void pairContacts(const std::vector<Body>& bodies, const MeshSoup& meshes, float dt,
+                  const std::vector<uint64_t>& pairs, std::vector<Contact>& out) {
+    out.clear();
+    const MeshSoupView soup = view(meshes);
+    Contact buf[kMaxContactsPerPair];
+    for (uint64_t p : pairs) {
+        uint32_t i = uint32_t(p >> 32), j = uint32_t(p);
+        int c = collidePair(bodies[i], bodies[j], i, j, soup, dt, buf, kMaxContactsPerPair);
+        out.insert(out.end(), buf, buf + c);
+    }
+    // The reference sorts contacts that arrived in atomic order, so contacts
+    // comparing equal (same pair, same point) have no defined order there.  A
+    // stable sort fixes them in narrow-phase emission order.
+    std::stable_sort(out.begin(), out.end(), canonicalLess);
+}
+
+// Greedy first-free-bit coloring, backend_cuda.cu:597-630.
+int colorContacts(const std::vector<Body>& bodies, const std::vector<Contact>& cs, std::vector<int>& colorOf) {
+    std::vector<uint64_t> mask(bodies.size(), 0);
+    std::vector<int> next(bodies.size(), 64);
+    colorOf.resize(cs.size());
+    int K = 0;
+    for (size_t k = 0; k < cs.size(); ++k) {
+        const Contact& c = cs[k];
+        bool sensor = bodies[c.a].isSensor() || bodies[c.b].isSensor();
+        bool sa = sensor || !bodies[c.a].isDynamic();
+        bool sb = sensor || !bodies[c.b].isDynamic();
+        uint64_t used = (sa ? 0 : mask[c.a]) | (sb ? 0 : mask[c.b]);
+        int color;
+        if (~used) { color = 0; while (used & (1ull << color)) ++color; }
+        else {
+            int na = sa ? 64 : next[c.a], nb = sb ? 64 : next[c.b];
+            color = na > nb ? na : nb;
+            if (!sa) next[c.a] = color + 1;
+            if (!sb) next[c.b] = color + 1;
+        }
+        if (color < 64) { if (!sa) mask[c.a] |= 1ull << color; if (!sb) mask[c.b] |= 1ull << color; }
+        colorOf[k] = color;
+        K = std::max(K, color + 1);
+    }
+    return K;
+}
+
+void colorOrder(std::vector<Contact>& cs, const std::vector<int>& colorOf, int K) {  // backend_cuda.cu:634-640
+    std::vector<size_t> start(K + 1, 0);
+    for (int c : colorOf) ++start[c + 1];
+    for (int c = 0; c < K; ++c) start[c + 1] += start[c];
+    std::vector<Contact> sorted(cs.size());
+    for (size_t k = 0; k < cs.size(); ++k) sorted[start[colorOf[k]]++] = cs[k];
+    cs.swap(sorted);
+}
+
+class CudaSemantics final : public Backend {
+public:
+    CudaSemantics() : cpu_(createCpuBackend()) { cpu_->setWorkerCount(1); }
+    const char* name() const override { return "cuda-semantics"; }
+    uint32_t lastVelocityIterations() const override { return iters_; }
+    void integrate(std::vector<Body>& bodies, const Vec3& g, float dt) override { cpu_->integrate(bodies, g, dt); }
+    bool wantsHostPairs() const override { return false; }
+    void findContacts(const std::vector<Body>& bodies, const MeshSoup& meshes, float dt,
+                      const std::vector<uint64_t>*, std::vector<Contact>& out) override {
+        std::vector<uint64_t> pairs;
+        exactPairs(bodies, meshes, dt, pairs);
+        pairContacts(bodies, meshes, dt, pairs, out);
+        colors = 0;
+    }
+    void sweeps(std::vector<Body>& bodies, std::vector<Contact>& cs, float dt, const SolverOptions& o) {
+        for (int it = 0; it < 2 * o.velocityIterations; ++it)
+            for (Contact& c : cs)
+                if (!bodies[c.a].isSensor() && !bodies[c.b].isSensor()) solveContact(bodies[c.a], bodies[c.b], c, dt, o);
+    }
+    void solveVelocities(std::vector<Body>& bodies, std::vector<Contact>& cs, float dt, bool warm,
+                         const SolverOptions& o) override {
+        iters_ = 0;
+        if (cs.empty()) return;
+        if (warm) {
+            std::stable_sort(cs.begin(), cs.end(), canonicalLess);
+            std::vector<int> colorOf;
+            colors = colorContacts(bodies, cs, colorOf);
+            colorOrder(cs, colorOf, colors);
+            for (Contact& c : cs)
+                if (!bodies[c.a].isSensor() && !bodies[c.b].isSensor()) warmStartContact(bodies[c.a], bodies[c.b], c);
+        }
+        sweeps(bodies, cs, dt, o);
+        iters_ = 2 * o.velocityIterations;
+    }
+    bool advanceSubsteps(std::vector<Body>& bodies, std::vector<Contact>& cs, std::vector<Joint>& joints,
+                         const Vec3& g, float h, int S, const SolverOptions& o) override {
+        if (!joints.empty() || o.iterationPolicy == IterationPolicy::Adaptive) return false;
+        solveVelocities(bodies, cs, h, true, o);
+        for (int s = 0; s < S; ++s) {
+            if (s > 0) {
+                cpu_->integrate(bodies, g, h);
+                if (!cs.empty()) sweeps(bodies, cs, h, o);
+            }
+            for (Body& b : bodies)
+                if (!(b.isStatic() || b.isLocked() || isFullyAsleep(b.asleep))) b.advanceTransform(h);
+        }
+        iters_ = 2 * o.velocityIterations * S;
+        return true;
+    }
+    int colors = 0;
+
+private:
+    std::unique_ptr<Backend> cpu_;
+    uint32_t iters_ = 0;
+};
+
+// --------------------------------------------------------------- fixtures --
+struct State { float p[3], q[4], v[3], w[3]; };
+std::vector<State> states(const std::vector<Body>& bodies) {
+    std::vector<State> s(bodies.size());
+    for (size_t i = 0; i < bodies.size(); ++i) {
+        const Body& b = bodies[i];
+        s[i] = {{b.position.x, b.position.y, b.position.z}, {b.orientation.x, b.orientation.y, b.orientation.z, b.orientation.w},
+                {b.velocity.x, b.velocity.y, b.velocity.z}, {b.angularVelocity.x, b.angularVelocity.y, b.angularVelocity.z}};
+    }
+    return s;
+}
+
+void writeSoup(const std::string& dir, const MeshSoup& m) {
+    writeNpy(dir + "/soup_vertices.npy", "<f4", {m.vertices.size(), 3}, m.vertices.data(), m.vertices.size() * 12);
+    writeNpy(dir + "/soup_indices.npy", "<u4", {m.indices.size()}, m.indices.data(), m.indices.size() * 4);
+    writeRecords(dir + "/soup_meshes.npy", m.meshes);
+    writeRecords(dir + "/soup_nodes.npy", m.bvhNodes);
+    writeNpy(dir + "/soup_trirefs.npy", "<u4", {m.bvhTriRefs.size()}, m.bvhTriRefs.data(), m.bvhTriRefs.size() * 4);
+    writeNpy(dir + "/soup_hullpts.npy", "<f4", {m.hullPoints.size(), 3}, m.hullPoints.data(), m.hullPoints.size() * 12);
+    writeNpy(dir + "/soup_hullfaces.npy", "<u4", {m.hullFaceIndices.size()}, m.hullFaceIndices.data(), m.hullFaceIndices.size() * 4);
+    writeRecords(dir + "/soup_children.npy", m.compoundChildren);
+}
+
+void writeMeta(const std::string& dir, const World& w, float dt, const std::vector<int>& steps, const std::string& extra) {
+    FILE* f = std::fopen((dir + "/meta.json").c_str(), "w");
+    const SolverOptions& o = w.solverOptions_;
+    std::fprintf(f, "{\"dt\": %.9g, \"substeps\": %d, \"gravity\": [%.9g, %.9g, %.9g], \"velocityIterations\": %d, "
+                    "\"positionIterations\": %d, \"frictionModel\": %d, \"steps\": [",
+                 dt, w.substeps, w.gravity.x, w.gravity.y, w.gravity.z, o.velocityIterations, o.positionIterations,
+                 (int)o.frictionModel);
+    for (size_t i = 0; i < steps.size(); ++i) std::fprintf(f, "%s%d", i ? ", " : "", steps[i]);
+    std::fprintf(f, "]%s}\n", extra.c_str());
+    std::fclose(f);
+}
+
+// Reference World trajectory with CUDA-backend semantics: initial records,
+// soup, and body states after each listed step (plus the solved contacts of
+// the first listed step, in color order).
+void trajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene,
+                const std::vector<int>& steps, bool coneBlock = false) {
+    std::string dir = out + "/" + name;
+    mkdir(dir.c_str(), 0755);
+    World w(BackendType::Cpu);
+    build(w, scene);
+    SolverOptions o = w.solverOptions_;
+    if (coneBlock) o.frictionModel = FrictionModel::ConeBlockSolver;
+    w.setSolverOptions(o);
+    w.backend_.reset(new CudaSemantics());
+    writeRecords(dir + "/bodies_in.npy", w.bodies_);
+    writeSoup(dir, w.meshes_);
+    int done = 0;
+    bool wroteContacts = false;
+    std::string colorsJson = ", \"contacts_at\": [";
+    std::string kJson = ", \"colors\": [";
+    for (int target : steps) {
+        while (done < target) { w.step(scene.dt); ++done; }
+        writeRecords(dir + "/state_" + std::to_string(target) + ".npy", states(w.bodies_));
+        colorsJson += (target == steps[0] ? "" : ", ") + std::to_string(w.contacts_.size());
+        kJson += (target == steps[0] ? "" : ", ") + std::to_string(static_cast<CudaSemantics*>(w.backend_.get())->colors);
+        if (!wroteContacts) { writeRecords(dir + "/contacts_" + std::to_string(target) + ".npy", w.contacts_); wroteContacts = true; }
+        std::fprintf(stderr, "  %s step %d: %zu contacts\n", name.c_str(), target, w.contacts_.size());
+    }
+    writeRecords(dir + "/bodies_out.npy", w.bodies_);
+    writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]");
+}
+
+// Reference narrow phase on snapshots of a reference trajectory: records,
+// the exact pair set (sorted (lo<<32|hi)) and every pair's contacts from
+// collidePair in canonical order.
+void narrowSnapshots(const std::string& out, const std::string& name, const vxscene::Scene& scene,
+                     const std::vector<int>& snaps) {
+    World w(BackendType::Cpu);
+    build(w, scene);
+    w.backend_.reset(new CudaSemantics());
+    int done = 0;
+    for (int target : snaps) {
+        while (done < target) { w.step(scene.dt); ++done; }
+        std::string dir = out + "/" + name + "_" + std::to_string(target);
+        mkdir(dir.c_str(), 0755);
+        writeRecords(dir + "/bodies_in.npy", w.bodies_);
+        writeSoup(dir, w.meshes_);
+        std::vector<uint64_t> pairs;
+        exactPairs(w.bodies_, w.meshes_, scene.dt, pairs);
+        std::vector<Contact> cs;
+        pairContacts(w.bodies_, w.meshes_, scene.dt, pairs, cs);
+        writeNpy(dir + "/pairs.npy", "<u8", {pairs.size()}, pairs.data(), pairs.size() * 8);
+        writeRecords(dir + "/contacts.npy", cs);
+        std::vector<float> lohi(w.bodies_.size() * 6);
+        for (size_t i = 0; i < w.bodies_.size(); ++i) {
+            const Body& b = w.bodies_[i];
+            Vec3 lo{}, hi{};
+            if (b.shape == ShapeType::Plane) { lo = {-1e30f, -1e30f, -1e30f}; hi = {1e30f, 1e30f, 1e30f}; }
+            else if (b.shape == ShapeType::Mesh) { lo = w.meshes_.meshes[b.meshIndex].aabbMin; hi = w.meshes_.meshes[b.meshIndex].aabbMax; }
+            else bodyAabb(b, scene.dt, lo, hi);
+            float* o = &lohi[6 * i];
+            o[0] = lo.x; o[1] = lo.y; o[2] = lo.z; o[3] = hi.x; o[4] = hi.y; o[5] = hi.z;
+        }
+        writeNpy(dir + "/aabbs.npy", "<f4", {w.bodies_.size(), 6}, lohi.data(), lohi.size() * 4);
+        // Solver isolation: color, order, warm start and 2I sweeps with the
+        // reference rows on these exact contacts (impulses zero).
+        std::vector<Body> bodies = w.bodies_;
+        std::vector<int> colorOf;
+        int K = colorContacts(bodies, cs, colorOf);
+        writeNpy(dir + "/colors.npy", "<i4", {colorOf.size()}, colorOf.data(), colorOf.size() * 4);
+        std::vector<Contact> solved = cs;
+        colorOrder(solved, colorOf, K);
+        const float h = scene.dt / scene.substeps;
+        for (Contact& c : solved)
+            if (!bodies[c.a].isSensor() && !bodies[c.b].isSensor()) warmStartContact(bodies[c.a], bodies[c.b], c);
+        for (int it = 0; it < 2 * 8; ++it)
+            for (Contact& c : solved)
+                if (!bodies[c.a].isSensor() && !bodies[c.b].isSensor()) solveContact(bodies[c.a], bodies[c.b], c, h, SolverOptions{});
+        writeRecords(dir + "/solve_bodies.npy", states(bodies));
+        std::vector<std::array<float, 9>> imp(solved.size());
+        for (size_t k = 0; k < solved.size(); ++k) {
+            const Contact& c = solved[k];
+            imp[k] = {c.point.x, c.point.y, c.point.z, c.normalImpulse, c.tangentImpulse1, c.tangentImpulse2,
+                      c.rollingImpulse1, c.rollingImpulse2, c.spinningImpulse};
+        }
+        writeNpy(dir + "/solve_impulses.npy", "<f4", {imp.size(), 9}, imp.data(), imp.size() * 36);
+        writeMeta(dir, w, scene.dt, {target}, ", \"colors\": " + std::to_string(K));
+        std::fprintf(stderr, "  %s: %zu pairs, %zu contacts, %d colors\n", dir.c_str(), pairs.size(), cs.size(), K);
+    }
+}
+
+// Reference integration on edge-case bodies: one velocity integration and one
+// transform advance (CpuBackend::integrate, Body::advanceTransform).
+void integrationCases(const std::string& out) {
+    std::string dir = out + "/integrate";
+    mkdir(dir.c_str(), 0755);
+    vxscene::Scene s = vxscene::freeFlight(64, 4242u);
+    World w(BackendType::Cpu);
+    build(w, s);
+    vxscene::Rng r(99u);
+    for (size_t i = 0; i < w.bodies_.size(); ++i) {
+        Body& b = w.bodies_[i];
+        b.force = {r.range(-50, 50), r.range(-50, 50), r.range(-50, 50)};
+        b.torque = {r.range(-9, 9), r.range(-9, 9), r.range(-9, 9)};
+        b.linearDamping = (i % 3 == 0) ? r.range(0, 2) : 0.0f;
+        b.angularDamping = (i % 5 == 0) ? r.range(0, 2) : 0.0f;
+        b.gravityScale = (i % 7 == 0) ? r.range(-1, 2) : 1.0f;
+        if (i % 11 == 3) b.fixedRotation = 1;
+        if (i % 13 == 4) b.ccdTuning.quality = MotionQuality::Locked;
+        if (i % 17 == 5) b.motionType = MotionType::Kinematic;
+        if (i % 19 == 6) b.asleep = 1;
+        if (i % 23 == 7) b.angularVelocity = {400.0f, -250.0f, 300.0f};
+        if (i % 29 == 8) b.position = {1.0e7f, -2.0e7f, 3.0e6f};   // sub-ULP displacement guard
+        b.orientation = normalize(b.orientation);
+    }
+    writeRecords(dir + "/bodies_in.npy", w.bodies_);
+    std::vector<Body> bodies = w.bodies_;
+    const float h = 1.0f / 240.0f;
+    std::unique_ptr<Backend> cpu(createCpuBackend());
+    cpu->setWorkerCount(1);
+    cpu->integrate(bodies, {0, -9.81f, 0}, h);
+    writeRecords(dir + "/after_integrate.npy", bodies);
+    for (Body& b : bodies)
+        if (!b.isStatic() && !isFullyAsleep(b.asleep)) b.advanceTransform(h);
+    writeRecords(dir + "/after_transform.npy", bodies);
+    for (int k = 0; k < 99; ++k) {
+        cpu->integrate(bodies, {0, -9.81f, 0}, h);
+        for (Body& b : bodies)
+            if (!b.isStatic() && !isFullyAsleep(b.asleep)) b.advanceTransform(h);
+    }
+    writeRecords(dir + "/after_100.npy", bodies);
+}
+
+int fixtures(const std::string& out) {
+    mkdir(out.c_str(), 0755);
+    integrationCases(out);
+    std::fprintf(stderr, "integrate done\n");
+    trajectory(out, "freeflight", vxscene::freeFlight(128), {1, 10, 300});
+    vxscene::Scene small = vxscene::lattice(4, 4, 4);
+    trajectory(out, "lattice64", small, {1, 2, 10, 60, 120});
+    trajectory(out, "lattice64_cone", small, {1, 10, 60}, true);
+    trajectory(out, "pile", vxscene::pile(160, 5.0f, 6.0f), {1, 5, 30, 90});
+    trajectory(out, "terrain", vxscene::terrain(16, 0.5f, 0.8f, 40), {1, 10, 60});
+    narrowSnapshots(out, "nlattice", vxscene::lattice(), {0, 45});
+    narrowSnapshots(out, "npile", vxscene::pile(200, 5.0f, 6.0f, 707u), {40});
+    narrowSnapshots(out, "nterrain", vxscene::terrain(16, 0.5f, 0.8f, 40, 313u), {30});
+    return 0;
+}
+
+// ------------------------------------------------------------------ bench --
+vxscene::Scene sceneByName(const std::string& name) {
+    if (name == "debris50k") return vxscene::debris(50000);
+    if (name == "lattice1k") return vxscene::lattice();
+    if (name.rfind("debris", 0) == 0) return vxscene::debris((uint32_t)std::atoi(name.c_str() + 6));
+    std::fprintf(stderr, "unknown scene %s\n", name.c_str());
+    std::exit(2);
+}
+
+int bench(const std::string& name, int steps, int workers, int warmup) {
+    vxscene::Scene s = sceneByName(name);
+    World w(BackendType::Cpu);
+    build(w, s);
+    w.setWorkerCount((uint32_t)workers);
+    for (int k = 0; k < warmup; ++k) w.step(s.dt);
+    std::vector<double> per;
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < steps; ++k) {
+        auto a = std::chrono::steady_clock::now();
+        w.step(s.dt);
+        per.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count());
+    }
+    double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::sort(per.begin(), per.end());
+    double median = per.empty() ? 0.0 : per[per.size() / 2];
+    std::printf("{\"scene\": \"%s\", \"bodies\": %zu, \"steps\": %d, \"warmup\": %d, \"workers\": %u, "
+                "\"seconds\": %.6f, \"median_step_s\": %.6f, \"body_steps_per_s\": %.3f, \"contacts_last\": %zu}\n",
+                name.c_str(), w.bodies_.size(), steps, warmup, w.backend_->workerCount(), sec, median,
+                (double)w.bodies_.size() * steps / sec, w.contacts_.size());
+    return 0;
+}
+
+
+// Per-step body states and contacts of a reference trajectory (debugging aid
+// for locating the first step where a replay departs from the reference).
+int trace(const std::string& out, const std::string& which, int steps) {
+    vxscene::Scene s = which == "terrain" ? vxscene::terrain(16, 0.5f, 0.8f, 40)
+                     : which == "pile" ? vxscene::pile(160, 5.0f, 6.0f) : vxscene::lattice(4, 4, 4);
+    mkdir(out.c_str(), 0755);
+    World w(BackendType::Cpu);
+    build(w, s);
+    w.backend_.reset(new CudaSemantics());
+    writeRecords(out + "/bodies_in.npy", w.bodies_);
+    writeSoup(out, w.meshes_);
+    for (int k = 1; k <= steps; ++k) {
+        w.step(s.dt);
+        writeRecords(out + "/state_" + std::to_string(k) + ".npy", states(w.bodies_));
+        writeRecords(out + "/contacts_" + std::to_string(k) + ".npy", w.contacts_);
+    }
+    writeMeta(out, w, s.dt, {steps}, "");
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "trace" && argc > 4) return trace(argv[2], argv[3], std::atoi(argv[4]));
+    if (mode == "fixtures" && argc > 2) return fixtures(argv[2]);
+    if (mode == "bench" && argc > 4) return bench(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argc > 5 ? std::atoi(argv[5]) : 1);
+    std::fprintf(stderr, "usage: %s fixtures <outdir> | bench <scene> <steps> <workers> [warmup]\n", argv[0]);
+    return 2;
+}

@@ -1,0 +1,30 @@
+"""Regenerate the golden vectors in this folder from the reference Velox CPU
+library (strict floating point), built from /root/reference by oracle/ref.mk.
+
+    python tests/golden/make_golden.py
+
+Writes one folder per case: input records, flat geometry, and reference
+outputs (see oracle/ref_harness.cpp for what each case pins).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def main():
+    if not os.path.isdir("/root/reference"):
+        sys.exit("the reference sources are not present; fixtures cannot be regenerated here")
+    subprocess.check_call(["make", "-s", "-f", os.path.join(ROOT, "oracle", "ref.mk"), "harness", "-j8"], cwd=ROOT)
+    for name in os.listdir(HERE):
+        path = os.path.join(HERE, name)
+        if os.path.isdir(path):
+            shutil.rmtree(path)
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixtures", HERE], cwd=ROOT)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,265 @@
+// Deterministic synthetic scene descriptions for the device-step workloads.
+//
+// A description lists bodies by shape, size, mass and initial state; it holds
+// no solver records.  Two builders turn it into bodies: the product's
+// scene_build.cpp (vx_body records + flat geometry for the HIP step) and any
+// host that owns a Velox World (World::addBox/addSphere/... calls).
+//
+// Random numbers come from the 32-bit LCG s = 1664525 s + 1013904223 with
+// u = (s >> 8) / 2^24, the generator Velox's own fuzz scenes use
+// (examples/fuzz_demo.cpp:23-31).  Orientations are uniform (Shoemake's
+// subgroup algorithm); normals use Box-Muller.
+#pragma once
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+namespace vxscene {
+
+struct Rng {
+    uint32_t s;
+    explicit Rng(uint32_t seed) : s(seed ? seed : 1u) {}
+    float next() { s = s * 1664525u + 1013904223u; return (s >> 8) * (1.0f / 16777216.0f); }
+    float range(float lo, float hi) { return lo + (hi - lo) * next(); }
+    int pick(int n) { return (int)(next() * n) % n; }
+    float normal() {  // Box-Muller, one value per call
+        float u1 = next(), u2 = next();
+        if (u1 < 1e-7f) u1 = 1e-7f;
+        return sqrtf(-2.0f * logf(u1)) * cosf(6.2831853f * u2);
+    }
+    void quat(float q[4]) {  // Shoemake: uniform on SO(3)
+        float u1 = next(), u2 = next(), u3 = next();
+        float a = sqrtf(1.0f - u1), b = sqrtf(u1);
+        q[0] = a * sinf(6.2831853f * u2);
+        q[1] = a * cosf(6.2831853f * u2);
+        q[2] = b * sinf(6.2831853f * u3);
+        q[3] = b * cosf(6.2831853f * u3);
+    }
+};
+
+enum Kind : uint8_t { Sphere, Box, Capsule, Hull, Plane, Heightfield };
+
+struct BodySpec {
+    Kind kind = Sphere;
+    float mass = 1.0f;            // 0 => static
+    float radius = 0.5f;          // sphere / capsule
+    float half[3] = {0.5f, 0.5f, 0.5f};
+    float halfHeight = 0.5f;      // capsule
+    float pos[3] = {0, 0, 0};
+    float quat[4] = {0, 0, 0, 1};
+    float vel[3] = {0, 0, 0};
+    float angVel[3] = {0, 0, 0};
+    float friction = 0.5f;        // Body default (body.h:143)
+    float restitution = 0.3f;     // Body default (body.h:142)
+    float normal[3] = {0, 1, 0};  // plane
+    float offset = 0.0f;          // plane
+    uint32_t hullFirst = 0, hullCount = 0;  // into Scene::hullPoints (xyz triples)
+    uint32_t hfIndex = 0;         // into Scene::heightfields
+    uint32_t world = 0;           // batched scenes: owning world
+};
+
+struct HeightfieldSpec {
+    uint32_t width = 2, depth = 2;
+    float cell = 1.0f;
+    float origin[3] = {0, 0, 0};
+    std::vector<float> heights;
+};
+
+struct Scene {
+    std::vector<BodySpec> bodies;
+    std::vector<float> hullPoints;      // xyz triples
+    std::vector<HeightfieldSpec> heightfields;
+    float gravity[3] = {0, -9.81f, 0};
+    float dt = 1.0f / 60.0f;
+    int substeps = 4;
+    int velocityIterations = 8;
+    uint32_t worlds = 1;
+};
+
+inline BodySpec plane(float friction = 0.5f, float restitution = 0.3f) {
+    BodySpec b; b.kind = Plane; b.mass = 0.0f; b.friction = friction; b.restitution = restitution; return b;
+}
+
+// Volume of the convex hull of `n` points in general position (brute-force
+// facet enumeration; n is small).  Used only to turn a density into a mass.
+inline double hullVolume(const float* p, uint32_t n) {
+    double cx = 0, cy = 0, cz = 0;
+    for (uint32_t i = 0; i < n; ++i) { cx += p[3 * i]; cy += p[3 * i + 1]; cz += p[3 * i + 2]; }
+    cx /= n; cy /= n; cz /= n;
+    double vol = 0.0;
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t j = i + 1; j < n; ++j)
+            for (uint32_t k = j + 1; k < n; ++k) {
+                double ax = p[3 * i], ay = p[3 * i + 1], az = p[3 * i + 2];
+                double ux = p[3 * j] - ax, uy = p[3 * j + 1] - ay, uz = p[3 * j + 2] - az;
+                double vx = p[3 * k] - ax, vy = p[3 * k + 1] - ay, vz = p[3 * k + 2] - az;
+                double nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+                int pos = 0, neg = 0;
+                for (uint32_t m = 0; m < n && !(pos && neg); ++m) {
+                    if (m == i || m == j || m == k) continue;
+                    double d = nx * (p[3 * m] - ax) + ny * (p[3 * m + 1] - ay) + nz * (p[3 * m + 2] - az);
+                    if (d > 1e-12) ++pos; else if (d < -1e-12) ++neg;
+                }
+                if (pos && neg) continue;
+                double side = nx * (cx - ax) + ny * (cy - ay) + nz * (cz - az);
+                vol += fabs(side) / 6.0;  // tetra (centroid, facet)
+            }
+    return vol;
+}
+
+// Velox's demo scenes seed the same way; one seed per workload.
+constexpr uint32_t kSeedLattice = 1001u, kSeedDebris = 2002u, kSeedCcd = 3003u,
+                   kSeedBatch = 4004u, kSeedDiff = 5005u;
+
+// 1,000 boxes (half-extent 0.5, density 1000) in a 10x10x10 lattice at 1.1 m
+// spacing with U(-0.05,0.05) jitter and uniform orientations, on a static
+// plane; friction 0.6, restitution 0.
+inline Scene lattice(int nx = 10, int ny = 10, int nz = 10, uint32_t seed = kSeedLattice) {
+    Scene s;
+    Rng r(seed);
+    s.bodies.push_back(plane(0.6f, 0.0f));
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                BodySpec b;
+                b.kind = Box;
+                b.mass = 1000.0f;
+                b.pos[0] = 1.1f * i - 4.95f + r.range(-0.05f, 0.05f);
+                b.pos[1] = 1.1f * j + 1.0f + r.range(-0.05f, 0.05f);
+                b.pos[2] = 1.1f * k - 4.95f + r.range(-0.05f, 0.05f);
+                r.quat(b.quat);
+                b.friction = 0.6f;
+                b.restitution = 0.0f;
+                s.bodies.push_back(b);
+            }
+    return s;
+}
+
+// Mixed debris: 30% spheres r~U(.1,.5), 30% boxes h~U(.1,.5)^3, 20% capsules
+// r~U(.05,.2) hh~U(.1,.4), 20% hulls of 24 points on an ellipsoid with semi-axes
+// U(.2,.6); positions uniform in [-X/2,X/2]x[1,1+Y]x[-Z/2,Z/2], v~N(0,2),
+// density 1000, above one static plane.  The default is the 50,000-body case.
+inline Scene debris(uint32_t count = 50000, float X = 100.0f, float Y = 50.0f, float Z = 100.0f,
+                    uint32_t seed = kSeedDebris) {
+    Scene s;
+    Rng r(seed);
+    s.bodies.push_back(plane());
+    std::vector<uint8_t> kinds(count);
+    uint32_t nS = count * 3 / 10, nB = count * 3 / 10, nC = count / 5;
+    for (uint32_t i = 0; i < count; ++i)
+        kinds[i] = i < nS ? Sphere : i < nS + nB ? Box : i < nS + nB + nC ? Capsule : Hull;
+    for (uint32_t i = count; i > 1; --i) {  // Fisher-Yates
+        uint32_t j = (uint32_t)r.pick((int)i);
+        uint8_t t = kinds[i - 1]; kinds[i - 1] = kinds[j]; kinds[j] = t;
+    }
+    const float rho = 1000.0f, pi = 3.14159265f;
+    for (uint32_t i = 0; i < count; ++i) {
+        BodySpec b;
+        b.kind = (Kind)kinds[i];
+        switch (b.kind) {
+        case Sphere:
+            b.radius = r.range(0.1f, 0.5f);
+            b.mass = rho * (4.0f / 3.0f) * pi * b.radius * b.radius * b.radius;
+            break;
+        case Box:
+            for (int a = 0; a < 3; ++a) b.half[a] = r.range(0.1f, 0.5f);
+            b.mass = rho * 8.0f * b.half[0] * b.half[1] * b.half[2];
+            break;
+        case Capsule:
+            b.radius = r.range(0.05f, 0.2f);
+            b.halfHeight = r.range(0.1f, 0.4f);
+            b.mass = rho * (pi * b.radius * b.radius * 2.0f * b.halfHeight +
+                            (4.0f / 3.0f) * pi * b.radius * b.radius * b.radius);
+            break;
+        default: {
+            float ax[3] = {r.range(0.2f, 0.6f), r.range(0.2f, 0.6f), r.range(0.2f, 0.6f)};
+            b.hullFirst = (uint32_t)(s.hullPoints.size() / 3);
+            b.hullCount = 24;
+            for (int k = 0; k < 24; ++k) {
+                float d[3] = {r.normal(), r.normal(), r.normal()};
+                float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                if (len < 1e-6f) { d[0] = 1; d[1] = 0; d[2] = 0; len = 1; }
+                for (int a = 0; a < 3; ++a) s.hullPoints.push_back(ax[a] * d[a] / len);
+            }
+            b.mass = rho * (float)hullVolume(&s.hullPoints[3 * b.hullFirst], 24);
+            break;
+        }
+        }
+        b.pos[0] = r.range(-0.5f * X, 0.5f * X);
+        b.pos[1] = r.range(1.0f, 1.0f + Y);
+        b.pos[2] = r.range(-0.5f * Z, 0.5f * Z);
+        r.quat(b.quat);
+        for (int a = 0; a < 3; ++a) b.vel[a] = 2.0f * r.normal();
+        s.bodies.push_back(b);
+    }
+    return s;
+}
+
+// Contact-free flight: bodies far apart (no plane), spinning, for integration
+// parity over many steps.
+inline Scene freeFlight(uint32_t count = 128, uint32_t seed = 77u) {
+    Scene s;
+    Rng r(seed);
+    for (uint32_t i = 0; i < count; ++i) {
+        BodySpec b;
+        b.kind = (Kind)(i % 4);
+        b.radius = r.range(0.2f, 0.5f);
+        b.halfHeight = r.range(0.2f, 0.6f);
+        for (int a = 0; a < 3; ++a) b.half[a] = r.range(0.1f, 0.9f);
+        if (b.kind == Hull) {
+            b.hullFirst = (uint32_t)(s.hullPoints.size() / 3);
+            b.hullCount = 12;
+            for (int k = 0; k < 12; ++k) {
+                float d[3] = {r.normal(), r.normal(), r.normal()};
+                float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+                for (int a = 0; a < 3; ++a) s.hullPoints.push_back(b.half[a] * d[a] / len);
+            }
+        }
+        b.mass = r.range(0.5f, 20.0f);
+        b.pos[0] = 40.0f * (float)(i % 8);
+        b.pos[1] = 40.0f * (float)((i / 8) % 8) + 500.0f;
+        b.pos[2] = 40.0f * (float)(i / 64);
+        r.quat(b.quat);
+        for (int a = 0; a < 3; ++a) { b.vel[a] = r.range(-3, 3); b.angVel[a] = r.range(-6, 6); }
+        s.bodies.push_back(b);
+    }
+    return s;
+}
+
+// Dense mixed pile for parity tests: a few hundred bodies dropped into a
+// small volume so that manifolds, EPA and deep stacks all occur.
+inline Scene pile(uint32_t count = 240, float X = 6.0f, float Y = 8.0f, uint32_t seed = 606u) {
+    Scene s = debris(count, X, Y, X, seed);
+    return s;
+}
+
+// Static heightfield terrain (value-noise octaves) with mixed bodies above it.
+inline Scene terrain(uint32_t side = 24, float cell = 0.5f, float amp = 1.0f, uint32_t count = 96,
+                     uint32_t seed = 909u) {
+    Scene s;
+    Rng r(seed);
+    HeightfieldSpec hf;
+    hf.width = hf.depth = side;
+    hf.cell = cell;
+    hf.origin[0] = -0.5f * cell * (side - 1);
+    hf.origin[1] = 0.0f;
+    hf.origin[2] = -0.5f * cell * (side - 1);
+    hf.heights.resize((size_t)side * side);
+    for (uint32_t z = 0; z < side; ++z)
+        for (uint32_t x = 0; x < side; ++x)
+            hf.heights[(size_t)z * side + x] = amp * (0.5f * sinf(0.37f * x) * cosf(0.29f * z) + 0.25f * sinf(1.3f * x + 0.7f * z));
+    s.heightfields.push_back(hf);
+    BodySpec m; m.kind = Heightfield; m.mass = 0.0f; m.hfIndex = 0;
+    s.bodies.push_back(m);
+    float half = 0.5f * cell * (side - 1) - 1.0f;
+    Scene d = debris(count, 2.0f * half, 4.0f, 2.0f * half, seed + 1);
+    s.hullPoints = d.hullPoints;
+    for (size_t i = 1; i < d.bodies.size(); ++i) {
+        BodySpec b = d.bodies[i];
+        b.pos[1] += 1.5f * amp;
+        s.bodies.push_back(b);
+    }
+    return s;
+}
+
+}  // namespace vxscene
