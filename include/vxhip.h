@@ -1,0 +1,172 @@
+/*
+ * vxhip.h -- C ABI of libvxhip, the MI355X (gfx950) device step for Velox.
+ *
+ * The library plays the role of the reference's device backend
+ * (src/backend_cuda.cu behind velox::Backend, include/velox/backend.h:111-172)
+ * plus the host-side stages of World::stepImpl that surround it
+ * (src/world.cpp:3830-4397).  Records cross the boundary in Velox's own byte
+ * layout (include/vx_records.h), so a velox::Backend implementation can pass
+ * `std::vector<Body>::data()` and `std::vector<Contact>::data()` straight in.
+ *
+ * Conventions
+ *   - Every entry point returns a status code: VXHIP_OK, VXHIP_ERR_MEMORY
+ *     (maps to BackendFailure{MemoryAllocation}), VXHIP_ERR_DEVICE_LOST (maps to
+ *     BackendFailure{DeviceLost}), VXHIP_ERR_INVALID (bad argument or scene),
+ *     VXHIP_ERR_CAPACITY (VeloxCapacityExceeded), VXHIP_ERR_UNAVAILABLE (no
+ *     gfx950 device; the reference factory's nullptr, backend.h:174-176).
+ *     The classification follows backend_cuda.cu:24-40.  No C++ exception
+ *     crosses the ABI.  vxhip_last_error() describes the latest failure on
+ *     the calling thread.
+ *   - Host buffers are owned by the caller; device buffers by the context.
+ *   - A context is used from one thread at a time (velox_c.h:20-22).
+ *   - Setting VELOX_HIP_TEST_FAIL=oom or =device-loss makes the next stage
+ *     entry point fail with the matching code without touching the device
+ *     (the host-forced fault hook of backend_cuda.cu:69-78).
+ */
+#ifndef VXHIP_H
+#define VXHIP_H
+
+#include "vx_records.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    VXHIP_OK = 0,
+    VXHIP_ERR_MEMORY = 1,
+    VXHIP_ERR_DEVICE_LOST = 2,
+    VXHIP_ERR_INVALID = 3,
+    VXHIP_ERR_CAPACITY = 4,
+    VXHIP_ERR_UNAVAILABLE = 5
+};
+
+/* BackendType value this library adds to velox::BackendType and
+ * velox_BackendType (appended after Vulkan = 3; backend.h:85-91, velox_c.h:50-55). */
+#define VXHIP_BACKEND_TYPE 4
+
+typedef struct vxhip_ctx vxhip_ctx;
+
+typedef struct {
+    float gravity[3];          /* World::gravity (world.h:434) */
+    int32_t substeps;          /* World::substeps (world.h:438) */
+    vx_solver_options options; /* World::solverOptions() (solver.h:23-35) */
+} vxhip_config;
+
+typedef struct {
+    uint32_t bodies;
+    uint32_t pairs;            /* broad-phase candidate pairs */
+    uint32_t contacts;         /* narrow-phase contacts */
+    uint32_t colors;           /* greedy colors */
+    uint32_t ccdCandidates;    /* pairs examined by the conservative-advancement net */
+    uint32_t ccdRescued;       /* pairs rewound to their time of impact */
+    uint32_t velocityIterations; /* sweeps run in the last step (lastVelocityIterations) */
+    uint32_t oversize;         /* bodies on the brute-force broad-phase path */
+    float cellSize;            /* broad-phase grid cell edge */
+    uint32_t reserved[7];
+} vxhip_stats;
+
+/* ---------------------------------------------------------------- setup -- */
+
+/* Number of usable gfx950 devices.  Replaces createCudaBackend()'s device probe
+ * (backend_cuda.cu:352-356). */
+int vxhip_device_count(int* count);
+
+/* Last failure text for the calling thread (never NULL). */
+const char* vxhip_last_error(void);
+
+/* Creates a context on the current device.  Replaces createCudaBackend()
+ * (backend.h:175). */
+int vxhip_create(vxhip_ctx** out);
+
+/* Releases every device buffer; never fails (CudaBackend::~CudaBackend,
+ * backend_cuda.cu:358-388). */
+void vxhip_destroy(vxhip_ctx* ctx);
+
+/* ------------------------------------------- velox::Backend equivalents -- */
+
+/* Backend::integrate (backend.h:115): velocity integration of `bodies` in place. */
+int vxhip_integrate(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, const float gravity[3], float dt);
+
+/* Backend::findContacts (backend.h:124-126): device broad phase + narrow phase.
+ * `out` receives *count contacts in canonical order (pair key, then point);
+ * VXHIP_ERR_CAPACITY with *count set to the required size when cap is short. */
+int vxhip_find_contacts(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx_soup* soup, float dt,
+                        vx_contact* out, uint32_t cap, uint32_t* count);
+
+/* Backend::solveVelocities (backend.h:131-134): colored sweeps over `contacts`.
+ * With warmStart != 0 the contacts are put into color order (the array is
+ * permuted in place, as the reference's device backend does) and warm started. */
+int vxhip_solve_velocities(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m,
+                           float dt, int warmStart, const vx_solver_options* options);
+
+/* Backend::advanceSubsteps (backend.h:138-142): all substeps of one step on the
+ * device (warm start, 2*I colored sweeps per substep, integration between
+ * substeps, transforms).  *advanced = 0 when the request is declined
+ * (Adaptive iteration policy), mirroring backend_cuda.cu:714-715. */
+int vxhip_advance_substeps(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m,
+                           const float gravity[3], float h, int substeps, const vx_solver_options* options,
+                           int* advanced);
+
+/* Backend::fetchImpulses (backend.h:146): accumulated impulses of the last
+ * solve, in the order the contacts were left. */
+int vxhip_fetch_impulses(vxhip_ctx* ctx, vx_contact* contacts, uint32_t m);
+
+/* Backend::invalidateCaches (backend.h:148): drop device mirrors. */
+int vxhip_invalidate_caches(vxhip_ctx* ctx);
+
+/* Backend::lastVelocityIterations (backend.h:161). */
+uint32_t vxhip_last_velocity_iterations(const vxhip_ctx* ctx);
+
+/* ------------------------------------------ device-resident world step -- */
+
+/* Uploads a scene and makes it device-resident: bodies, flat geometry and
+ * step configuration.  Replaces the per-stage uploads of the reference's
+ * device backend (uploadBodies / uploadMeshes, backend_cuda.cu:790-798, 893-915). */
+int vxhip_world_load(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx_soup* soup,
+                     const vxhip_config* config);
+
+/* Advances the resident world `steps` times by dt: the device form of
+ * World::stepImpl's physics (world.cpp:3975-4397 without events, sleeping,
+ * joints or host callbacks): orientation renormalisation, integration, broad
+ * phase, narrow phase, warm-start matching, greedy coloring, substepped
+ * colored TGS solve, conservative-advancement net, positional correction,
+ * force clearing.  Asynchronous with respect to the host. */
+int vxhip_world_step(vxhip_ctx* ctx, float dt, uint32_t steps);
+
+/* Waits for queued device work. */
+int vxhip_world_sync(vxhip_ctx* ctx);
+
+/* Copies resident body records to the host (World::body / bodyState). */
+int vxhip_world_download_bodies(vxhip_ctx* ctx, vx_body* out, uint32_t n);
+
+/* Replaces resident body records (snapshot restore: invalidateCaches + upload). */
+int vxhip_world_upload_bodies(vxhip_ctx* ctx, const vx_body* in, uint32_t n);
+
+/* Contacts solved by the last step, in color order (World's contacts_). */
+int vxhip_world_download_contacts(vxhip_ctx* ctx, vx_contact* out, uint32_t cap, uint32_t* count);
+
+/* Counters of the last step (StepStats subset, world.h:175-208). */
+int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
+
+/* ------------------------------------------------------- stage probes -- */
+/* Single stages on caller data, for parity checks of one stage at a time. */
+
+/* Broad phase only: candidate pairs as sorted contact keys (a<<32|b, convex
+ * body first for plane/mesh pairs). */
+int vxhip_probe_pairs(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx_soup* soup, float dt,
+                      uint64_t* out, uint32_t cap, uint32_t* count);
+
+/* Greedy coloring of contacts in the given order (backend_cuda.cu:597-630). */
+int vxhip_probe_color(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx_contact* contacts, uint32_t m,
+                      uint32_t* colors, uint32_t* numColors);
+
+/* Warm-start matching of `cur` against `prev` (sorted by pair key), in place
+ * (world.cpp:4133-4184). */
+int vxhip_probe_warm_start(vxhip_ctx* ctx, vx_contact* cur, uint32_t m, const vx_contact* prev, uint32_t mp);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* VXHIP_H */

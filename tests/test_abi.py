@@ -1,0 +1,42 @@
+"""The C-ABI library loads and exports every entry point include/vxhip.h
+declares (no device calls: runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from velox_amd import device
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "vxhip.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|uint32_t|const char\*)\s+(vxhip_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_expected_surface():
+    names = declared_symbols()
+    assert len(names) >= 20
+    assert set(names) == set(device.SIGNATURES), set(names) ^ set(device.SIGNATURES)
+
+
+@pytest.mark.skipif(not os.path.exists(device.LIBRARY), reason="libvxhip.so not built")
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(device.LIBRARY)
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(device.LIBRARY), reason="libvxhip.so not built")
+def test_last_error_is_callable_without_device():
+    lib = device.library()
+    assert isinstance(lib.vxhip_last_error(), bytes)
+
+
+def test_record_layouts_match_header():
+    from velox_amd.records import BODY, CONTACT
+    assert BODY.itemsize == 320 and CONTACT.itemsize == 128
+    assert BODY.fields["shape"][1] == 232 and BODY.fields["compoundCount"][1] == 296
+    assert CONTACT.fields["spinningImpulse"][1] == 116

@@ -1,0 +1,225 @@
+// Conservative-advancement safety net on the device (world.cpp:4227-4357).
+//
+// The reference walks the unique contact pairs in key order on the host, and
+// a rescue rewinds bodies that later pairs read.  kCcdCheck evaluates every
+// pair in parallel against the end-of-substeps state; kCcdSequential then
+// replays the reference order exactly, revisiting a pair only when it was
+// flagged or one of its bodies was moved by an earlier rescue (any other
+// pair sees the same state the parallel check saw).
+#pragma once
+#include "vx_kernels.hpp"
+
+namespace vxk {
+
+struct Gap { float gap; V3 normal; };
+
+__device__ inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup& soup, float search) {
+    if (tb.shape == VX_SHAPE_PLANE) {
+        Convex c = makeConvex(ta, soup);
+        V3 pn = v3(tb.planeNormal);
+        V3 deep = c.support(-pn);
+        return {dot(pn, deep) - tb.planeOffset - c.radius, pn};
+    }
+    if (tb.shape == VX_SHAPE_MESH) {
+        Gap best{1e30f, {0, 1, 0}};
+        meshGapProbe1(ta, tb, soup, search, best.gap, best.normal);
+        return best;
+    }
+    GjkResult r = gjkDistance(makeConvex(ta, soup), makeConvex(tb, soup));
+    return {r.distance, r.normal};
+}
+
+// world.cpp:2916-2958 (compound children expanded without recursion).
+__device__ inline Gap gapAt(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0, V3 pb, Q4 qb, const Soup& soup,
+                            float search) {
+    vx_body ta = a0; ta.position = raw(pa); ta.orientation = raw(qa);
+    vx_body tb = b0; tb.position = raw(pb); tb.orientation = raw(qb);
+    if (ta.shape == VX_SHAPE_COMPOUND) {
+        Gap best{1e30f, {0, 1, 0}};
+        for (uint32_t i = 0; i < ta.compoundCount; ++i) {
+            vx_body ch = childBody(ta, soup.children[ta.compoundFirst + i]);
+            Gap g;
+            if (tb.shape == VX_SHAPE_COMPOUND) {
+                g = {1e30f, {0, 1, 0}};
+                for (uint32_t k = 0; k < tb.compoundCount; ++k) {
+                    vx_body cb = childBody(tb, soup.children[tb.compoundFirst + k]);
+                    Gap h = gapSimple(ch, cb, soup, search);
+                    if (h.gap < g.gap) g = h;
+                }
+            } else {
+                g = gapSimple(ch, tb, soup, search);
+            }
+            if (g.gap < best.gap) best = g;
+        }
+        return best;
+    }
+    if (tb.shape == VX_SHAPE_COMPOUND) {
+        Gap best{1e30f, {0, 1, 0}};
+        for (uint32_t i = 0; i < tb.compoundCount; ++i) {
+            vx_body ch = childBody(tb, soup.children[tb.compoundFirst + i]);
+            Gap g = gapSimple(ta, ch, soup, search);
+            if (g.gap < best.gap) best = g;
+        }
+        return best;
+    }
+    return gapSimple(ta, tb, soup, search);
+}
+
+__device__ inline Q4 orientationAt(const vx_body& body, Q4 start, V3 L, float elapsed) {
+    vx_body s = body;
+    s.orientation = raw(start);
+    s.angularVelocity = raw(invInertiaMul(s, L));
+    advanceOrientation(s, elapsed);
+    return q4(s.orientation);
+}
+
+__device__ inline void ccdBodies(const vx_body* bodies, uint64_t key, int nb, uint32_t& i, uint32_t& j) {
+    i = (uint32_t)(key >> nb);
+    j = (uint32_t)(key & ((1ull << nb) - 1));
+    if (!isDynamic(bodies[i]) && isDynamic(bodies[j])) { uint32_t t = i; i = j; j = t; }
+}
+
+__device__ inline bool ccdEligible(const vx_body& a, const vx_body& b) {
+    if (isSensor(a) || isSensor(b) || !isDynamic(a)) return false;
+    return !(!a.ccdTuning.enableContinuous || a.ccdTuning.quality == VX_QUALITY_LOW || isLocked(a) ||
+             maxPointSpeed(a) < a.ccdTuning.minVelocityForCCD);
+}
+
+__device__ inline float ccdSearch(const vx_body& a, float dt) {
+    return maxPointSpeed(a) * dt + a.radius + length(v3(a.halfExtents)) + a.capsuleHalfHeight + 0.1f;
+}
+
+// Flags pairs (that produced contacts) whose end state penetrates deeper than
+// 8 mm and records the first flagged index.
+__global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict__ bodies, Soup soup, float dt,
+                                                      const uint64_t* __restrict__ pairs,
+                                                      const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                      uint8_t* __restrict__ flag, uint32_t* __restrict__ flagged,
+                                                      uint32_t* __restrict__ firstFlagged) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= np) return;
+    uint8_t f = 0;
+    if (counts[p]) {
+        uint32_t i, j;
+        ccdBodies(bodies, pairs[p], nb, i, j);
+        const vx_body a = bodies[i], b = bodies[j];
+        if (ccdEligible(a, b)) {
+            Gap end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup,
+                            ccdSearch(a, dt));
+            if (end.gap < -8e-3f) f = 1;
+        }
+    }
+    flag[p] = f;
+    if (f) {
+        atomicAdd(flagged, 1u);
+        atomicMin(firstFlagged, p);
+    }
+}
+
+// The reference's per-pair rescue (world.cpp:4276-4355) on one lane.
+__device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const vx_quat* prevQ, uint32_t i, uint32_t j,
+                                 const Soup& soup, float dt) {
+    vx_body a = bodies[i], b = bodies[j];
+    if (!ccdEligible(a, b)) return false;
+    const float search = ccdSearch(a, dt);
+    Gap end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup, search);
+    if (end.gap >= -8e-3f) return false;
+    V3 pa0 = v3(prevPos[i]), da = v3(a.position) - pa0;
+    Q4 qa0 = q4(prevQ[i]);
+    V3 pb0 = isStatic(b) ? v3(b.position) : v3(prevPos[j]);
+    V3 db = v3(b.position) - pb0;
+    Q4 qb0 = isStatic(b) ? q4(b.orientation) : q4(prevQ[j]);
+    float bound = maxPointSpeed(a) * dt + (isStatic(b) ? 0.0f : maxPointSpeed(b) * dt);
+    if (bound < 1e-9f) return false;
+    V3 La = angularMomentum(a);
+    V3 Lb = isDynamic(b) ? angularMomentum(b) : V3{};
+    float t = 0.0f;
+    V3 n = end.normal;
+    for (int it = 0; it < 24 && t < 1.0f; ++it) {
+        Q4 qa = orientationAt(a, qa0, La, dt * t);
+        Q4 qb = isStatic(b) ? qb0 : orientationAt(b, qb0, Lb, dt * t);
+        Gap g = gapAt(a, pa0 + da * t, qa, b, pb0 + db * t, qb, soup, search);
+        n = g.normal;
+        if (g.gap < 1e-3f) break;
+        t = fmn(1.0f, t + g.gap / bound);
+    }
+    if (t >= 1.0f) return false;
+    a.position = raw(pa0 + da * t);
+    a.orientation = raw(qa0);
+    a.angularVelocity = raw(invInertiaMul(a, La));
+    advanceOrientation(a, dt * t);
+    if (isDynamic(b)) {
+        b.position = raw(pb0 + db * t);
+        b.orientation = raw(qb0);
+        b.angularVelocity = raw(invInertiaMul(b, Lb));
+        advanceOrientation(b, dt * t);
+    }
+    float vn = dot(v3(a.velocity) - v3(b.velocity), n);
+    float ims = solverInvMass(a) + solverInvMass(b);
+    if (vn < 0.0f && ims > 0.0f) {
+        float rest = vn < -1.0f ? combine(a.restitution, b.restitution, a.restitutionCombine, b.restitutionCombine) : 0.0f;
+        float imp = -(1.0f + rest) * vn / ims;
+        a.velocity = raw(v3(a.velocity) + n * (imp * solverInvMass(a)));
+        if (isDynamic(b)) b.velocity = raw(v3(b.velocity) - n * (imp * solverInvMass(b)));
+    }
+    float remaining = dt * (1.0f - t);
+    advanceTransform(a, remaining);
+    if (isDynamic(b)) advanceTransform(b, remaining);
+    bodies[i].position = a.position; bodies[i].orientation = a.orientation;
+    bodies[i].velocity = a.velocity; bodies[i].angularVelocity = a.angularVelocity;
+    if (isDynamic(b)) {
+        bodies[j].position = b.position; bodies[j].orientation = b.orientation;
+        bodies[j].velocity = b.velocity; bodies[j].angularVelocity = b.angularVelocity;
+    }
+    return true;
+}
+
+constexpr int kCcdThreads = 1024;
+
+__global__ __launch_bounds__(kCcdThreads) void kCcdSequential(vx_body* bodies, const vx_vec3* __restrict__ prevPos,
+                                                              const vx_quat* __restrict__ prevQ, Soup soup, float dt,
+                                                              const uint64_t* __restrict__ pairs,
+                                                              const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                              const uint8_t* __restrict__ flag,
+                                                              const uint32_t* __restrict__ flagged,
+                                                              const uint32_t* __restrict__ firstFlagged,
+                                                              uint8_t* dirty, uint32_t* __restrict__ examined,
+                                                              uint32_t* __restrict__ rescued) {
+    __shared__ uint32_t next;
+    if (*flagged == 0) return;
+    const uint32_t start = *firstFlagged;
+    uint32_t cursor = start;  // first pair still to consider
+    for (uint32_t base = start - start % kCcdThreads; base < np; base += kCcdThreads) {
+        for (;;) {
+            if (threadIdx.x == 0) next = 0xffffffffu;
+            __syncthreads();
+            uint32_t p = base + threadIdx.x;
+            if (p < np && p >= cursor && counts[p]) {
+                uint32_t i, j;
+                ccdBodies(bodies, pairs[p], nb, i, j);
+                if (flag[p] || __hip_atomic_load(&dirty[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
+                    __hip_atomic_load(&dirty[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                    atomicMin(&next, p);
+            }
+            __syncthreads();
+            const uint32_t q = next;
+            if (q == 0xffffffffu) break;
+            if (threadIdx.x == 0) {
+                uint32_t i, j;
+                ccdBodies(bodies, pairs[q], nb, i, j);
+                ++*examined;
+                if (ccdRescue(bodies, prevPos, prevQ, i, j, soup, dt)) {
+                    ++*rescued;
+                    dirty[i] = 1;
+                    dirty[j] = 1;
+                }
+            }
+            cursor = q + 1;
+            __threadfence_block();
+            __syncthreads();
+        }
+        cursor = base + kCcdThreads;
+    }
+}
+
+}  // namespace vxk

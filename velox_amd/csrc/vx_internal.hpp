@@ -1,0 +1,21 @@
+// Internal declarations shared by the HIP translation units of libvxhip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace vxhost {
+
+size_t radixTempBytes(uint32_t n);
+hipError_t radixSort(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t* valsTmp, uint32_t n, int bits,
+                     void* temp, hipStream_t s);
+size_t scanTempBytes(uint32_t n);
+hipError_t exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, void* temp, hipStream_t s);
+
+inline int bitsFor(uint64_t maxValue) {  // bits needed to represent 0..maxValue
+    int b = 0;
+    while (b < 64 && (maxValue >> b) != 0) ++b;
+    return b;
+}
+
+}  // namespace vxhost

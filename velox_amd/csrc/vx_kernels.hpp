@@ -1,0 +1,541 @@
+// Device kernels of the HIP device step (one body, pair or contact per lane).
+#pragma once
+#include "vxd_solver.hpp"
+
+namespace vxk {
+
+using namespace vxd;
+
+constexpr int kThreads = 256;
+constexpr uint32_t kNone = 0xffffffffu;
+constexpr int kMaxPairContacts = kMaxContactsPerPair;  // 12 (narrowphase.h:12)
+
+struct Grid {           // dense cell grid of the broad phase
+    float cell;         // cell edge
+    int minx, miny, minz;
+    uint32_t nx, ny, nz;
+    uint64_t ncells;    // also the key of oversize bodies
+};
+
+__device__ inline float4 f4(V3 v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
+__device__ inline V3 v3f(float4 v) { return {v.x, v.y, v.z}; }
+
+// ------------------------------------------------------------- bodies --
+// world.cpp:3975 (renormalise), solver.cpp:258-270 (first-substep velocity),
+// world.cpp:4186-4188 (pre-step pose for the CCD net).
+__global__ __launch_bounds__(kThreads) void kBeginStep(vx_body* __restrict__ bodies, uint32_t n, V3 g, float h,
+                                                       vx_vec3* __restrict__ prevPos, vx_quat* __restrict__ prevQ,
+                                                       int integrate) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    vx_body b = bodies[i];
+    b.orientation = raw(qnormalize(q4(b.orientation)));
+    if (integrate) integrateBody(b, g, h);
+    bodies[i].orientation = b.orientation;
+    bodies[i].velocity = b.velocity;
+    bodies[i].angularVelocity = b.angularVelocity;
+    if (prevPos) { prevPos[i] = b.position; prevQ[i] = b.orientation; }
+}
+
+__global__ __launch_bounds__(kThreads) void kIntegrate(vx_body* __restrict__ bodies, uint32_t n, V3 g, float h) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    vx_body b = bodies[i];
+    if (!isDynamic(b) || isLocked(b) || asleepFully(b)) return;
+    integrateBody(b, g, h);
+    bodies[i].velocity = b.velocity;
+    bodies[i].angularVelocity = b.angularVelocity;
+}
+
+// world.cpp:4219-4222 / backend_cuda.cu:91-97
+__global__ __launch_bounds__(kThreads) void kTransforms(vx_body* __restrict__ bodies, uint32_t n, float h) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    vx_body b = bodies[i];
+    if (isStatic(b) || isLocked(b) || asleepFully(b)) return;
+    advanceTransform(b, h);
+    bodies[i].position = b.position;
+    bodies[i].orientation = b.orientation;
+    bodies[i].angularVelocity = b.angularVelocity;
+}
+
+// world.cpp:4493-4496
+__global__ __launch_bounds__(kThreads) void kClearForces(vx_body* __restrict__ bodies, uint32_t n) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    bodies[i].force = {0, 0, 0};
+    bodies[i].torque = {0, 0, 0};
+}
+
+// ------------------------------------------------------------ broad phase --
+__device__ inline bool oversized(const vx_body& b, float extent, float cell) {
+    return extent > cell || b.shape == VX_SHAPE_PLANE || b.shape == VX_SHAPE_MESH;
+}
+
+// narrowphase.h:369-398; planes unbounded, meshes use their root box
+// (backend_cuda.cu:103-129).  The largest finite extent sets the cell edge.
+__global__ __launch_bounds__(kThreads) void kAabb(const vx_body* __restrict__ bodies, uint32_t n, Soup soup, float dt,
+                                                  float4* __restrict__ lo, float4* __restrict__ hi,
+                                                  uint32_t* __restrict__ extentBits) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    float ext = 0.0f;
+    if (i < n) {
+        const vx_body b = bodies[i];
+        V3 l, h;
+        if (b.shape == VX_SHAPE_PLANE) { l = {-1e30f, -1e30f, -1e30f}; h = {1e30f, 1e30f, 1e30f}; }
+        else if (b.shape == VX_SHAPE_MESH) {
+            const vx_mesh& m = soup.meshes[b.meshIndex];
+            l = v3(m.aabbMin); h = v3(m.aabbMax);
+        } else {
+            bodyAabb(b, dt, l, h);
+            float e = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
+            if (e < 1e6f) ext = e;
+        }
+        lo[i] = f4(l);
+        hi[i] = f4(h);
+    }
+    // One atomic per wavefront: positive floats order like their bit patterns.
+    for (int off = 32; off > 0; off >>= 1) ext = fmaxf(ext, __shfl_xor(ext, off));
+    if ((threadIdx.x & 63) == 0 && ext > 0.0f) atomicMax(extentBits, __float_as_uint(ext));
+}
+
+__device__ inline float gridCellOf(const uint32_t* extentBits) {
+    float c = __uint_as_float(*extentBits);
+    // The grid edge sits just above the largest extent so that overlapping
+    // boxes always land in neighbouring cells despite rounding.
+    return c > 0.0f ? c * 1.0001f : 1.0f;
+}
+
+__device__ inline void cellOf(float4 lo, float4 hi, float inv, int& cx, int& cy, int& cz) {
+    cx = (int)floorf((lo.x + hi.x) * 0.5f * inv);
+    cy = (int)floorf((lo.y + hi.y) * 0.5f * inv);
+    cz = (int)floorf((lo.z + hi.z) * 0.5f * inv);
+}
+
+__global__ __launch_bounds__(kThreads) void kCellBounds(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
+                                                        const float4* __restrict__ hi, uint32_t n,
+                                                        const uint32_t* __restrict__ extentBits, int* __restrict__ bounds,
+                                                        uint32_t* __restrict__ oversizeCount) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const float cell = gridCellOf(extentBits);
+    float4 l = lo[i], h = hi[i];
+    float ext = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
+    const uint8_t shape = bodies[i].shape;
+    if (ext > cell || shape == VX_SHAPE_PLANE || shape == VX_SHAPE_MESH) { atomicAdd(oversizeCount, 1u); return; }
+    int cx, cy, cz;
+    cellOf(l, h, 1.0f / cell, cx, cy, cz);
+    atomicMin(&bounds[0], cx); atomicMin(&bounds[1], cy); atomicMin(&bounds[2], cz);
+    atomicMax(&bounds[3], cx); atomicMax(&bounds[4], cy); atomicMax(&bounds[5], cz);
+}
+
+__global__ __launch_bounds__(kThreads) void kCellKeys(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
+                                                      const float4* __restrict__ hi, uint32_t n, Grid g,
+                                                      uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                                                      uint32_t* __restrict__ oversize, uint32_t* __restrict__ oversizeCount) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    idx[i] = i;
+    float4 l = lo[i], h = hi[i];
+    float ext = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
+    const uint8_t shape = bodies[i].shape;
+    if (ext > g.cell || shape == VX_SHAPE_PLANE || shape == VX_SHAPE_MESH) {
+        keys[i] = g.ncells;
+        uint32_t slot = atomicAdd(oversizeCount, 1u);
+        oversize[slot] = i;
+        return;
+    }
+    int cx, cy, cz;
+    cellOf(l, h, 1.0f / g.cell, cx, cy, cz);
+    keys[i] = ((uint64_t)(cx - g.minx) * g.ny + (uint64_t)(cy - g.miny)) * g.nz + (uint64_t)(cz - g.minz);
+}
+
+__device__ inline bool pairAllowed(const vx_body& a, const vx_body& b) {
+    if (inert(a) && inert(b)) return false;
+    return canCollide(a, b);
+}
+
+// Contact key of a pair: plane/mesh pairs name the convex body first
+// (narrowphase.h:665-671), packed as (first << nb) | second.
+__device__ inline uint64_t packPair(const vx_body* bodies, uint32_t i, uint32_t j, int nb) {
+    uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    uint8_t s = bodies[lo].shape;
+    if (s == VX_SHAPE_PLANE || s == VX_SHAPE_MESH) return ((uint64_t)hi << nb) | lo;
+    return ((uint64_t)lo << nb) | hi;
+}
+
+__device__ inline uint32_t lowerBound(const uint64_t* keys, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// Inclusive wavefront prefix sum of a per-lane count.
+__device__ inline uint32_t waveInclusive(uint32_t v) {
+    const int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t t = __shfl_up(v, off);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Reserves `count` slots for this lane with one atomic per wavefront.
+__device__ inline uint32_t waveReserve(uint32_t count, uint32_t* counter) {
+    uint32_t incl = waveInclusive(count);
+    uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 63 && total) base = atomicAdd(counter, total);
+    base = __shfl(base, 63);
+    return base + incl - count;
+}
+
+// 27-neighbourhood candidates of each non-oversize body in sorted order,
+// exact AABB test, inert/filter rules of backend_cuda.cu:174-218; each pair
+// found once (from its lower body index).  Two passes: count, then write.
+template <bool kWrite>
+__device__ inline uint32_t gridVisit(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
+                                     const float4* __restrict__ hi, const uint64_t* __restrict__ keys,
+                                     const uint32_t* __restrict__ idx, uint32_t n, const Grid& g, uint32_t s, int nb,
+                                     uint64_t* out, uint32_t at, uint32_t cap) {
+    const uint64_t key = keys[s];
+    const uint32_t i = idx[s];
+    const uint32_t z = (uint32_t)(key % g.nz);
+    const uint32_t y = (uint32_t)((key / g.nz) % g.ny);
+    const uint32_t x = (uint32_t)(key / ((uint64_t)g.nz * g.ny));
+    const float4 li = lo[i], hiI = hi[i];
+    const vx_body& a = bodies[i];
+    uint32_t count = 0;
+    for (int dx = -1; dx <= 1; ++dx) {
+        if ((dx < 0 && x == 0) || (uint32_t)(x + dx) >= g.nx) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            if ((dy < 0 && y == 0) || (uint32_t)(y + dy) >= g.ny) continue;
+            for (int dz = -1; dz <= 1; ++dz) {
+                if ((dz < 0 && z == 0) || (uint32_t)(z + dz) >= g.nz) continue;
+                const uint64_t nk = ((uint64_t)(x + dx) * g.ny + (y + dy)) * g.nz + (z + dz);
+                for (uint32_t t = lowerBound(keys, n, nk); t < n && keys[t] == nk; ++t) {
+                    const uint32_t j = idx[t];
+                    if (j <= i) continue;
+                    const float4 lj = lo[j], hj = hi[j];
+                    if (!(hiI.x >= lj.x && li.x <= hj.x && hiI.y >= lj.y && li.y <= hj.y && hiI.z >= lj.z && li.z <= hj.z))
+                        continue;
+                    if (!pairAllowed(a, bodies[j])) continue;
+                    if (kWrite && at + count < cap) out[at + count] = ((uint64_t)i << nb) | j;
+                    ++count;
+                }
+            }
+        }
+    }
+    return count;
+}
+
+__global__ __launch_bounds__(kThreads) void kGridPairs(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
+                                                       const float4* __restrict__ hi, const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ idx, uint32_t n, Grid g, int nb,
+                                                       uint64_t* __restrict__ pairs, uint32_t* __restrict__ pairCount,
+                                                       uint32_t cap) {
+    uint32_t s = blockIdx.x * kThreads + threadIdx.x;
+    const bool valid = s < n && keys[s] != g.ncells;
+    uint32_t cnt = valid ? gridVisit<false>(bodies, lo, hi, keys, idx, n, g, s, nb, nullptr, 0, 0) : 0u;
+    uint32_t at = waveReserve(cnt, pairCount);
+    if (valid && cnt) gridVisit<true>(bodies, lo, hi, keys, idx, n, g, s, nb, pairs, at, cap);
+}
+
+// Oversize bodies (planes, meshes, huge sweeps) against every body
+// (backend_cuda.cu:221-250).
+__global__ __launch_bounds__(kThreads) void kOversizePairs(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
+                                                           const float4* __restrict__ hi,
+                                                           const uint32_t* __restrict__ oversize, uint32_t oversizeCount,
+                                                           uint32_t n, Grid g, int nb, uint64_t* __restrict__ pairs,
+                                                           uint32_t* __restrict__ pairCount, uint32_t cap) {
+    uint64_t flat = (uint64_t)blockIdx.x * kThreads + threadIdx.x;
+    bool hit = false;
+    uint64_t key = 0;
+    if (flat < (uint64_t)oversizeCount * n) {
+        uint32_t o = (uint32_t)(flat / n), j = (uint32_t)(flat % n);
+        uint32_t i = oversize[o];
+        if (j != i) {
+            float4 lj = lo[j], hj = hi[j];
+            float extJ = fmaxf(hj.x - lj.x, fmaxf(hj.y - lj.y, hj.z - lj.z));
+            bool jOver = oversized(bodies[j], extJ, g.cell);
+            float4 li = lo[i], hiI = hi[i];
+            if (!(jOver && j < i) &&
+                hiI.x >= lj.x && li.x <= hj.x && hiI.y >= lj.y && li.y <= hj.y && hiI.z >= lj.z && li.z <= hj.z &&
+                pairAllowed(bodies[i], bodies[j])) {
+                hit = true;
+                key = packPair(bodies, i, j, nb);
+            }
+        }
+    }
+    uint32_t at = waveReserve(hit ? 1u : 0u, pairCount);
+    if (hit && at < cap) pairs[at] = key;
+}
+
+// ----------------------------------------------------------- narrow phase --
+__device__ inline bool canonicalLess(const vx_contact& x, const vx_contact& y) {  // backend_cuda.cu:587-595
+    if (x.point.x != y.point.x) return x.point.x < y.point.x;
+    if (x.point.y != y.point.y) return x.point.y < y.point.y;
+    return x.point.z < y.point.z;
+}
+
+// One pair per lane: collidePair, then the pair's contacts in point order
+// (stable, so equal points keep emission order) into the pair's 12 slots.
+__global__ __launch_bounds__(kThreads) void kNarrow(const vx_body* __restrict__ bodies, Soup soup, float dt,
+                                                    const uint64_t* __restrict__ pairs, uint32_t np, int nb,
+                                                    vx_contact* __restrict__ slots, uint32_t* __restrict__ counts) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= np) return;
+    uint64_t key = pairs[p];
+    uint32_t ca = (uint32_t)(key >> nb), cb = (uint32_t)(key & ((1ull << nb) - 1));
+    uint32_t i = ca < cb ? ca : cb, j = ca < cb ? cb : ca;
+    const vx_body A = bodies[i], B = bodies[j];
+    vx_contact buf[kMaxPairContacts];
+    int cnt = collidePair(A, B, i, j, soup, dt, buf, kMaxPairContacts);
+    for (int k = 1; k < cnt; ++k) {
+        vx_contact c = buf[k];
+        int m = k - 1;
+        while (m >= 0 && canonicalLess(c, buf[m])) { buf[m + 1] = buf[m]; --m; }
+        buf[m + 1] = c;
+    }
+    vx_contact* dst = slots + (size_t)p * kMaxPairContacts;
+    for (int k = 0; k < cnt; ++k) dst[k] = buf[k];
+    counts[p] = (uint32_t)cnt;
+}
+
+__global__ __launch_bounds__(kThreads) void kCompact(const vx_contact* __restrict__ slots, const uint32_t* __restrict__ counts,
+                                                     const uint32_t* __restrict__ offsets, uint32_t np,
+                                                     vx_contact* __restrict__ out) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= np) return;
+    uint32_t c = counts[p], at = offsets[p];
+    for (uint32_t k = 0; k < c; ++k) out[at + k] = slots[(size_t)p * kMaxPairContacts + k];
+}
+
+// -------------------------------------------------------- warm starting --
+// world.cpp:4111-4128; order-dependent, so one lane walks the contacts.  Only
+// launched when a body is asleep.
+__global__ void kWakePass(vx_body* bodies, const vx_contact* cs, uint32_t m) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    for (uint32_t k = 0; k < m; ++k) {
+        vx_body& a = bodies[cs[k].a];
+        vx_body& b = bodies[cs[k].b];
+        if (isSensor(a) || isSensor(b)) continue;
+        bool impact = cs[k].vn0 < -0.1f;
+        bool movingA = lengthSq(v3(a.velocity)) + lengthSq(v3(a.angularVelocity)) > 1e-3f;
+        bool movingB = lengthSq(v3(b.velocity)) + lengthSq(v3(b.angularVelocity)) > 1e-3f;
+        if (asleepFully(a) && !isStatic(b) && !asleepFully(b) && (impact || movingB)) { a.asleep = 0; a.sleepTimer = 0.0f; }
+        if (asleepFully(b) && !isStatic(a) && !asleepFully(a) && (impact || movingA)) { b.asleep = 0; b.sleepTimer = 0.0f; }
+    }
+}
+
+__device__ inline uint64_t pairKeyOf(const vx_contact& c) { return (uint64_t)c.a << 32 | c.b; }
+
+__device__ inline void copyImpulses(vx_contact& c, const vx_contact& p) {
+    c.normalImpulse = p.normalImpulse; c.tangentImpulse1 = p.tangentImpulse1; c.tangentImpulse2 = p.tangentImpulse2;
+    c.rollingImpulse1 = p.rollingImpulse1; c.rollingImpulse2 = p.rollingImpulse2; c.spinningImpulse = p.spinningImpulse;
+}
+
+// world.cpp:4133-4184: feature-key match, vn0 carry, nearest point within 5 cm.
+__global__ __launch_bounds__(kThreads) void kWarmMatch(vx_contact* __restrict__ cur, uint32_t m,
+                                                       const vx_contact* __restrict__ prev, uint32_t mp) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= m || mp == 0) return;
+    vx_contact c = cur[k];
+    const uint64_t key = pairKeyOf(c);
+    uint32_t lo = 0, hi = mp;
+    while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (pairKeyOf(prev[mid]) < key) lo = mid + 1; else hi = mid; }
+    bool matched = false;
+    if (c.featureKey != 0)
+        for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t)
+            if (prev[t].featureKey == c.featureKey) {
+                copyImpulses(c, prev[t]);
+                if (c.vn0 < -1.0f && prev[t].vn0 < c.vn0) c.vn0 = prev[t].vn0;
+                matched = true;
+                break;
+            }
+    if (!matched) {
+        if (c.vn0 < -1.0f)
+            for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t)
+                if (prev[t].vn0 < c.vn0) c.vn0 = prev[t].vn0;
+        float best = 2.5e-3f;
+        for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t) {
+            float d = lengthSq(v3(prev[t].point) - v3(c.point));
+            if (d < best) { best = d; copyImpulses(c, prev[t]); }
+        }
+    }
+    cur[k] = c;
+}
+
+// --------------------------------------------------------------- coloring --
+__device__ inline bool constrainedSide(const vx_body* bodies, const vx_contact& c, int side) {
+    bool sensor = isSensor(bodies[c.a]) || isSensor(bodies[c.b]);
+    return !sensor && isDynamic(bodies[side ? c.b : c.a]);
+}
+
+// One (body << cb | contact) entry per constrained contact side.
+__global__ __launch_bounds__(kThreads) void kColorEntries(const vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,
+                                                          uint32_t m, int cb, uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ vals, uint32_t* __restrict__ counter) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    bool s0 = false, s1 = false;
+    vx_contact c;
+    if (k < m) {
+        c = cs[k];
+        s0 = constrainedSide(bodies, c, 0);
+        s1 = constrainedSide(bodies, c, 1);
+    }
+    uint32_t at = waveReserve((s0 ? 1u : 0u) + (s1 ? 1u : 0u), counter);
+    if (s0) { keys[at] = ((uint64_t)c.a << cb) | k; vals[at] = k << 1; ++at; }
+    if (s1) { keys[at] = ((uint64_t)c.b << cb) | k; vals[at] = (k << 1) | 1u; }
+}
+
+// For each constrained side, the previous contact side on the same body.
+__global__ __launch_bounds__(kThreads) void kColorPreds(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                        uint32_t e, int cb, uint32_t* __restrict__ pred) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= e) return;
+    uint32_t side = vals[t];
+    uint32_t p = kNone;
+    if (t > 0 && (keys[t - 1] >> cb) == (keys[t] >> cb)) p = vals[t - 1];
+    pred[side] = p;
+}
+
+constexpr int kColorThreads = 1024;
+
+// Greedy first-free-bit coloring (backend_cuda.cu:597-630) in contact order,
+// reproduced exactly by dataflow inside one workgroup: a contact's colour
+// depends only on the masks its two bodies carry after their previous
+// contacts, so each lane waits for those two predecessors and nothing else.
+// All waves share one CU, so workgroup-scope release/acquire orders the
+// hand-offs.  The lowest unfinished contact is always ready, hence progress;
+// a spin budget still bounds every wait.
+__global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __restrict__ bodies,
+                                                              const vx_contact* __restrict__ cs, uint32_t m,
+                                                              const uint32_t* __restrict__ pred,
+                                                              unsigned long long* __restrict__ maskAfter,
+                                                              int* __restrict__ nextAfter, uint32_t* color,
+                                                              uint32_t* __restrict__ colorCount,
+                                                              uint32_t* __restrict__ numColors, uint32_t* __restrict__ error) {
+    uint32_t k = threadIdx.x;
+    uint32_t spins = 0;
+    while (k < m) {
+        const uint32_t pa = pred[2 * k], pb = pred[2 * k + 1];
+        const bool readyA = pa == kNone || __hip_atomic_load(&color[pa >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        const bool readyB = pb == kNone || __hip_atomic_load(&color[pb >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        if (readyA && readyB) {
+            const vx_contact& c = cs[k];
+            const bool sa = !constrainedSide(bodies, c, 0), sb = !constrainedSide(bodies, c, 1);
+            unsigned long long maskA = 0, maskB = 0;
+            int nextA = 64, nextB = 64;
+            if (!sa && pa != kNone) { maskA = maskAfter[pa]; nextA = nextAfter[pa]; }
+            if (!sb && pb != kNone) { maskB = maskAfter[pb]; nextB = nextAfter[pb]; }
+            const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
+            int col;
+            if (~used) {
+                col = __ffsll((long long)~used) - 1;
+            } else {
+                int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
+                col = na > nb2 ? na : nb2;
+                if (!sa) nextA = col + 1;
+                if (!sb) nextB = col + 1;
+            }
+            if (col < 64) {
+                if (!sa) maskA |= 1ull << col;
+                if (!sb) maskB |= 1ull << col;
+            }
+            maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
+            maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
+            atomicAdd(&colorCount[col], 1u);
+            atomicMax(numColors, (uint32_t)col + 1u);
+            __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            k += kColorThreads;
+            spins = 0;
+        } else if (++spins > (1u << 26)) {
+            atomicOr(error, 1u);
+            break;
+        }
+    }
+}
+
+// ----------------------------------------------------------------- solver --
+__global__ __launch_bounds__(kThreads) void kGather(const vx_contact* __restrict__ in, const uint32_t* __restrict__ order,
+                                                    uint32_t m, vx_contact* __restrict__ out) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m) out[k] = in[order[k]];
+}
+
+__global__ __launch_bounds__(kThreads) void kScatterBack(const vx_contact* __restrict__ in, const uint32_t* __restrict__ order,
+                                                         uint32_t m, vx_contact* __restrict__ out) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m) out[order[k]] = in[k];
+}
+
+// Writes the solved bodies of one contact back: only the rows the solver
+// changes (velocities; warm start also refreshes nothing else).
+__device__ inline void storeVel(vx_body* bodies, uint32_t i, const vx_body& b) {
+    bodies[i].velocity = b.velocity;
+    bodies[i].angularVelocity = b.angularVelocity;
+}
+
+// One colour: contacts touch disjoint dynamic bodies (backend_cuda.cu:301-311).
+__global__ __launch_bounds__(kThreads) void kWarmStartColor(vx_body* __restrict__ bodies, vx_contact* __restrict__ cs,
+                                                            const uint32_t* __restrict__ colorStart, uint32_t color) {
+    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    vx_contact c = cs[first + k];
+    vx_body a = bodies[c.a], b = bodies[c.b];
+    if (isSensor(a) || isSensor(b)) return;
+    warmStart(a, b, c);
+    if (isDynamic(a)) storeVel(bodies, c.a, a);
+    if (isDynamic(b)) storeVel(bodies, c.b, b);
+    cs[first + k].point = c.point;
+}
+
+__global__ __launch_bounds__(kThreads) void kSolveColor(vx_body* __restrict__ bodies, vx_contact* __restrict__ cs,
+                                                        const uint32_t* __restrict__ colorStart, uint32_t color, float h,
+                                                        vx_solver_options opt) {
+    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    vx_contact c = cs[first + k];
+    vx_body a = bodies[c.a], b = bodies[c.b];
+    if (isSensor(a) || isSensor(b)) return;
+    solveContact(a, b, c, h, opt);
+    if (isDynamic(a)) storeVel(bodies, c.a, a);
+    if (isDynamic(b)) storeVel(bodies, c.b, b);
+    cs[first + k] = c;
+}
+
+// world.cpp:4364-4397, one colour at a time.
+__global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,
+                                                           const uint32_t* __restrict__ colorStart, uint32_t color, float h,
+                                                           vx_solver_options opt) {
+    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    const vx_contact c = cs[first + k];
+    vx_body a = bodies[c.a], b = bodies[c.b];
+    if (isSensor(a) || isSensor(b)) return;
+    float ims = solverInvMass(a) + solverInvMass(b);
+    if (ims <= 0.0f) return;
+    float g = liveGap(a, b, c);
+    if (g >= -1e-3f) return;
+    float resolve = 0.6f;
+    if (opt.enableStackStabilization) {
+        V3 rv = v3(a.velocity) - v3(b.velocity);
+        if (lengthSq(rv) < 1e-4f) {
+            const float stiff = opt.stackStiffness / 1000.0f;
+            const float damp = 1.0f + opt.stackDamping * h;
+            resolve += 0.2f * stiff / damp;
+        }
+    }
+    float push = -resolve * (g + 1e-3f) / ims;
+    if (isDynamic(a) && !asleepFully(a)) bodies[c.a].position = raw(v3(a.position) + v3(c.normal) * (push * solverInvMass(a)));
+    if (isDynamic(b) && !asleepFully(b)) bodies[c.b].position = raw(v3(b.position) - v3(c.normal) * (push * solverInvMass(b)));
+}
+
+}  // namespace vxk

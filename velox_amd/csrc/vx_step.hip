@@ -1,0 +1,773 @@
+// Host orchestration and C ABI of libvxhip: the device step of Velox on one
+// MI355X (gfx950), device-resident across steps.  See include/vxhip.h.
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "vx_ccd.hpp"
+#include "vx_internal.hpp"
+#include "../../include/vxhip.h"
+
+using namespace vxk;
+
+namespace {
+
+thread_local std::string gLastError = "no error";
+
+struct Failure {
+    int code;
+};
+
+int classify(hipError_t e) {  // backend_cuda.cu:24-40
+    switch (e) {
+    case hipErrorOutOfMemory: return VXHIP_ERR_MEMORY;
+    case hipErrorNoDevice:
+    case hipErrorLaunchFailure:
+    case hipErrorLaunchTimeOut:
+    case hipErrorUnknown:
+    case hipErrorNotInitialized:
+    case hipErrorIllegalAddress: return VXHIP_ERR_DEVICE_LOST;
+    default: return VXHIP_ERR_INVALID;
+    }
+}
+
+[[noreturn]] void fail(int code, const std::string& what) {
+    gLastError = what;
+    throw Failure{code};
+}
+
+void check(hipError_t e, const char* expr, int line) {
+    if (e == hipSuccess) return;
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "vxhip: %s (%s) at vx_step.hip:%d", hipGetErrorString(e), expr, line);
+    fail(classify(e), buf);
+}
+#define HIPCK(x) check((x), #x, __LINE__)
+
+// Host-forced fault hook (backend_cuda.cu:69-78).
+void injectFailureForTesting() {
+    const char* mode = std::getenv("VELOX_HIP_TEST_FAIL");
+    if (!mode || !mode[0]) return;
+    if (!std::strcmp(mode, "oom")) fail(VXHIP_ERR_MEMORY, "vxhip test injection: allocation failure");
+    if (!std::strcmp(mode, "device-loss")) fail(VXHIP_ERR_DEVICE_LOST, "vxhip test injection: device loss");
+}
+
+template <class T>
+struct Buf {
+    T* p = nullptr;
+    size_t cap = 0;
+    void reserve(size_t n) {
+        if (n <= cap) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        size_t want = n + n / 4 + 64;
+        HIPCK(hipMalloc((void**)&p, want * sizeof(T)));
+        cap = want;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+inline uint32_t blocks(uint64_t n, int t = kThreads) { return (uint32_t)((n + t - 1) / t); }
+
+}  // namespace
+
+struct vxhip_ctx {
+    hipStream_t stream = nullptr;
+    // resident scene
+    uint32_t n = 0;
+    Buf<vx_body> bodies;
+    Buf<vx_vec3> vertices, hullPts;
+    Buf<uint32_t> indices, triRefs, hullFaces;
+    Buf<vx_mesh> meshes;
+    Buf<vx_bvh_node> nodes;
+    Buf<vx_compound_child> children;
+    Soup soup;
+    size_t soupSig[8] = {};
+    vxhip_config cfg{};
+    bool anyAsleep = false;
+    // per-step scratch
+    Buf<vx_vec3> prevPos;
+    Buf<vx_quat> prevQ;
+    Buf<float4> lo, hi;
+    Buf<uint64_t> keys, keysTmp;
+    Buf<uint32_t> idx, idxTmp, oversize;
+    Buf<uint64_t> pairs;
+    Buf<uint32_t> pairVals, counts, offsets;
+    Buf<vx_contact> slots, contacts, solve, prev;
+    Buf<uint32_t> colorVals, colorVals2, pred, color, colorCount, order, colorStart;
+    Buf<unsigned long long> maskAfter;
+    Buf<int> nextAfter;
+    Buf<uint8_t> ccdFlag, dirty;
+    Buf<uint8_t> temp;
+    Buf<uint32_t> counters;  // device counters, see Counter
+    uint32_t* hostCounters = nullptr;
+    uint32_t pairCap = 0;
+    uint32_t pairsCount = 0;
+    uint32_t contactCount = 0;
+    uint32_t prevCount = 0;
+    uint32_t solveCount = 0;
+    uint32_t numColors = 0;
+    std::vector<uint32_t> colorStartHost;
+    vxhip_stats stats{};
+    uint32_t lastIterations = 0;
+    bool loaded = false;
+};
+
+namespace {
+
+enum Counter : uint32_t {
+    cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
+    cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
+    cOversize2 = 17, cCount = 24
+};
+
+void* tempFor(vxhip_ctx* c, size_t bytes) {
+    c->temp.reserve(bytes);
+    return c->temp.p;
+}
+
+void readCounters(vxhip_ctx* c) {
+    HIPCK(hipMemcpyAsync(c->hostCounters, c->counters.p, cCount * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+}
+
+__global__ void kResetCounters(uint32_t* k) {
+    uint32_t t = threadIdx.x;
+    if (t >= cCount) return;
+    uint32_t v = 0;
+    if (t >= cBounds && t < cBounds + 3) v = 0x7fffffffu;       // min cell
+    else if (t >= cBounds + 3 && t < cBounds + 6) v = 0x80000000u;  // max cell
+    else if (t == cCcdFirst) v = 0xffffffffu;
+    k[t] = v;
+}
+
+__global__ void kFill8(uint8_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+__global__ void kFill32(uint32_t* p, uint32_t n, uint32_t v) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void kIota(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < n) p[i] = i;
+}
+__global__ void kColorKeys(const uint32_t* color, uint32_t m, uint64_t* keys, uint32_t* vals) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m) { keys[k] = color[k]; vals[k] = k; }
+}
+
+void fill32(vxhip_ctx* c, uint32_t* p, uint32_t n, uint32_t v) {
+    if (n) kFill32<<<blocks(n), kThreads, 0, c->stream>>>(p, n, v);
+}
+
+template <class T>
+void upload(vxhip_ctx* c, Buf<T>& b, const T* host, uint32_t count) {
+    b.reserve(count ? count : 1);
+    if (count) HIPCK(hipMemcpyAsync(b.p, host, (size_t)count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+}
+
+void uploadSoup(vxhip_ctx* c, const vx_soup* s) {
+    vx_soup e{};
+    if (!s) s = &e;
+    upload(c, c->vertices, s->vertices, s->vertexCount);
+    upload(c, c->indices, s->indices, s->indexCount);
+    upload(c, c->meshes, s->meshes, s->meshCount);
+    upload(c, c->nodes, s->bvhNodes, s->bvhNodeCount);
+    upload(c, c->triRefs, s->bvhTriRefs, s->bvhTriRefCount);
+    upload(c, c->hullPts, s->hullPoints, s->hullPointCount);
+    upload(c, c->hullFaces, s->hullFaceIndices, s->hullFaceIndexCount);
+    upload(c, c->children, s->compoundChildren, s->compoundChildCount);
+    c->soup.vertices = c->vertices.p; c->soup.indices = c->indices.p; c->soup.meshes = c->meshes.p;
+    c->soup.nodes = c->nodes.p; c->soup.triRefs = c->triRefs.p; c->soup.hullPoints = c->hullPts.p;
+    c->soup.hullFaceIndices = c->hullFaces.p; c->soup.children = c->children.p;
+}
+
+void validateScene(const vx_body* b, uint32_t n, const vx_soup* s) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const vx_body& x = b[i];
+        if (x.shape > VX_SHAPE_ELLIPSOID) fail(VXHIP_ERR_INVALID, "vxhip: body has an unknown shape");
+        if (x.shape == VX_SHAPE_MESH && (!s || x.meshIndex >= s->meshCount))
+            fail(VXHIP_ERR_INVALID, "vxhip: mesh body references an invalid mesh");
+        if (x.shape == VX_SHAPE_HULL && (!s || x.hullCount < 4 || (uint64_t)x.hullFirst + x.hullCount > s->hullPointCount ||
+                                         (uint64_t)x.hullFaceFirst + 3ull * x.hullFaceCount > s->hullFaceIndexCount))
+            fail(VXHIP_ERR_INVALID, "vxhip: hull body references invalid point storage");
+        if (x.shape == VX_SHAPE_COMPOUND &&
+            (!s || x.compoundCount == 0 || (uint64_t)x.compoundFirst + x.compoundCount > s->compoundChildCount))
+            fail(VXHIP_ERR_INVALID, "vxhip: compound body references invalid child storage");
+    }
+}
+
+void setBodies(vxhip_ctx* c, const vx_body* b, uint32_t n) {
+    c->n = n;
+    upload(c, c->bodies, b, n);
+    c->anyAsleep = false;
+    for (uint32_t i = 0; i < n; ++i) c->anyAsleep |= b[i].asleep == 1;
+}
+
+int bodyBits(uint32_t n) { return vxhost::bitsFor(n ? n - 1 : 0) ? vxhost::bitsFor(n ? n - 1 : 0) : 1; }
+
+// --------------------------------------------------------------- stages --
+// Broad phase: fills c->pairs (sorted packed contact keys), returns count.
+uint32_t broadPhase(vxhip_ctx* c, float dt) {
+    const uint32_t n = c->n;
+    hipStream_t s = c->stream;
+    c->lo.reserve(n); c->hi.reserve(n);
+    c->keys.reserve(n); c->keysTmp.reserve(n); c->idx.reserve(n); c->idxTmp.reserve(n); c->oversize.reserve(n);
+    uint32_t* k = c->counters.p;
+    kResetCounters<<<1, 64, 0, s>>>(k);
+    kAabb<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, dt, c->lo.p, c->hi.p, k + cExtent);
+    kCellBounds<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, k + cExtent, (int*)(k + cBounds),
+                                               k + cOversize2);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    const uint32_t* h = c->hostCounters;
+    float ext;
+    std::memcpy(&ext, &h[cExtent], 4);
+    Grid g{};
+    g.cell = ext > 0.0f ? ext * 1.0001f : 1.0f;
+    const int* bnd = (const int*)(h + cBounds);
+    const uint32_t oversizeCount = h[cOversize2];
+    if (oversizeCount < n) {
+        g.minx = bnd[0]; g.miny = bnd[1]; g.minz = bnd[2];
+        g.nx = (uint32_t)((int64_t)bnd[3] - bnd[0] + 1);
+        g.ny = (uint32_t)((int64_t)bnd[4] - bnd[1] + 1);
+        g.nz = (uint32_t)((int64_t)bnd[5] - bnd[2] + 1);
+    } else {
+        g.nx = g.ny = g.nz = 1;
+    }
+    g.ncells = (uint64_t)g.nx * g.ny * g.nz;
+    c->stats.cellSize = g.cell;
+    c->stats.oversize = oversizeCount;
+    kCellKeys<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, g, c->keys.p, c->idx.p, c->oversize.p,
+                                             k + cOversize);
+    HIPCK(vxhost::radixSort(c->keys.p, c->idx.p, c->keysTmp.p, c->idxTmp.p, n, vxhost::bitsFor(g.ncells),
+                            tempFor(c, vxhost::radixTempBytes(n)), s));
+    const int nb = bodyBits(n);
+    if (c->pairCap == 0) c->pairCap = 32 * n + 1024;
+    for (;;) {
+        c->pairs.reserve(c->pairCap);
+        kResetCounters<<<1, 64, 0, s>>>(k);  // only cPairs matters here; others re-derived
+        kGridPairs<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->keys.p, c->idx.p, n, g, nb,
+                                                  c->pairs.p, k + cPairs, c->pairCap);
+        if (oversizeCount)
+            kOversizePairs<<<blocks((uint64_t)oversizeCount * n), kThreads, 0, s>>>(
+                c->bodies.p, c->lo.p, c->hi.p, c->oversize.p, oversizeCount, n, g, nb, c->pairs.p, k + cPairs, c->pairCap);
+        HIPCK(hipGetLastError());
+        readCounters(c);
+        uint32_t np = c->hostCounters[cPairs];
+        if (np <= c->pairCap) {
+            c->pairsCount = np;
+            break;
+        }
+        c->pairCap = np + np / 4;
+    }
+    const uint32_t np = c->pairsCount;
+    c->pairVals.reserve(np ? np : 1);
+    c->keysTmp.reserve(np ? np : 1);
+    c->idxTmp.reserve(np ? np : 1);
+    HIPCK(vxhost::radixSort(c->pairs.p, c->pairVals.p, c->keysTmp.p, c->idxTmp.p, np, 2 * nb,
+                            tempFor(c, vxhost::radixTempBytes(np)), s));
+    return np;
+}
+
+
+// Narrow phase over the sorted pairs: contacts in canonical order into
+// c->contacts, returns the count.
+uint32_t narrowPhase(vxhip_ctx* c, uint32_t np, float dt) {
+    hipStream_t s = c->stream;
+    const int nb = bodyBits(c->n);
+    c->slots.reserve((size_t)(np ? np : 1) * kMaxPairContacts);
+    c->counts.reserve(np + 1);
+    c->offsets.reserve(np + 1);
+    if (np) {
+        kNarrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, np, nb, c->slots.p, c->counts.p);
+        HIPCK(hipGetLastError());
+    }
+    HIPCK(vxhost::exclusiveScan(c->counts.p, c->offsets.p, np, c->counters.p + cContacts,
+                                tempFor(c, vxhost::scanTempBytes(np)), s));
+    readCounters(c);
+    const uint32_t m = c->hostCounters[cContacts];
+    c->contacts.reserve(m ? m : 1);
+    if (np) kCompact<<<blocks(np), kThreads, 0, s>>>(c->slots.p, c->counts.p, c->offsets.p, np, c->contacts.p);
+    HIPCK(hipGetLastError());
+    c->contactCount = m;
+    return m;
+}
+
+// Greedy coloring of c->contacts[0..m) in order; colours into c->color, returns K.
+uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
+    hipStream_t s = c->stream;
+    uint32_t* k = c->counters.p;
+    c->color.reserve(m ? m : 1);
+    c->colorCount.reserve(m + 65);
+    if (m == 0) return 0;
+    const int cb = vxhost::bitsFor(m - 1) ? vxhost::bitsFor(m - 1) : 1;
+    const int nb = bodyBits(c->n);
+    c->keys.reserve(2 * (size_t)m); c->keysTmp.reserve(2 * (size_t)m);
+    c->colorVals.reserve(2 * (size_t)m); c->colorVals2.reserve(2 * (size_t)m);
+    c->pred.reserve(2 * (size_t)m);
+    c->maskAfter.reserve(2 * (size_t)m);
+    c->nextAfter.reserve(2 * (size_t)m);
+    kResetCounters<<<1, 64, 0, s>>>(k);
+    fill32(c, c->pred.p, 2 * m, kNone);
+    fill32(c, c->color.p, m, kNone);
+    fill32(c, c->colorCount.p, m + 65, 0);
+    kColorEntries<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, cs, m, cb, c->keys.p, c->colorVals.p, k + cEntries);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    const uint32_t e = c->hostCounters[cEntries];
+    HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + cb,
+                            tempFor(c, vxhost::radixTempBytes(e)), s));
+    if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, cb, c->pred.p);
+    kGreedyColor<<<1, kColorThreads, 0, s>>>(c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p, c->color.p,
+                                             c->colorCount.p, k + cColors, k + cColorError);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: coloring did not converge");
+    return c->hostCounters[cColors];
+}
+
+// Stable bucket of the coloured contacts by colour (backend_cuda.cu:634-640):
+// c->order[i] = canonical index of the i-th contact in colour order; colour
+// ranges into c->colorStart (device and host).
+void colorOrder(vxhip_ctx* c, uint32_t m, uint32_t K) {
+    hipStream_t s = c->stream;
+    c->order.reserve(m ? m : 1);
+    c->colorStart.reserve(K + 2);
+    if (m == 0) { c->colorStartHost.assign(1, 0); return; }
+    c->keys.reserve(m); c->keysTmp.reserve(m); c->colorVals2.reserve(m);
+    kColorKeys<<<blocks(m), kThreads, 0, s>>>(c->color.p, m, c->keys.p, c->order.p);
+    HIPCK(vxhost::radixSort(c->keys.p, c->order.p, c->keysTmp.p, c->colorVals2.p, m,
+                            vxhost::bitsFor(K ? K - 1 : 0), tempFor(c, vxhost::radixTempBytes(m)), s));
+    HIPCK(vxhost::exclusiveScan(c->colorCount.p, c->colorStart.p, K, c->colorStart.p + K,
+                                tempFor(c, vxhost::scanTempBytes(K)), s));
+    c->colorStartHost.resize(K + 1);
+    HIPCK(hipMemcpyAsync(c->colorStartHost.data(), c->colorStart.p, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+}
+
+uint32_t maxColorSize(const std::vector<uint32_t>& start) {
+    uint32_t mx = 0;
+    for (size_t i = 0; i + 1 < start.size(); ++i) mx = std::max(mx, start[i + 1] - start[i]);
+    return mx;
+}
+
+void warmStartColors(vxhip_ctx* c, vx_contact* solve, uint32_t K) {
+    for (uint32_t col = 0; col < K; ++col) {
+        uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
+        if (cnt) kWarmStartColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col);
+    }
+    HIPCK(hipGetLastError());
+}
+
+void solveSweeps(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx_solver_options& opt) {
+    const int sweeps = 2 * opt.velocityIterations;
+    for (int it = 0; it < sweeps; ++it)
+        for (uint32_t col = 0; col < K; ++col) {
+            uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
+            if (cnt) kSolveColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+        }
+    HIPCK(hipGetLastError());
+}
+
+void positionCorrection(vxhip_ctx* c, const vx_contact* solve, uint32_t K, float h, const vx_solver_options& opt) {
+    for (int it = 0; it < opt.positionIterations; ++it)
+        for (uint32_t col = 0; col < K; ++col) {
+            uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
+            if (cnt) kPositionColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+        }
+    HIPCK(hipGetLastError());
+}
+
+void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
+    if (np == 0) return;
+    hipStream_t s = c->stream;
+    uint32_t* k = c->counters.p;
+    const int nb = bodyBits(c->n);
+    c->ccdFlag.reserve(np);
+    c->dirty.reserve(c->n);
+    kFill8<<<blocks(c->n), kThreads, 0, s>>>(c->dirty.p, c->n);
+    kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb, c->ccdFlag.p,
+                                              k + cCcdFlagged, k + cCcdFirst);
+    kCcdSequential<<<1, kCcdThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->counts.p,
+                                             np, nb, c->ccdFlag.p, k + cCcdFlagged, k + cCcdFirst, c->dirty.p,
+                                             k + cCcdExamined, k + cCcdRescued);
+    HIPCK(hipGetLastError());
+}
+
+// One full device step of the resident world.
+void worldStep(vxhip_ctx* c, float dt) {
+    injectFailureForTesting();
+    const vxhip_config& cfg = c->cfg;
+    const int S = cfg.substeps > 0 ? cfg.substeps : 1;
+    const float h = dt / S;
+    const V3 g{cfg.gravity[0], cfg.gravity[1], cfg.gravity[2]};
+    const uint32_t n = c->n;
+    hipStream_t s = c->stream;
+    c->prevPos.reserve(n); c->prevQ.reserve(n);
+    uint32_t* k = c->counters.p;
+    kResetCounters<<<1, 64, 0, s>>>(k);
+    kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
+    HIPCK(hipGetLastError());
+    const uint32_t np = broadPhase(c, dt);
+    const uint32_t m = narrowPhase(c, np, dt);
+    if (c->anyAsleep && m) kWakePass<<<1, 64, 0, s>>>(c->bodies.p, c->contacts.p, m);
+    if (m && c->prevCount) kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
+    const uint32_t K = colorContacts(c, c->contacts.p, m);
+    colorOrder(c, m, K);
+    c->solve.reserve(m ? m : 1);
+    if (m) kGather<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, c->order.p, m, c->solve.p);
+    for (int st = 0; st < S; ++st) {
+        if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
+        if (m) {
+            if (st == 0) warmStartColors(c, c->solve.p, K);
+            solveSweeps(c, c->solve.p, K, h, cfg.options);
+        }
+        kTransforms<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, h);
+    }
+    HIPCK(hipGetLastError());
+    kResetCounters<<<1, 64, 0, s>>>(k);
+    ccdNet(c, np, dt);
+    if (m) positionCorrection(c, c->solve.p, K, h, cfg.options);
+    kClearForces<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n);
+    c->prev.reserve(m ? m : 1);
+    if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p);
+    HIPCK(hipGetLastError());
+    c->prevCount = m;
+    c->solveCount = m;
+    c->numColors = K;
+    c->lastIterations = (uint32_t)(2 * cfg.options.velocityIterations * S);
+    readCounters(c);
+    c->stats.bodies = n;
+    c->stats.pairs = np;
+    c->stats.contacts = m;
+    c->stats.colors = K;
+    c->stats.ccdCandidates = c->hostCounters[cCcdExamined];
+    c->stats.ccdRescued = c->hostCounters[cCcdRescued];
+    c->stats.velocityIterations = c->lastIterations;
+}
+
+template <class F>
+int guarded(F&& f) {
+    try {
+        f();
+        return VXHIP_OK;
+    } catch (const Failure& e) {
+        return e.code;
+    } catch (const std::exception& e) {
+        gLastError = e.what();
+        return VXHIP_ERR_INVALID;
+    }
+}
+
+vxhip_ctx* newContext() {
+    vxhip_ctx* c = new vxhip_ctx();
+    HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->counters.reserve(cCount);
+    HIPCK(hipHostMalloc((void**)&c->hostCounters, cCount * sizeof(uint32_t), hipHostMallocDefault));
+    return c;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+const char* vxhip_last_error(void) { return gLastError.c_str(); }
+
+int vxhip_device_count(int* count) {
+    return guarded([&] {
+        if (!count) fail(VXHIP_ERR_INVALID, "vxhip: null count");
+        *count = 0;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) return;
+        for (int d = 0; d < n; ++d) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, d) == hipSuccess && std::strncmp(prop.gcnArchName, "gfx950", 6) == 0)
+                ++*count;
+        }
+    });
+}
+
+int vxhip_create(vxhip_ctx** out) {
+    return guarded([&] {
+        if (!out) fail(VXHIP_ERR_INVALID, "vxhip: null output");
+        *out = nullptr;
+        int count = 0;
+        vxhip_device_count(&count);
+        if (count == 0) fail(VXHIP_ERR_UNAVAILABLE, "vxhip: no gfx950 device");
+        *out = newContext();
+    });
+}
+
+void vxhip_destroy(vxhip_ctx* c) {
+    if (!c) return;
+    hipError_t e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) std::fprintf(stderr, "vxhip cleanup: %s\n", hipGetErrorString(e));
+    c->bodies.release(); c->vertices.release(); c->hullPts.release(); c->indices.release(); c->triRefs.release();
+    c->hullFaces.release(); c->meshes.release(); c->nodes.release(); c->children.release(); c->prevPos.release();
+    c->prevQ.release(); c->lo.release(); c->hi.release(); c->keys.release(); c->keysTmp.release(); c->idx.release();
+    c->idxTmp.release(); c->oversize.release(); c->pairs.release(); c->pairVals.release(); c->counts.release();
+    c->offsets.release(); c->slots.release(); c->contacts.release(); c->solve.release(); c->prev.release();
+    c->colorVals.release(); c->colorVals2.release(); c->pred.release(); c->color.release(); c->colorCount.release();
+    c->order.release(); c->colorStart.release(); c->maskAfter.release(); c->nextAfter.release(); c->ccdFlag.release();
+    c->dirty.release(); c->temp.release(); c->counters.release();
+    if (c->hostCounters) (void)hipHostFree(c->hostCounters);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int vxhip_world_load(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_soup* soup, const vxhip_config* cfg) {
+    return guarded([&] {
+        if (!c || (!bodies && n) || !cfg) fail(VXHIP_ERR_INVALID, "vxhip: invalid world arguments");
+        validateScene(bodies, n, soup);
+        c->cfg = *cfg;
+        uploadSoup(c, soup);
+        setBodies(c, bodies, n);
+        c->prevCount = 0;
+        c->solveCount = 0;
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->loaded = true;
+    });
+}
+
+int vxhip_world_step(vxhip_ctx* c, float dt, uint32_t steps) {
+    return guarded([&] {
+        if (!c || !c->loaded) fail(VXHIP_ERR_INVALID, "vxhip: no world loaded");
+        if (!(dt >= 0.0f) || dt > 1e30f) fail(VXHIP_ERR_INVALID, "vxhip: timestep must be finite and non-negative");
+        if (dt == 0.0f) return;
+        for (uint32_t i = 0; i < steps; ++i) worldStep(c, dt);
+    });
+}
+
+int vxhip_world_sync(vxhip_ctx* c) {
+    return guarded([&] { HIPCK(hipStreamSynchronize(c->stream)); });
+}
+
+int vxhip_world_download_bodies(vxhip_ctx* c, vx_body* out, uint32_t n) {
+    return guarded([&] {
+        if (!c || n != c->n) fail(VXHIP_ERR_INVALID, "vxhip: body count mismatch");
+        if (n) HIPCK(hipMemcpyAsync(out, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_world_upload_bodies(vxhip_ctx* c, const vx_body* in, uint32_t n) {
+    return guarded([&] {
+        if (!c || n != c->n) fail(VXHIP_ERR_INVALID, "vxhip: body count mismatch");
+        setBodies(c, in, n);
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_world_download_contacts(vxhip_ctx* c, vx_contact* out, uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        if (!c || !count) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *count = c->solveCount;
+        if (c->solveCount > cap) fail(VXHIP_ERR_CAPACITY, "vxhip: contact buffer too small");
+        if (c->solveCount)
+            HIPCK(hipMemcpyAsync(out, c->solve.p, (size_t)c->solveCount * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_world_stats(vxhip_ctx* c, vxhip_stats* out) {
+    return guarded([&] {
+        if (!c || !out) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *out = c->stats;
+    });
+}
+
+uint32_t vxhip_last_velocity_iterations(const vxhip_ctx* c) { return c ? c->lastIterations : 0; }
+
+int vxhip_invalidate_caches(vxhip_ctx* c) {
+    return guarded([&] {
+        if (!c) fail(VXHIP_ERR_INVALID, "vxhip: null context");
+        c->solveCount = 0;
+        c->prevCount = 0;
+        for (size_t& s : c->soupSig) s = (size_t)-1;
+    });
+}
+
+
+// --------------------------------------------------- Backend equivalents --
+// These take host records every call, like the reference's device backend
+// (backend_cuda.cu:393-768); they share the context's device buffers with the
+// resident world (calling them replaces the resident bodies).
+
+int vxhip_integrate(vxhip_ctx* c, vx_body* bodies, uint32_t n, const float gravity[3], float dt) {
+    return guarded([&] {
+        injectFailureForTesting();
+        if (!c || (!bodies && n) || !gravity) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (n == 0) return;
+        setBodies(c, bodies, n);
+        kIntegrate<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, V3{gravity[0], gravity[1], gravity[2]}, dt);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+static void unpackPairs(uint64_t* keys, uint32_t np, int nb) {
+    for (uint32_t p = 0; p < np; ++p) {
+        uint64_t k = keys[p];
+        keys[p] = ((k >> nb) << 32) | (k & ((1ull << nb) - 1));
+    }
+}
+
+int vxhip_probe_pairs(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_soup* soup, float dt, uint64_t* out,
+                      uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        if (!c || !count || (!bodies && n)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        validateScene(bodies, n, soup);
+        *count = 0;
+        if (n < 2) return;
+        uploadSoup(c, soup);
+        setBodies(c, bodies, n);
+        const uint32_t np = broadPhase(c, dt);
+        *count = np;
+        if (np > cap) fail(VXHIP_ERR_CAPACITY, "vxhip: pair buffer too small");
+        if (np) HIPCK(hipMemcpyAsync(out, c->pairs.p, (size_t)np * sizeof(uint64_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        unpackPairs(out, np, bodyBits(n));
+    });
+}
+
+int vxhip_find_contacts(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_soup* soup, float dt,
+                        vx_contact* out, uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        injectFailureForTesting();
+        if (!c || !count || (!bodies && n)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        validateScene(bodies, n, soup);
+        *count = 0;
+        if (n < 2) return;
+        uploadSoup(c, soup);
+        setBodies(c, bodies, n);
+        const uint32_t np = broadPhase(c, dt);
+        const uint32_t m = narrowPhase(c, np, dt);
+        *count = m;
+        if (m > cap) fail(VXHIP_ERR_CAPACITY, "vxhip: contact buffer too small");
+        if (m) HIPCK(hipMemcpyAsync(out, c->contacts.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_probe_color(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_contact* contacts, uint32_t m,
+                      uint32_t* colors, uint32_t* numColors) {
+    return guarded([&] {
+        if (!c || !numColors || (!bodies && n) || (!contacts && m)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        setBodies(c, bodies, n);
+        c->contacts.reserve(m ? m : 1);
+        if (m) HIPCK(hipMemcpyAsync(c->contacts.p, contacts, (size_t)m * sizeof(vx_contact), hipMemcpyHostToDevice, c->stream));
+        const uint32_t K = colorContacts(c, c->contacts.p, m);
+        *numColors = K;
+        if (m) HIPCK(hipMemcpyAsync(colors, c->color.p, (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_probe_warm_start(vxhip_ctx* c, vx_contact* cur, uint32_t m, const vx_contact* prev, uint32_t mp) {
+    return guarded([&] {
+        if (!c || (!cur && m) || (!prev && mp)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (m == 0) return;
+        c->contacts.reserve(m);
+        c->prev.reserve(mp ? mp : 1);
+        HIPCK(hipMemcpyAsync(c->contacts.p, cur, (size_t)m * sizeof(vx_contact), hipMemcpyHostToDevice, c->stream));
+        if (mp) HIPCK(hipMemcpyAsync(c->prev.p, prev, (size_t)mp * sizeof(vx_contact), hipMemcpyHostToDevice, c->stream));
+        if (mp) kWarmMatch<<<blocks(m), kThreads, 0, c->stream>>>(c->contacts.p, m, c->prev.p, mp);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(cur, c->contacts.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->prevCount = 0;
+    });
+}
+
+static bool hostCanonicalLess(const vx_contact& x, const vx_contact& y) {  // backend_cuda.cu:587-595
+    uint64_t kx = (uint64_t)x.a << 32 | x.b, ky = (uint64_t)y.a << 32 | y.b;
+    if (kx != ky) return kx < ky;
+    if (x.point.x != y.point.x) return x.point.x < y.point.x;
+    if (x.point.y != y.point.y) return x.point.y < y.point.y;
+    return x.point.z < y.point.z;
+}
+
+// First-substep work of solveVelocities: canonical order, colour, colour
+// order, warm start.  Leaves the colour-ordered contacts in c->solve.
+static void prepareSolve(vxhip_ctx* c, vx_contact* contacts, uint32_t m) {
+    std::stable_sort(contacts, contacts + m, hostCanonicalLess);
+    c->contacts.reserve(m);
+    HIPCK(hipMemcpyAsync(c->contacts.p, contacts, (size_t)m * sizeof(vx_contact), hipMemcpyHostToDevice, c->stream));
+    const uint32_t K = colorContacts(c, c->contacts.p, m);
+    colorOrder(c, m, K);
+    c->solve.reserve(m);
+    kGather<<<blocks(m), kThreads, 0, c->stream>>>(c->contacts.p, c->order.p, m, c->solve.p);
+    c->numColors = K;
+    c->solveCount = m;
+    warmStartColors(c, c->solve.p, K);
+}
+
+int vxhip_solve_velocities(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m, float dt,
+                           int warmStart, const vx_solver_options* options) {
+    return guarded([&] {
+        injectFailureForTesting();
+        if (!c || !options || (!bodies && n) || (!contacts && m)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        c->lastIterations = 0;
+        if (m == 0) return;
+        setBodies(c, bodies, n);
+        if (warmStart) prepareSolve(c, contacts, m);
+        else if (c->solveCount != m) fail(VXHIP_ERR_INVALID, "vxhip: solve without a warm-started contact set");
+        solveSweeps(c, c->solve.p, c->numColors, dt, *options);
+        c->lastIterations = (uint32_t)(2 * options->velocityIterations);
+        HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
+        if (warmStart)
+            HIPCK(hipMemcpyAsync(contacts, c->solve.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_advance_substeps(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m,
+                           const float gravity[3], float h, int substeps, const vx_solver_options* options,
+                           int* advanced) {
+    return guarded([&] {
+        injectFailureForTesting();
+        if (!c || !options || !advanced || !gravity || (!bodies && n) || (!contacts && m))
+            fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *advanced = 0;
+        if (options->iterationPolicy == VX_ITERATION_ADAPTIVE) return;  // backend_cuda.cu:714-715
+        setBodies(c, bodies, n);
+        const V3 g{gravity[0], gravity[1], gravity[2]};
+        if (m) prepareSolve(c, contacts, m);
+        else c->solveCount = 0;
+        const int S = substeps > 0 ? substeps : 1;
+        for (int s = 0; s < S; ++s) {
+            if (s > 0) kIntegrate<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, g, h);
+            if (m) solveSweeps(c, c->solve.p, c->numColors, h, *options);
+            kTransforms<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, h);
+        }
+        HIPCK(hipGetLastError());
+        c->lastIterations = (uint32_t)(2 * options->velocityIterations * S);
+        HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
+        if (m) HIPCK(hipMemcpyAsync(contacts, c->solve.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        *advanced = 1;
+    });
+}
+
+int vxhip_fetch_impulses(vxhip_ctx* c, vx_contact* contacts, uint32_t m) {
+    return guarded([&] {
+        if (!c || (!contacts && m)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (c->solveCount == 0 || m != c->solveCount) return;  // backend_cuda.cu:763-768
+        HIPCK(hipMemcpyAsync(contacts, c->solve.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+}  // extern "C"

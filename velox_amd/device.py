@@ -1,0 +1,216 @@
+"""ctypes binding of libvxhip (include/vxhip.h), the MI355X device step.
+
+The library is built in-tree (velox_amd/_build/libvxhip.so) by
+velox_amd.build.build(); loading fails loudly when it is missing or when no
+gfx950 device is present -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from .records import BODY, CONTACT, SolverOptions, default_options, ptr
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "_build", "libvxhip.so")
+
+OK, ERR_MEMORY, ERR_DEVICE_LOST, ERR_INVALID, ERR_CAPACITY, ERR_UNAVAILABLE = range(6)
+
+
+class VxhipError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"vxhip error {code}: {message}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [("gravity", C.c_float * 3), ("substeps", C.c_int32), ("options", SolverOptions)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("bodies", C.c_uint32), ("pairs", C.c_uint32), ("contacts", C.c_uint32), ("colors", C.c_uint32),
+                ("ccdCandidates", C.c_uint32), ("ccdRescued", C.c_uint32), ("velocityIterations", C.c_uint32),
+                ("oversize", C.c_uint32), ("cellSize", C.c_float), ("reserved", C.c_uint32 * 7)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
+# Exported symbols and their signatures (include/vxhip.h).
+_vp, _u32, _f32, _i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int
+SIGNATURES = {
+    "vxhip_device_count": (_i32, [_vp]),
+    "vxhip_last_error": (C.c_char_p, []),
+    "vxhip_create": (_i32, [_vp]),
+    "vxhip_destroy": (None, [_vp]),
+    "vxhip_integrate": (_i32, [_vp, _vp, _u32, _vp, _f32]),
+    "vxhip_find_contacts": (_i32, [_vp, _vp, _u32, _vp, _f32, _vp, _u32, _vp]),
+    "vxhip_solve_velocities": (_i32, [_vp, _vp, _u32, _vp, _u32, _f32, _i32, _vp]),
+    "vxhip_advance_substeps": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _f32, _i32, _vp, _vp]),
+    "vxhip_fetch_impulses": (_i32, [_vp, _vp, _u32]),
+    "vxhip_invalidate_caches": (_i32, [_vp]),
+    "vxhip_last_velocity_iterations": (_u32, [_vp]),
+    "vxhip_world_load": (_i32, [_vp, _vp, _u32, _vp, _vp]),
+    "vxhip_world_step": (_i32, [_vp, _f32, _u32]),
+    "vxhip_world_sync": (_i32, [_vp]),
+    "vxhip_world_download_bodies": (_i32, [_vp, _vp, _u32]),
+    "vxhip_world_upload_bodies": (_i32, [_vp, _vp, _u32]),
+    "vxhip_world_download_contacts": (_i32, [_vp, _vp, _u32, _vp]),
+    "vxhip_world_stats": (_i32, [_vp, _vp]),
+    "vxhip_probe_pairs": (_i32, [_vp, _vp, _u32, _vp, _f32, _vp, _u32, _vp]),
+    "vxhip_probe_color": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
+    "vxhip_probe_warm_start": (_i32, [_vp, _vp, _u32, _vp, _u32]),
+}
+
+_lib = None
+
+
+def library():
+    """Load libvxhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBRARY):
+            raise RuntimeError(f"{LIBRARY} is missing: run velox_amd.build.build() (or __graft_entry__.build())")
+        lib = C.CDLL(LIBRARY)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = lib
+    return _lib
+
+
+def check(code):
+    if code != OK:
+        raise VxhipError(code, library().vxhip_last_error().decode())
+
+
+def device_count():
+    n = C.c_int(0)
+    check(library().vxhip_device_count(C.byref(n)))
+    return n.value
+
+
+class Context:
+    """One vxhip context: a device-resident world plus the Backend-style stage
+    entry points."""
+
+    def __init__(self):
+        lib = library()
+        h = C.c_void_p()
+        check(lib.vxhip_create(C.byref(h)))
+        self._h = h
+        self.n = 0
+
+    def close(self):
+        if self._h:
+            library().vxhip_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- resident world --------------------------------------------------
+    def load(self, bodies, soup, gravity=(0.0, -9.81, 0.0), substeps=4, options=None):
+        self._soup = soup
+        self._view = soup.view()
+        cfg = Config((C.c_float * 3)(*gravity), substeps, options or default_options())
+        bodies = np.ascontiguousarray(bodies)
+        assert bodies.dtype == BODY
+        check(library().vxhip_world_load(self._h, ptr(bodies), len(bodies), C.byref(self._view), C.byref(cfg)))
+        self.n = len(bodies)
+        return self
+
+    def step(self, dt, steps=1):
+        check(library().vxhip_world_step(self._h, float(dt), int(steps)))
+
+    def sync(self):
+        check(library().vxhip_world_sync(self._h))
+
+    def bodies(self):
+        out = np.zeros(self.n, BODY)
+        check(library().vxhip_world_download_bodies(self._h, ptr(out), self.n))
+        return out
+
+    def upload_bodies(self, bodies):
+        bodies = np.ascontiguousarray(bodies)
+        check(library().vxhip_world_upload_bodies(self._h, ptr(bodies), len(bodies)))
+
+    def contacts(self):
+        st = self.stats()
+        out = np.zeros(max(1, st["contacts"]), CONTACT)
+        cnt = C.c_uint32(0)
+        check(library().vxhip_world_download_contacts(self._h, ptr(out), len(out), C.byref(cnt)))
+        return out[:cnt.value]
+
+    def stats(self):
+        s = Stats()
+        check(library().vxhip_world_stats(self._h, C.byref(s)))
+        return s.as_dict()
+
+    # -- stage probes -----------------------------------------------------
+    def probe_pairs(self, bodies, soup, dt, cap=None):
+        bodies = np.ascontiguousarray(bodies)
+        view = soup.view()
+        cap = cap or max(1024, 64 * len(bodies))
+        out = np.zeros(cap, np.uint64)
+        cnt = C.c_uint32(0)
+        code = library().vxhip_probe_pairs(self._h, ptr(bodies), len(bodies), C.byref(view), float(dt), ptr(out),
+                                           cap, C.byref(cnt))
+        if code == ERR_CAPACITY:
+            return self.probe_pairs(bodies, soup, dt, cnt.value + 16)
+        check(code)
+        return out[:cnt.value]
+
+    def find_contacts(self, bodies, soup, dt, cap=None):
+        bodies = np.ascontiguousarray(bodies)
+        view = soup.view()
+        cap = cap or max(1024, 16 * len(bodies))
+        out = np.zeros(cap, CONTACT)
+        cnt = C.c_uint32(0)
+        code = library().vxhip_find_contacts(self._h, ptr(bodies), len(bodies), C.byref(view), float(dt), ptr(out),
+                                             cap, C.byref(cnt))
+        if code == ERR_CAPACITY:
+            return self.find_contacts(bodies, soup, dt, cnt.value + 16)
+        check(code)
+        return out[:cnt.value]
+
+    def color(self, bodies, contacts):
+        bodies = np.ascontiguousarray(bodies)
+        contacts = np.ascontiguousarray(contacts)
+        colors = np.zeros(max(1, len(contacts)), np.uint32)
+        k = C.c_uint32(0)
+        check(library().vxhip_probe_color(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts),
+                                          ptr(colors), C.byref(k)))
+        return colors[:len(contacts)], k.value
+
+    def warm_start_match(self, cur, prev):
+        cur = np.ascontiguousarray(cur).copy() if False else np.ascontiguousarray(cur)
+        prev = np.ascontiguousarray(prev)
+        check(library().vxhip_probe_warm_start(self._h, ptr(cur), len(cur), ptr(prev), len(prev)))
+        return cur
+
+    def integrate(self, bodies, gravity, dt):
+        g = (C.c_float * 3)(*gravity)
+        check(library().vxhip_integrate(self._h, ptr(bodies), len(bodies), g, float(dt)))
+
+    def solve_velocities(self, bodies, contacts, dt, warm_start, options=None):
+        opts = options or default_options()
+        check(library().vxhip_solve_velocities(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts),
+                                               float(dt), int(warm_start), C.byref(opts)))
+
+    def advance_substeps(self, bodies, contacts, gravity, h, substeps, options=None):
+        opts = options or default_options()
+        g = (C.c_float * 3)(*gravity)
+        adv = C.c_int(0)
+        check(library().vxhip_advance_substeps(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts), g,
+                                               float(h), int(substeps), C.byref(opts), C.byref(adv)))
+        return bool(adv.value)
+
+    def fetch_impulses(self, contacts):
+        check(library().vxhip_fetch_impulses(self._h, ptr(contacts), len(contacts)))
+
+    def last_velocity_iterations(self):
+        return library().vxhip_last_velocity_iterations(self._h)
