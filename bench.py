@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Throughput of the HIP device step on one MI355X.
+
+Workload: the 50,000-body mixed-debris scene (30% spheres, 30% boxes, 20%
+capsules, 20% 24-point hulls above a plane, FP32, dt = 1/60, 4 substeps,
+8 velocity iterations).  A step is one full device step of the resident
+world (integration, broad phase, narrow phase, warm-start matching, greedy
+coloring, colored TGS sweeps over 4 substeps, CCD net, positional
+correction).  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: body-steps/s (whole job), the dominant kernel's
+roofline (colored solve; algorithmic bytes per contact visit over its
+event-timed device time) and the reference CPU backend timed on this host.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes of one contact visit by the colored solve (per contact per
+# sweep): 96 B of contact rows read, 24 B of impulses written, and for each of
+# the two bodies 28 B pose + 32 B mass properties read, 24 B velocity read and
+# 24 B velocity written.
+SOLVE_BYTES_PER_VISIT = 96 + 24 + 2 * (28 + 32 + 24 + 24)
+
+
+def run_cpu_baseline(bodies, steps, workers):
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe, "bench", f"debris{bodies}", str(steps), str(workers), "1"], capture_output=True,
+                         text=True, timeout=900)
+    if out.returncode != 0:
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return r
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--bodies", type=int, default=50000)
+    ap.add_argument("--cpu-steps", type=int, default=30, help="reference CPU sample length (steps)")
+    ap.add_argument("--cpu-workers", type=int, default=16, help="reference CPU worker threads")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the device step has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from velox_amd.device import Context, Scene
+    scene = Scene("debris", args.bodies)
+    n = len(scene.bodies)
+    ctx = Context()
+    ctx.load_scene(scene)
+    dt = scene.dt
+    if args.warmup:
+        ctx.step(dt, args.warmup)
+        ctx.sync()
+    ctx.load_scene(scene)  # timed steps start from the initial state
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(dt, args.steps)
+    ctx.sync()
+    seconds = time.perf_counter() - t0
+    last = ctx.stats()
+    value = n * args.steps / seconds
+
+    roofline = None
+    timings = None
+    if not args.no_profile:
+        # Same trajectory again with per-launch HIP events on the library's stream.
+        ctx.load_scene(scene)
+        ctx.profile(True)
+        ctx.step(dt, args.steps)
+        ctx.sync()
+        timings = ctx.timings()
+        ctx.profile(False)
+        if timings["solveKernelMs"] > 0:
+            visits = timings["solveContactVisits"]
+            launches = timings["solveKernelLaunches"]
+            avg_ms = timings["solveKernelMs"] / launches
+            bytes_per_launch = SOLVE_BYTES_PER_VISIT * visits / launches
+            achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+            roofline = {"kernel": "kSolveColor", "bound": "hbm", "achieved": round(achieved, 3),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                        "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches,
+                        "bytes_per_launch": round(bytes_per_launch, 1)}
+
+    cpu = None
+    if not args.no_cpu:
+        r = run_cpu_baseline(args.bodies, args.cpu_steps, args.cpu_workers)
+        if r:
+            # The GPU over the same sample: the first cpu-steps steps.
+            ctx.load_scene(scene)
+            ctx.sync()
+            t = time.perf_counter()
+            ctx.step(dt, 1 + args.cpu_steps)
+            ctx.sync()
+            gpu_sample = n * (1 + args.cpu_steps) / (time.perf_counter() - t)
+            cpu = {"value": round(r["body_steps_per_s"], 1), "unit": "body-steps/s", "cores": r["workers"],
+                   "kind": "reference",
+                   "sample": f"reference World(BackendType::Cpu), relaxed -O3 build, steps 2-{1 + args.cpu_steps} of "
+                             f"the same {n}-body debris run (1 warm-up step excluded), {r['workers']} worker threads; "
+                             f"GPU over steps 1-{1 + args.cpu_steps}: {gpu_sample:.0f} body-steps/s",
+                   "gpu_same_sample": round(gpu_sample, 1)}
+    ctx.close()
+
+    line = {
+        "metric": "body-steps/s @50k mixed bodies",
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(seconds * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"mixed debris, {args.bodies} bodies + plane, dt=1/60, 4 substeps, 8 iterations",
+                   "bodies": n, "steps": args.steps, "contacts_last_step": last["contacts"],
+                   "colors_last_step": last["colors"], "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if timings:
+        line["stage_ms_per_step"] = {k[:-2]: round(v / max(1, timings["steps"]), 4)
+                                     for k, v in timings.items() if k.endswith("Ms") and k != "solveKernelMs"}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,23 @@ int vxhip_world_download_contacts(vxhip_ctx* ctx, vx_contact* out, uint32_t cap,
 /* Counters of the last step (StepStats subset, world.h:175-208). */
 int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
 
+/* Stage timings measured with HIP events on the context's own stream while
+ * profiling is enabled (accumulated over steps; StepStats timings,
+ * world.cpp:4523-4575).  Every colour-solve launch is bracketed by its own
+ * event pair, giving the solver kernel's device time and the contacts it
+ * visited. */
+typedef struct {
+    uint32_t steps;
+    float beginMs, broadMs, narrowMs, matchMs, colorMs, solveMs, ccdMs, positionMs, finishMs;
+    float solveKernelMs;           /* sum over colour-solve launches */
+    uint32_t solveKernelLaunches;
+    uint64_t solveContactVisits;   /* contacts processed by those launches */
+} vxhip_timings;
+
+/* Enables (non-zero) or disables per-stage event timing and resets it. */
+int vxhip_world_profile(vxhip_ctx* ctx, int enable);
+int vxhip_world_timings(vxhip_ctx* ctx, vxhip_timings* out);
+
 /* ------------------------------------------------------- stage probes -- */
 /* Single stages on caller data, for parity checks of one stage at a time. */
 
@@ -164,6 +181,22 @@ int vxhip_probe_color(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const v
 /* Warm-start matching of `cur` against `prev` (sorted by pair key), in place
  * (world.cpp:4133-4184). */
 int vxhip_probe_warm_start(vxhip_ctx* ctx, vx_contact* cur, uint32_t m, const vx_contact* prev, uint32_t mp);
+
+/* ------------------------------------------------------ synthetic scenes -- */
+/* Host utility: builds body records and flat geometry for the synthetic
+ * workloads the device step is measured on (no reference counterpart; the
+ * reference builds scenes through World::addBox/addSphere/addCapsule/
+ * addConvexHull/addStaticPlane/addStaticHeightfield, world.cpp:741-1477).
+ * Names: "lattice" (param = boxes per side), "debris" (param = body count),
+ * "pile", "terrain", "freeflight". */
+typedef struct vxhip_scene vxhip_scene;
+int vxhip_scene_build(const char* name, uint32_t param, vxhip_scene** out);
+uint32_t vxhip_scene_body_count(const vxhip_scene* scene);
+const vx_body* vxhip_scene_bodies(const vxhip_scene* scene);
+const uint32_t* vxhip_scene_worlds(const vxhip_scene* scene, uint32_t* worldCount);
+int vxhip_scene_soup(const vxhip_scene* scene, vx_soup* out);
+int vxhip_scene_config(const vxhip_scene* scene, vxhip_config* config, float* dt);
+void vxhip_scene_free(vxhip_scene* scene);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -11,26 +11,28 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 OUT = os.path.join(HERE, "_build")
 SOURCES = ["vx_step.hip", "vx_sort.hip"]
+HOST_SOURCES = ["scene_build.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-I" + os.path.join(ROOT, "include")]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
          "-fhip-fp32-correctly-rounded-divide-sqrt", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-I" + os.path.join(ROOT, "include")]
 
 
-def _compile(src, obj):
-    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(HERE, "csrc", src), "-o", obj]
-    subprocess.check_call(cmd)
-
-
 def build(jobs=None):
     """Compile every HIP translation unit (in parallel) and link libvxhip.so."""
     os.makedirs(OUT, exist_ok=True)
     objs = [os.path.join(OUT, s.replace(".hip", ".o")) for s in SOURCES]
+    hobjs = [os.path.join(OUT, s.replace(".cpp", ".o")) for s in HOST_SOURCES]
     procs = [subprocess.Popen([HIPCC] + FLAGS + ["-c", os.path.join(HERE, "csrc", s), "-o", o])
              for s, o in zip(SOURCES, objs)]
+    procs += [subprocess.Popen([CXX] + HOST_FLAGS + ["-c", os.path.join(HERE, "csrc", s), "-o", o])
+              for s, o in zip(HOST_SOURCES, hobjs)]
     codes = [p.wait() for p in procs]
     if any(codes):
-        raise RuntimeError(f"hipcc failed: {dict(zip(SOURCES, codes))}")
+        raise RuntimeError(f"compile failed: {dict(zip(SOURCES + HOST_SOURCES, codes))}")
+    objs += hobjs
     lib = os.path.join(OUT, "libvxhip.so")
     subprocess.check_call([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", lib + ".tmp"])
     os.replace(lib + ".tmp", lib)

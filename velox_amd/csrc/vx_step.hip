@@ -118,6 +118,13 @@ struct vxhip_ctx {
     vxhip_stats stats{};
     uint32_t lastIterations = 0;
     bool loaded = false;
+    // profiling (vxhip_world_profile): events on this context's stream
+    bool profile = false;
+    std::vector<hipEvent_t> stageEv;     // one per stage boundary
+    std::vector<hipEvent_t> launchEv;    // pairs around colour-solve launches
+    uint32_t launchUsed = 0;
+    std::vector<uint32_t> launchContacts;
+    vxhip_timings timings{};
 };
 
 namespace {
@@ -136,6 +143,53 @@ void* tempFor(vxhip_ctx* c, size_t bytes) {
 void readCounters(vxhip_ctx* c) {
     HIPCK(hipMemcpyAsync(c->hostCounters, c->counters.p, cCount * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
     HIPCK(hipStreamSynchronize(c->stream));
+}
+
+enum Stage : int { sBegin = 0, sBroad, sNarrow, sMatch, sColor, sSolve, sCcd, sPosition, sFinish, sEnd, sCount };
+
+void markStage(vxhip_ctx* c, int stage) {
+    if (c->profile) HIPCK(hipEventRecord(c->stageEv[stage], c->stream));
+}
+
+// Events bracket every colour-solve launch while profiling.
+void solveLaunchBegin(vxhip_ctx* c) {
+    if (!c->profile) return;
+    if (c->launchUsed + 2 > c->launchEv.size()) {
+        size_t old = c->launchEv.size();
+        c->launchEv.resize(old + 4096);
+        for (size_t i = old; i < c->launchEv.size(); ++i) HIPCK(hipEventCreate(&c->launchEv[i]));
+    }
+    HIPCK(hipEventRecord(c->launchEv[c->launchUsed], c->stream));
+}
+
+void solveLaunchEnd(vxhip_ctx* c, uint32_t contacts) {
+    if (!c->profile) return;
+    HIPCK(hipEventRecord(c->launchEv[c->launchUsed + 1], c->stream));
+    c->launchUsed += 2;
+    c->launchContacts.push_back(contacts);
+}
+
+void collectProfile(vxhip_ctx* c) {
+    if (!c->profile) return;
+    HIPCK(hipStreamSynchronize(c->stream));
+    float* stageMs[sCount - 1] = {&c->timings.beginMs, &c->timings.broadMs, &c->timings.narrowMs, &c->timings.matchMs,
+                                  &c->timings.colorMs, &c->timings.solveMs, &c->timings.ccdMs, &c->timings.positionMs,
+                                  &c->timings.finishMs};
+    for (int st = 0; st + 1 < sCount; ++st) {
+        float ms = 0.0f;
+        HIPCK(hipEventElapsedTime(&ms, c->stageEv[st], c->stageEv[st + 1]));
+        *stageMs[st] += ms;
+    }
+    for (uint32_t i = 0; i < c->launchUsed; i += 2) {
+        float ms = 0.0f;
+        HIPCK(hipEventElapsedTime(&ms, c->launchEv[i], c->launchEv[i + 1]));
+        c->timings.solveKernelMs += ms;
+        c->timings.solveKernelLaunches += 1;
+        c->timings.solveContactVisits += c->launchContacts[i / 2];
+    }
+    c->launchUsed = 0;
+    c->launchContacts.clear();
+    c->timings.steps += 1;
 }
 
 __global__ void kResetCounters(uint32_t* k) {
@@ -374,7 +428,10 @@ void solveSweeps(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx_
     for (int it = 0; it < sweeps; ++it)
         for (uint32_t col = 0; col < K; ++col) {
             uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
-            if (cnt) kSolveColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+            if (!cnt) continue;
+            solveLaunchBegin(c);
+            kSolveColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+            solveLaunchEnd(c, cnt);
         }
     HIPCK(hipGetLastError());
 }
@@ -415,17 +472,23 @@ void worldStep(vxhip_ctx* c, float dt) {
     hipStream_t s = c->stream;
     c->prevPos.reserve(n); c->prevQ.reserve(n);
     uint32_t* k = c->counters.p;
+    markStage(c, sBegin);
     kResetCounters<<<1, 64, 0, s>>>(k);
     kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
     HIPCK(hipGetLastError());
+    markStage(c, sBroad);
     const uint32_t np = broadPhase(c, dt);
+    markStage(c, sNarrow);
     const uint32_t m = narrowPhase(c, np, dt);
+    markStage(c, sMatch);
     if (c->anyAsleep && m) kWakePass<<<1, 64, 0, s>>>(c->bodies.p, c->contacts.p, m);
     if (m && c->prevCount) kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
+    markStage(c, sColor);
     const uint32_t K = colorContacts(c, c->contacts.p, m);
     colorOrder(c, m, K);
     c->solve.reserve(m ? m : 1);
     if (m) kGather<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, c->order.p, m, c->solve.p);
+    markStage(c, sSolve);
     for (int st = 0; st < S; ++st) {
         if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
         if (m) {
@@ -435,9 +498,12 @@ void worldStep(vxhip_ctx* c, float dt) {
         kTransforms<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, h);
     }
     HIPCK(hipGetLastError());
+    markStage(c, sCcd);
     kResetCounters<<<1, 64, 0, s>>>(k);
     ccdNet(c, np, dt);
+    markStage(c, sPosition);
     if (m) positionCorrection(c, c->solve.p, K, h, cfg.options);
+    markStage(c, sFinish);
     kClearForces<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n);
     c->prev.reserve(m ? m : 1);
     if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p);
@@ -446,7 +512,9 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->solveCount = m;
     c->numColors = K;
     c->lastIterations = (uint32_t)(2 * cfg.options.velocityIterations * S);
+    markStage(c, sEnd);
     readCounters(c);
+    collectProfile(c);
     c->stats.bodies = n;
     c->stats.pairs = np;
     c->stats.contacts = m;
@@ -521,6 +589,8 @@ void vxhip_destroy(vxhip_ctx* c) {
     c->colorVals.release(); c->colorVals2.release(); c->pred.release(); c->color.release(); c->colorCount.release();
     c->order.release(); c->colorStart.release(); c->maskAfter.release(); c->nextAfter.release(); c->ccdFlag.release();
     c->dirty.release(); c->temp.release(); c->counters.release();
+    for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -588,6 +658,27 @@ int vxhip_world_stats(vxhip_ctx* c, vxhip_stats* out) {
 }
 
 uint32_t vxhip_last_velocity_iterations(const vxhip_ctx* c) { return c ? c->lastIterations : 0; }
+
+int vxhip_world_profile(vxhip_ctx* c, int enable) {
+    return guarded([&] {
+        if (!c) fail(VXHIP_ERR_INVALID, "vxhip: null context");
+        if (enable && c->stageEv.empty()) {
+            c->stageEv.resize(sCount);
+            for (hipEvent_t& e : c->stageEv) HIPCK(hipEventCreate(&e));
+        }
+        c->profile = enable != 0;
+        c->timings = vxhip_timings{};
+        c->launchUsed = 0;
+        c->launchContacts.clear();
+    });
+}
+
+int vxhip_world_timings(vxhip_ctx* c, vxhip_timings* out) {
+    return guarded([&] {
+        if (!c || !out) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *out = c->timings;
+    });
+}
 
 int vxhip_invalidate_caches(vxhip_ctx* c) {
     return guarded([&] {

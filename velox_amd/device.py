@@ -60,6 +60,15 @@ SIGNATURES = {
     "vxhip_probe_pairs": (_i32, [_vp, _vp, _u32, _vp, _f32, _vp, _u32, _vp]),
     "vxhip_probe_color": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "vxhip_probe_warm_start": (_i32, [_vp, _vp, _u32, _vp, _u32]),
+    "vxhip_world_profile": (_i32, [_vp, _i32]),
+    "vxhip_world_timings": (_i32, [_vp, _vp]),
+    "vxhip_scene_build": (_i32, [C.c_char_p, _u32, _vp]),
+    "vxhip_scene_body_count": (_u32, [_vp]),
+    "vxhip_scene_bodies": (_vp, [_vp]),
+    "vxhip_scene_worlds": (_vp, [_vp, _vp]),
+    "vxhip_scene_soup": (_i32, [_vp, _vp]),
+    "vxhip_scene_config": (_i32, [_vp, _vp, _vp]),
+    "vxhip_scene_free": (None, [_vp]),
 }
 
 _lib = None
@@ -90,6 +99,71 @@ def device_count():
     return n.value
 
 
+class Timings(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("beginMs", C.c_float), ("broadMs", C.c_float), ("narrowMs", C.c_float),
+                ("matchMs", C.c_float), ("colorMs", C.c_float), ("solveMs", C.c_float), ("ccdMs", C.c_float),
+                ("positionMs", C.c_float), ("finishMs", C.c_float), ("solveKernelMs", C.c_float),
+                ("solveKernelLaunches", C.c_uint32), ("solveContactVisits", C.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class Scene:
+    """Body records + flat geometry of a synthetic workload, built host-side by
+    the library (vxhip_scene_build)."""
+
+    def __init__(self, name, param=0):
+        from .records import SoupArrays
+        lib = library()
+        h = C.c_void_p()
+        check(lib.vxhip_scene_build(name.encode(), int(param), C.byref(h)))
+        try:
+            n = lib.vxhip_scene_body_count(h)
+            raw = (C.c_uint8 * (320 * n)).from_address(lib.vxhip_scene_bodies(h)) if n else b""
+            self.bodies = np.frombuffer(bytes(raw), np.uint8).reshape(-1, 320).copy().view(BODY).reshape(-1)
+            wc = C.c_uint32(0)
+            wp = lib.vxhip_scene_worlds(h, C.byref(wc))
+            self.worlds = wc.value
+            self.world_of = np.frombuffer(bytes((C.c_uint8 * (4 * n)).from_address(wp)), np.uint32).copy() \
+                if n else np.zeros(0, np.uint32)
+            view = SoupView()
+            check(lib.vxhip_scene_soup(h, C.byref(view)))
+            cfg = Config()
+            dt = C.c_float(0)
+            check(lib.vxhip_scene_config(h, C.byref(cfg), C.byref(dt)))
+            self.gravity = tuple(cfg.gravity)
+            self.substeps = cfg.substeps
+            self.dt = dt.value
+
+            def grab(p, count, dtype, width):
+                if not count:
+                    return None
+                item = np.dtype(dtype).itemsize * (width or 1)
+                a = np.frombuffer(bytes((C.c_uint8 * (item * count)).from_address(p)), dtype)
+                return a.reshape(count, width).copy() if width else a.copy()
+            self.soup = SoupArrays({
+                "vertices": grab(view.vertices, view.vertexCount, np.float32, 3),
+                "indices": grab(view.indices, view.indexCount, np.uint32, 0),
+                "meshes": grab(view.meshes, view.meshCount, np.uint8, 52),
+                "nodes": grab(view.bvhNodes, view.bvhNodeCount, np.uint8, 32),
+                "trirefs": grab(view.bvhTriRefs, view.bvhTriRefCount, np.uint32, 0),
+                "hullpts": grab(view.hullPoints, view.hullPointCount, np.float32, 3),
+                "hullfaces": grab(view.hullFaceIndices, view.hullFaceIndexCount, np.uint32, 0),
+            })
+        finally:
+            lib.vxhip_scene_free(h)
+
+
+class SoupView(C.Structure):
+    _fields_ = [("vertices", C.c_void_p), ("vertexCount", C.c_uint32), ("indices", C.c_void_p),
+                ("indexCount", C.c_uint32), ("meshes", C.c_void_p), ("meshCount", C.c_uint32),
+                ("bvhNodes", C.c_void_p), ("bvhNodeCount", C.c_uint32), ("bvhTriRefs", C.c_void_p),
+                ("bvhTriRefCount", C.c_uint32), ("hullPoints", C.c_void_p), ("hullPointCount", C.c_uint32),
+                ("hullFaceIndices", C.c_void_p), ("hullFaceIndexCount", C.c_uint32),
+                ("compoundChildren", C.c_void_p), ("compoundChildCount", C.c_uint32)]
+
+
 class Context:
     """One vxhip context: a device-resident world plus the Backend-style stage
     entry points."""
@@ -113,6 +187,9 @@ class Context:
             pass
 
     # -- resident world --------------------------------------------------
+    def load_scene(self, scene, options=None):
+        return self.load(scene.bodies, scene.soup, gravity=scene.gravity, substeps=scene.substeps, options=options)
+
     def load(self, bodies, soup, gravity=(0.0, -9.81, 0.0), substeps=4, options=None):
         self._soup = soup
         self._view = soup.view()
@@ -144,6 +221,14 @@ class Context:
         cnt = C.c_uint32(0)
         check(library().vxhip_world_download_contacts(self._h, ptr(out), len(out), C.byref(cnt)))
         return out[:cnt.value]
+
+    def profile(self, enable=True):
+        check(library().vxhip_world_profile(self._h, int(enable)))
+
+    def timings(self):
+        t = Timings()
+        check(library().vxhip_world_timings(self._h, C.byref(t)))
+        return t.as_dict()
 
     def stats(self):
         s = Stats()
