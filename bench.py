@@ -52,11 +52,21 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=16, help="reference CPU worker threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="run one plain device copy of this size first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the device step has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(args.calibrate_copy << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     from velox_amd.device import Context, Scene
     scene = Scene("debris", args.bodies)
@@ -92,7 +102,7 @@ def main():
             avg_ms = timings["solveKernelMs"] / launches
             bytes_per_launch = SOLVE_BYTES_PER_VISIT * visits / launches
             achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-            roofline = {"kernel": "kSolveColor", "bound": "hbm", "achieved": round(achieved, 3),
+            roofline = {"kernel": "kSweepColor", "bound": "hbm", "achieved": round(achieved, 3),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
                         "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches,
                         "bytes_per_launch": round(bytes_per_launch, 1)}

@@ -2,10 +2,10 @@
 //
 // The reference walks the unique contact pairs in key order on the host, and
 // a rescue rewinds bodies that later pairs read.  kCcdCheck evaluates every
-// pair in parallel against the end-of-substeps state; kCcdSequential then
-// replays the reference order exactly, revisiting a pair only when it was
-// flagged or one of its bodies was moved by an earlier rescue (any other
-// pair sees the same state the parallel check saw).
+// pair in parallel against the end-of-substeps state; kCcdDataflow then
+// reproduces the reference order exactly, revisiting a pair only when it was
+// flagged or one of its bodies was moved by an earlier rescue (any other pair
+// sees the same state the parallel check saw).
 #pragma once
 #include "vx_kernels.hpp"
 
@@ -94,7 +94,8 @@ __device__ inline float ccdSearch(const vx_body& a, float dt) {
 __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict__ bodies, Soup soup, float dt,
                                                       const uint64_t* __restrict__ pairs,
                                                       const uint32_t* __restrict__ counts, uint32_t np, int nb,
-                                                      uint8_t* __restrict__ flag, uint32_t* __restrict__ flagged,
+                                                      uint8_t* __restrict__ flag, float4* __restrict__ endGap,
+                                                      uint32_t* __restrict__ flagged,
                                                       uint32_t* __restrict__ firstFlagged) {
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
@@ -106,7 +107,10 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
         if (ccdEligible(a, b)) {
             Gap end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup,
                             ccdSearch(a, dt));
-            if (end.gap < -8e-3f) f = 1;
+            if (end.gap < -8e-3f) {
+                f = 1;
+                endGap[p] = make_float4(end.normal.x, end.normal.y, end.normal.z, end.gap);
+            }
         }
     }
     flag[p] = f;
@@ -117,12 +121,16 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
 }
 
 // The reference's per-pair rescue (world.cpp:4276-4355) on one lane.
+// `known` carries the end gap the parallel check measured when neither body
+// has moved since (same state, same result).
 __device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const vx_quat* prevQ, uint32_t i, uint32_t j,
-                                 const Soup& soup, float dt) {
+                                 const Soup& soup, float dt, const float4* known) {
     vx_body a = bodies[i], b = bodies[j];
     if (!ccdEligible(a, b)) return false;
     const float search = ccdSearch(a, dt);
-    Gap end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup, search);
+    Gap end;
+    if (known) end = {known->w, {known->x, known->y, known->z}};
+    else end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup, search);
     if (end.gap >= -8e-3f) return false;
     V3 pa0 = v3(prevPos[i]), da = v3(a.position) - pa0;
     Q4 qa0 = q4(prevQ[i]);
@@ -176,49 +184,76 @@ __device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const 
 
 constexpr int kCcdThreads = 1024;
 
-__global__ __launch_bounds__(kCcdThreads) void kCcdSequential(vx_body* bodies, const vx_vec3* __restrict__ prevPos,
-                                                              const vx_quat* __restrict__ prevQ, Soup soup, float dt,
-                                                              const uint64_t* __restrict__ pairs,
-                                                              const uint32_t* __restrict__ counts, uint32_t np, int nb,
-                                                              const uint8_t* __restrict__ flag,
-                                                              const uint32_t* __restrict__ flagged,
-                                                              const uint32_t* __restrict__ firstFlagged,
-                                                              uint8_t* dirty, uint32_t* __restrict__ examined,
-                                                              uint32_t* __restrict__ rescued) {
-    __shared__ uint32_t next;
+// One (body << pb | pair) entry per dynamic body of every pair that produced
+// contacts; sorted, they give each pair its predecessor on each body.
+__global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restrict__ bodies,
+                                                        const uint64_t* __restrict__ pairs,
+                                                        const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                        int pbits, uint64_t* __restrict__ keys,
+                                                        uint32_t* __restrict__ vals, uint32_t* __restrict__ counter) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    bool s0 = false, s1 = false;
+    uint32_t i = 0, j = 0;
+    if (p < np && counts[p]) {
+        ccdBodies(bodies, pairs[p], nb, i, j);
+        s0 = isDynamic(bodies[i]);
+        s1 = isDynamic(bodies[j]);
+    }
+    uint32_t at = waveReserve((s0 ? 1u : 0u) + (s1 ? 1u : 0u), counter);
+    if (s0) { keys[at] = ((uint64_t)i << pbits) | p; vals[at] = p << 1; ++at; }
+    if (s1) { keys[at] = ((uint64_t)j << pbits) | p; vals[at] = (p << 1) | 1u; }
+}
+
+// The net in reference order by dataflow across workgroups: a pair waits for
+// the previous pairs on its dynamic bodies (one atomic word each: done bit and
+// a "moved" bit per body), re-examines itself only when flagged or when one of
+// its bodies was rewound earlier, and passes the moved bits on.  Body data is
+// published with release/acquire at device scope only along chains that
+// carry a move; all other hand-offs are relaxed.  Lanes walk their pairs in
+// increasing order and the grid is sized to be co-resident, so the lowest
+// unfinished pair is always running and ready; a spin budget bounds waits.
+constexpr int kCcdBlocks = 128;
+
+__global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const vx_vec3* __restrict__ prevPos,
+                                                         const vx_quat* __restrict__ prevQ, Soup soup, float dt,
+                                                         const uint64_t* __restrict__ pairs,
+                                                         const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                         const uint8_t* __restrict__ flag,
+                                                         const float4* __restrict__ endGap,
+                                                         const uint32_t* __restrict__ flagged,
+                                                         const uint32_t* __restrict__ pred, uint32_t* word,
+                                                         uint32_t* __restrict__ examined,
+                                                         uint32_t* __restrict__ rescued, uint32_t* __restrict__ error) {
     if (*flagged == 0) return;
-    const uint32_t start = *firstFlagged;
-    uint32_t cursor = start;  // first pair still to consider
-    for (uint32_t base = start - start % kCcdThreads; base < np; base += kCcdThreads) {
-        for (;;) {
-            if (threadIdx.x == 0) next = 0xffffffffu;
-            __syncthreads();
-            uint32_t p = base + threadIdx.x;
-            if (p < np && p >= cursor && counts[p]) {
+    const uint32_t stride = gridDim.x * kThreads;
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t spins = 0;
+    while (p < np) {
+        if (!counts[p]) { p += stride; continue; }
+        const uint32_t pa = pred[2 * p], pb = pred[2 * p + 1];
+        const uint32_t wa = pa == kNone ? 1u : __hip_atomic_load(&word[pa >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t wb = pb == kNone ? 1u : __hip_atomic_load(&word[pb >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((wa & 1u) && (wb & 1u)) {
+            const bool movedA = pa != kNone && ((wa >> (1 + (pa & 1))) & 1u);
+            const bool movedB = pb != kNone && ((wb >> (1 + (pb & 1))) & 1u);
+            if (movedA || movedB) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            bool r = false;
+            if (flag[p] || movedA || movedB) {
                 uint32_t i, j;
                 ccdBodies(bodies, pairs[p], nb, i, j);
-                if (flag[p] || __hip_atomic_load(&dirty[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
-                    __hip_atomic_load(&dirty[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-                    atomicMin(&next, p);
+                atomicAdd(examined, 1u);
+                r = ccdRescue(bodies, prevPos, prevQ, i, j, soup, dt, (movedA || movedB) ? nullptr : &endGap[p]);
+                if (r) atomicAdd(rescued, 1u);
             }
-            __syncthreads();
-            const uint32_t q = next;
-            if (q == 0xffffffffu) break;
-            if (threadIdx.x == 0) {
-                uint32_t i, j;
-                ccdBodies(bodies, pairs[q], nb, i, j);
-                ++*examined;
-                if (ccdRescue(bodies, prevPos, prevQ, i, j, soup, dt)) {
-                    ++*rescued;
-                    dirty[i] = 1;
-                    dirty[j] = 1;
-                }
-            }
-            cursor = q + 1;
-            __threadfence_block();
-            __syncthreads();
+            const uint32_t out = 1u | ((movedA || r) ? 2u : 0u) | ((movedB || r) ? 4u : 0u);
+            if (out != 1u) __hip_atomic_store(&word[p], out, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            else __hip_atomic_store(&word[p], out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p += stride;
+            spins = 0;
+        } else if (++spins > (1u << 24)) {
+            atomicOr(error, 1u);
+            break;
         }
-        cursor = base + kCcdThreads;
     }
 }
 

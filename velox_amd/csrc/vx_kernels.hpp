@@ -495,21 +495,6 @@ __global__ __launch_bounds__(kThreads) void kWarmStartColor(vx_body* __restrict_
     cs[first + k].point = c.point;
 }
 
-__global__ __launch_bounds__(kThreads) void kSolveColor(vx_body* __restrict__ bodies, vx_contact* __restrict__ cs,
-                                                        const uint32_t* __restrict__ colorStart, uint32_t color, float h,
-                                                        vx_solver_options opt) {
-    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
-    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= count) return;
-    vx_contact c = cs[first + k];
-    vx_body a = bodies[c.a], b = bodies[c.b];
-    if (isSensor(a) || isSensor(b)) return;
-    solveContact(a, b, c, h, opt);
-    if (isDynamic(a)) storeVel(bodies, c.a, a);
-    if (isDynamic(b)) storeVel(bodies, c.b, b);
-    cs[first + k] = c;
-}
-
 // world.cpp:4364-4397, one colour at a time.
 __global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,
                                                            const uint32_t* __restrict__ colorStart, uint32_t color, float h,

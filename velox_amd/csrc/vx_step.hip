@@ -7,6 +7,7 @@
 #include <string>
 #include <vector>
 #include "vx_ccd.hpp"
+#include "vx_solve.hpp"
 #include "vx_internal.hpp"
 #include "../../include/vxhip.h"
 
@@ -104,7 +105,11 @@ struct vxhip_ctx {
     Buf<uint32_t> colorVals, colorVals2, pred, color, colorCount, order, colorStart;
     Buf<unsigned long long> maskAfter;
     Buf<int> nextAfter;
-    Buf<uint8_t> ccdFlag, dirty;
+    Buf<uint8_t> ccdFlag;
+    Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
+    Buf<uint4> sMeta;
+    Buf<uint32_t> ccdDone;
+    Buf<float4> ccdGap;
     Buf<uint8_t> temp;
     Buf<uint32_t> counters;  // device counters, see Counter
     uint32_t* hostCounters = nullptr;
@@ -202,10 +207,6 @@ __global__ void kResetCounters(uint32_t* k) {
     k[t] = v;
 }
 
-__global__ void kFill8(uint8_t* p, uint32_t n) {
-    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i < n) p[i] = 0;
-}
 __global__ void kFill32(uint32_t* p, uint32_t n, uint32_t v) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i < n) p[i] = v;
@@ -423,16 +424,31 @@ void warmStartColors(vxhip_ctx* c, vx_contact* solve, uint32_t K) {
     HIPCK(hipGetLastError());
 }
 
+// One substep's velocity solve: compact body arrays, per-substep rows, then
+// 2*I sweeps of per-colour launches (backend_cuda.cu:661-683), velocities and
+// impulses written back.
 void solveSweeps(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx_solver_options& opt) {
+    const uint32_t n = c->n, m = c->solveCount;
+    hipStream_t s = c->stream;
+    c->sVel.reserve(n); c->sAng.reserve(n); c->sInvI.reserve(n); c->sFrame.reserve(n);
+    c->sRows.reserve(8 * (size_t)m); c->sImp.reserve(2 * (size_t)m); c->sMeta.reserve(m);
+    SolveArrays sa{c->sVel.p, c->sAng.p, c->sInvI.p, c->sFrame.p, c->sRows.p, c->sMeta.p, c->sImp.p};
+    kGatherSolveBodies<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, sa);
+    kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
+    kPrepareRows<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, solve, m, h, sa);
+    const bool cone = opt.frictionModel == VX_FRICTION_CONE_BLOCK;
     const int sweeps = 2 * opt.velocityIterations;
     for (int it = 0; it < sweeps; ++it)
         for (uint32_t col = 0; col < K; ++col) {
             uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
             if (!cnt) continue;
             solveLaunchBegin(c);
-            kSolveColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+            if (cone) kSweepColor<true><<<blocks(cnt), kThreads, 0, s>>>(sa, c->colorStart.p, col);
+            else kSweepColor<false><<<blocks(cnt), kThreads, 0, s>>>(sa, c->colorStart.p, col);
             solveLaunchEnd(c, cnt);
         }
+    kScatterVelocities<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, sa);
+    kStoreImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
     HIPCK(hipGetLastError());
 }
 
@@ -450,14 +466,32 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     hipStream_t s = c->stream;
     uint32_t* k = c->counters.p;
     const int nb = bodyBits(c->n);
+    const int pbits = vxhost::bitsFor(np - 1) ? vxhost::bitsFor(np - 1) : 1;
     c->ccdFlag.reserve(np);
-    c->dirty.reserve(c->n);
-    kFill8<<<blocks(c->n), kThreads, 0, s>>>(c->dirty.p, c->n);
+    c->ccdGap.reserve(np);
     kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb, c->ccdFlag.p,
-                                              k + cCcdFlagged, k + cCcdFirst);
-    kCcdSequential<<<1, kCcdThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->counts.p,
-                                             np, nb, c->ccdFlag.p, k + cCcdFlagged, k + cCcdFirst, c->dirty.p,
-                                             k + cCcdExamined, k + cCcdRescued);
+                                              c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    if (c->hostCounters[cCcdFlagged] == 0) return;
+    c->keys.reserve(2 * (size_t)np); c->keysTmp.reserve(2 * (size_t)np);
+    c->colorVals.reserve(2 * (size_t)np); c->colorVals2.reserve(2 * (size_t)np);
+    c->pred.reserve(2 * (size_t)np);
+    c->ccdDone.reserve(np);
+    fill32(c, c->pred.p, 2 * np, kNone);
+    fill32(c, c->ccdDone.p, np, 0);
+    kCcdEntries<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, pbits, c->keys.p,
+                                                c->colorVals.p, k + cEntries);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    const uint32_t e = c->hostCounters[cEntries];
+    HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + pbits,
+                            tempFor(c, vxhost::radixTempBytes(e)), s));
+    if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, pbits, c->pred.p);
+    const uint32_t grid = std::min<uint32_t>(blocks(np), kCcdBlocks);
+    kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->counts.p,
+                                           np, nb, c->ccdFlag.p, c->ccdGap.p, k + cCcdFlagged, c->pred.p, c->ccdDone.p,
+                                           k + cCcdExamined, k + cCcdRescued, k + cColorError);
     HIPCK(hipGetLastError());
 }
 
@@ -488,6 +522,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     colorOrder(c, m, K);
     c->solve.reserve(m ? m : 1);
     if (m) kGather<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, c->order.p, m, c->solve.p);
+    c->solveCount = m;
     markStage(c, sSolve);
     for (int st = 0; st < S; ++st) {
         if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
@@ -514,6 +549,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->lastIterations = (uint32_t)(2 * cfg.options.velocityIterations * S);
     markStage(c, sEnd);
     readCounters(c);
+    if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: CCD dataflow did not converge");
     collectProfile(c);
     c->stats.bodies = n;
     c->stats.pairs = np;
@@ -588,7 +624,9 @@ void vxhip_destroy(vxhip_ctx* c) {
     c->offsets.release(); c->slots.release(); c->contacts.release(); c->solve.release(); c->prev.release();
     c->colorVals.release(); c->colorVals2.release(); c->pred.release(); c->color.release(); c->colorCount.release();
     c->order.release(); c->colorStart.release(); c->maskAfter.release(); c->nextAfter.release(); c->ccdFlag.release();
-    c->dirty.release(); c->temp.release(); c->counters.release();
+    c->temp.release(); c->counters.release(); c->ccdDone.release(); c->ccdGap.release();
+    c->sVel.release(); c->sAng.release(); c->sInvI.release(); c->sFrame.release(); c->sRows.release(); c->sImp.release();
+    c->sMeta.release();
     for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
