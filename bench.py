@@ -45,13 +45,18 @@ def run_cpu_baseline(bodies, steps, workers):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
+    # The reference itself diverges on this scene near step 135 (thin capsules
+    # spun about their axis inflate maxPointSpeed-swept boxes until they pair
+    # with every body); both implementations reproduce it, so the timed window
+    # is the physically valid part of the run.
+    ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--bodies", type=int, default=50000)
     ap.add_argument("--cpu-steps", type=int, default=30, help="reference CPU sample length (steps)")
     ap.add_argument("--cpu-workers", type=int, default=16, help="reference CPU worker threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--progress", type=int, default=25, help="steps between progress lines")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="run one plain device copy of this size first (HBM counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -80,8 +85,17 @@ def main():
     ctx.load_scene(scene)  # timed steps start from the initial state
     ctx.sync()
     t0 = time.perf_counter()
-    ctx.step(dt, args.steps)
-    ctx.sync()
+    done = 0
+    while done < args.steps:  # progress every `chunk` steps (one host sync each)
+        k = min(args.progress, args.steps - done)
+        ctx.step(dt, k)
+        ctx.sync()
+        done += k
+        st = ctx.stats()
+        print(f"[bench] step {done}/{args.steps} t={time.perf_counter() - t0:.2f}s contacts={st['contacts']} "
+              f"colors={st['colors']} pairs={st['pairs']} ccd={st['ccdCandidates']}/{st['ccdRescued']} "
+              f"islands={st['islands']} island_contacts={st['islandContacts']} big={st['bigIslandContacts']}",
+              file=sys.stderr, flush=True)
     seconds = time.perf_counter() - t0
     last = ctx.stats()
     value = n * args.steps / seconds
@@ -139,7 +153,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"mixed debris, {args.bodies} bodies + plane, dt=1/60, 4 substeps, 8 iterations",
+        "config": {"workload": f"mixed debris, {args.bodies} bodies + plane, dt=1/60, 4 substeps, 8 iterations, "
+                               f"steps 1-{args.steps}",
                    "bodies": n, "steps": args.steps, "contacts_last_step": last["contacts"],
                    "colors_last_step": last["colors"], "parallelism": "single GPU"},
         "roofline": roofline,

@@ -63,7 +63,10 @@ typedef struct {
     uint32_t velocityIterations; /* sweeps run in the last step (lastVelocityIterations) */
     uint32_t oversize;         /* bodies on the brute-force broad-phase path */
     float cellSize;            /* broad-phase grid cell edge */
-    uint32_t reserved[7];
+    uint32_t islands;          /* small contact islands solved whole by one workgroup */
+    uint32_t islandContacts;   /* contacts in those islands */
+    uint32_t bigIslandContacts;/* contacts solved colour by colour (islands above the size limit) */
+    uint32_t reserved[4];
 } vxhip_stats;
 
 /* ---------------------------------------------------------------- setup -- */

@@ -465,10 +465,31 @@ int bench(const std::string& name, int steps, int workers, int warmup) {
     for (int k = 0; k < warmup; ++k) w.step(s.dt);
     std::vector<double> per;
     auto t0 = std::chrono::steady_clock::now();
+    const bool progress = std::getenv("VXREF_PROGRESS") != nullptr;
     for (int k = 0; k < steps; ++k) {
         auto a = std::chrono::steady_clock::now();
         w.step(s.dt);
         per.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count());
+        if (progress && (k + 1) % 10 == 0) {
+            double maxSpeed = 0.0, minY = 1e30;
+            for (const Body& b : w.bodies_)
+                if (b.isDynamic()) { maxSpeed = std::max(maxSpeed, (double)length(b.velocity)); minY = std::min(minY, (double)b.position.y); }
+            std::fprintf(stderr, "[ref] step %d contacts=%zu step_s=%.3f maxSpeed=%.2f minY=%.3f\n", warmup + k + 1,
+                         w.contacts_.size(), per.back(), maxSpeed, minY);
+            if (std::getenv("VXREF_TOP")) {
+                std::vector<int> per(w.bodies_.size(), 0);
+                for (const Contact& c : w.contacts_) { ++per[c.a]; ++per[c.b]; }
+                for (int t = 0; t < 4; ++t) {
+                    size_t best = 1;
+                    for (size_t i = 1; i < per.size(); ++i) if (per[i] > per[best]) best = i;
+                    const Body& b = w.bodies_[best];
+                    std::fprintf(stderr, "    body %zu contacts=%d shape=%d r=%.3f he=(%.2f %.2f %.2f) hh=%.2f pos=(%.2f %.2f %.2f) v=%.2f hull=%u\n",
+                                 best, per[best], (int)b.shape, b.radius, b.halfExtents.x, b.halfExtents.y, b.halfExtents.z,
+                                 b.capsuleHalfHeight, b.position.x, b.position.y, b.position.z, length(b.velocity), b.hullCount);
+                    per[best] = -1;
+                }
+            }
+        }
     }
     double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::sort(per.begin(), per.end());

@@ -12,7 +12,7 @@
 
 namespace vxk {
 
-enum RowFlag : uint32_t { kDynA = 1, kDynB = 2, kRotA = 4, kRotB = 8, kSkip = 16 };
+enum RowFlag : uint32_t { kDynA = 1, kDynB = 2, kRotA = 4, kRotB = 8, kSkip = 16, kSensorRow = 32 };
 
 struct SolveArrays {
     float4* vel;      // per body: linear velocity
@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kThreads) void kPrepareRows(const vx_body* __restri
     if (isDynamic(b) && !rotationLocked(b)) flags |= kRotB;
     float4* row = sa.rows + 8 * (size_t)k;
     if (isSensor(a) || isSensor(b)) {
-        sa.meta[k] = make_uint4(c.a, c.b, flags | kSkip, 0);
+        sa.meta[k] = make_uint4(c.a, c.b, flags | kSkip | kSensorRow, 0);
         return;
     }
     const float4 fa = sa.frame[c.a], fb = sa.frame[c.b], ia = sa.invI[c.a], ib = sa.invI[c.b];
@@ -128,30 +128,27 @@ __global__ __launch_bounds__(kThreads) void kPrepareRows(const vx_body* __restri
 
 __device__ inline V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
 
-// One colour of one sweep: the velocity-dependent rows of solveContact
-// (narrowphase.h:183-365) for contacts [colorStart[color], colorStart[color+1]).
+// Velocity-dependent rows of solveContact (narrowphase.h:183-365) for one
+// contact whose pose-dependent rows are already loaded.
+struct RowRegs {
+    float4 r[8];
+    uint32_t flags;
+};
+
 template <bool kCone>
-__device__ inline void solveRow(SolveArrays sa, uint32_t k) {
-    const uint4 meta = sa.meta[k];
-    const uint32_t flags = meta.z;
-    if (flags & kSkip) return;
-    const uint32_t ia = meta.x, ib = meta.y;
-    const float4* row = sa.rows + 8 * (size_t)k;
-    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4], r5 = row[5], r6 = row[6], r7 = row[7];
+__device__ inline void solveRegs(const RowRegs& R, float4 fa, float4 iia, float4 fb, float4 iib, V3& vA, V3& wA, V3& vB,
+                                 V3& wB, float& cN, float& cT1, float& cT2, float& cR1, float& cR2, float& cS) {
+    const uint32_t flags = R.flags;
+    const float4 r0 = R.r[0], r1 = R.r[1], r2 = R.r[2], r3 = R.r[3], r4 = R.r[4], r5 = R.r[5], r6 = R.r[6], r7 = R.r[7];
     const V3 ra = xyz(r0), rb = xyz(r1), n = xyz(r2), t1 = xyz(r3), t2 = xyz(r4);
     const float kN = r0.w, k1 = r1.w, k2 = r2.w, targetBase = r3.w, lg = r4.w;
     const float vn0 = r5.x, rest = r5.y, mu1 = r5.z, mu2 = r5.w;
     const float rolling = r6.x, spinning = r6.y, kr1 = r6.z, kr2 = r6.w;
     const float ks = r7.x, kN1 = r7.y, kN2 = r7.z, k12 = r7.w;
     const bool dynA = flags & kDynA, dynB = flags & kDynB, rotA = flags & kRotA, rotB = flags & kRotB;
-    const float4 fa = sa.frame[ia], fb = sa.frame[ib], iia = sa.invI[ia], iib = sa.invI[ib];
     const float simA = iia.w, simB = iib.w;
     auto IA = [&](V3 v) { return frameMul(fa, iia, v, rotA); };
     auto IB = [&](V3 v) { return frameMul(fb, iib, v, rotB); };
-    V3 vA = xyz(sa.vel[ia]), wA = xyz(sa.ang[ia]), vB = xyz(sa.vel[ib]), wB = xyz(sa.ang[ib]);
-    float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
-    float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
-
     float vn = dot((vA + cross(wA, ra)) - (vB + cross(wB, rb)), n);
     float target = targetBase;
     if (vn0 < -1.0f && (lg <= 0.0f || vn > 0.0f)) target = fmx(target, -rest * vn0);
@@ -172,7 +169,7 @@ __device__ inline void solveRow(SolveArrays sa, uint32_t k) {
             if (m1 <= 0.0f) n1 = 0.0f;
             if (m2 <= 0.0f) n2 = 0.0f;
             float el = (m1 > 0.0f ? n1 * n1 / (m1 * m1) : 0.0f) + (m2 > 0.0f ? n2 * n2 / (m2 * m2) : 0.0f);
-            if (el > 1.0f) { float s = 1.0f / sqrtf(el); n1 *= s; n2 *= s; }
+            if (el > 1.0f) { float sc = 1.0f / sqrtf(el); n1 *= sc; n2 *= sc; }
             cT1 = n1;
             cT2 = n2;
             V3 fi = t1 * (n1 - o1) + t2 * (n2 - o2);
@@ -201,7 +198,7 @@ __device__ inline void solveRow(SolveArrays sa, uint32_t k) {
         if (m1 <= 0.0f) n1 = 0.0f;
         if (m2 <= 0.0f) n2 = 0.0f;
         const float el = (m1 > 0.0f ? n1 * n1 / (m1 * m1) : 0.0f) + (m2 > 0.0f ? n2 * n2 / (m2 * m2) : 0.0f);
-        if (el > 1.0f) { const float s = 1.0f / sqrtf(el); n1 *= s; n2 *= s; }
+        if (el > 1.0f) { const float sc = 1.0f / sqrtf(el); n1 *= sc; n2 *= sc; }
         cN = nn;
         cT1 = n1;
         cT2 = n2;
@@ -216,7 +213,7 @@ __device__ inline void solveRow(SolveArrays sa, uint32_t k) {
         float n2 = o2 - dot(wr, t2) / kr2;
         float lim = rolling * cN;
         float mag = sqrtf(n1 * n1 + n2 * n2);
-        if (mag > lim && mag > 1e-9f) { float s = lim / mag; n1 *= s; n2 *= s; }
+        if (mag > lim && mag > 1e-9f) { float sc = lim / mag; n1 *= sc; n2 *= sc; }
         cR1 = n1;
         cR2 = n2;
         V3 tq = t1 * (n1 - o1) + t2 * (n2 - o2);
@@ -234,8 +231,23 @@ __device__ inline void solveRow(SolveArrays sa, uint32_t k) {
         if (dynA) wA += IA(tq);
         if (dynB) wB -= IB(tq);
     }
-    if (dynA) { sa.vel[ia] = make_float4(vA.x, vA.y, vA.z, 0.0f); sa.ang[ia] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
-    if (dynB) { sa.vel[ib] = make_float4(vB.x, vB.y, vB.z, 0.0f); sa.ang[ib] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
+}
+
+// One contact on the compact global arrays.
+template <bool kCone>
+__device__ inline void solveRow(SolveArrays sa, uint32_t k) {
+    const uint4 meta = sa.meta[k];
+    if (meta.z & kSkip) return;
+    const uint32_t ia = meta.x, ib = meta.y;
+    RowRegs R;
+    for (int q = 0; q < 8; ++q) R.r[q] = sa.rows[8 * (size_t)k + q];
+    R.flags = meta.z;
+    V3 vA = xyz(sa.vel[ia]), wA = xyz(sa.ang[ia]), vB = xyz(sa.vel[ib]), wB = xyz(sa.ang[ib]);
+    float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
+    float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
+    solveRegs<kCone>(R, sa.frame[ia], sa.invI[ia], sa.frame[ib], sa.invI[ib], vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
+    if (meta.z & kDynA) { sa.vel[ia] = make_float4(vA.x, vA.y, vA.z, 0.0f); sa.ang[ia] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
+    if (meta.z & kDynB) { sa.vel[ib] = make_float4(vB.x, vB.y, vB.z, 0.0f); sa.ang[ib] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
     sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
     sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
 }
@@ -246,6 +258,304 @@ __global__ __launch_bounds__(kThreads) void kSweepColor(SolveArrays sa, const ui
     const uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
     const uint32_t stride = gridDim.x * kThreads;
     for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < count; k += stride) solveRow<kCone>(sa, first + k);
+}
+
+}  // namespace vxk
+
+namespace vxk {
+
+// Warm start of one contact on the compact arrays (narrowphase.h:80-124);
+// anchors come from the substep rows (same arithmetic as recomputing them).
+__device__ inline void warmRow(SolveArrays sa, uint32_t k) {
+    const uint4 meta = sa.meta[k];
+    const uint32_t flags = meta.z;
+    const float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
+    const float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
+    if ((flags & kSensorRow) || (cN <= 0.0f && cT1 == 0.0f && cT2 == 0.0f && cR1 == 0.0f && cR2 == 0.0f && cS == 0.0f))
+        return;
+    const uint32_t ia = meta.x, ib = meta.y;
+    const float4* row = sa.rows + 8 * (size_t)k;
+    const V3 ra = xyz(row[0]), rb = xyz(row[1]), n = xyz(row[2]), t1 = xyz(row[3]), t2 = xyz(row[4]);
+    const bool dynA = flags & kDynA, dynB = flags & kDynB, rotA = flags & kRotA, rotB = flags & kRotB;
+    const float4 fa = sa.frame[ia], fb = sa.frame[ib], iia = sa.invI[ia], iib = sa.invI[ib];
+    V3 imp = n * cN + t1 * cT1 + t2 * cT2;
+    V3 ang = t1 * cR1 + t2 * cR2 + n * cS;
+    if (dynA) {
+        V3 v = xyz(sa.vel[ia]) + imp * iia.w;
+        V3 w = xyz(sa.ang[ia]);
+        w += frameMul(fa, iia, cross(ra, imp), rotA);
+        w += frameMul(fa, iia, ang, rotA);
+        sa.vel[ia] = make_float4(v.x, v.y, v.z, 0.0f);
+        sa.ang[ia] = make_float4(w.x, w.y, w.z, 0.0f);
+    }
+    if (dynB) {
+        V3 v = xyz(sa.vel[ib]) - imp * iib.w;
+        V3 w = xyz(sa.ang[ib]);
+        w -= frameMul(fb, iib, cross(rb, imp), rotB);
+        w -= frameMul(fb, iib, ang, rotB);
+        sa.vel[ib] = make_float4(v.x, v.y, v.z, 0.0f);
+        sa.ang[ib] = make_float4(w.x, w.y, w.z, 0.0f);
+    }
+}
+
+// One colour of the contacts listed in idx[colorStart[c] .. colorStart[c+1]).
+template <int kMode>  // 0 = two-axis sweep, 1 = cone-block sweep, 2 = warm start
+__global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, const uint32_t* __restrict__ idx,
+                                                           const uint32_t* __restrict__ colorStart, uint32_t color) {
+    const uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
+    const uint32_t stride = gridDim.x * kThreads;
+    for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < count; j += stride) {
+        const uint32_t k = idx[first + j];
+        if (kMode == 2) warmRow(sa, k);
+        else solveRow<kMode == 1>(sa, k);
+    }
+}
+
+// --------------------------------------------------------------- islands --
+// Connected components of the contact graph over dynamic bodies
+// (the reference's buildIslands relation, solver.cpp:507-581).  Lock-free
+// union-find: hook the larger root under the smaller with compare-and-swap;
+// parent reads are device-coherent atomic loads.
+__device__ inline uint32_t ufFind(uint32_t* parent, uint32_t x) {
+    for (;;) {
+        uint32_t p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == x) return x;
+        uint32_t gp = __hip_atomic_load(&parent[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gp != p) __hip_atomic_store(&parent[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = p;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void kIslandUnion(const uint4* __restrict__ meta, uint32_t m, uint32_t* parent) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= m) return;
+    const uint4 mt = meta[k];
+    if ((mt.z & kSensorRow) || !(mt.z & kDynA) || !(mt.z & kDynB)) return;
+    uint32_t a = mt.x, b = mt.y;
+    for (;;) {
+        a = ufFind(parent, a);
+        b = ufFind(parent, b);
+        if (a == b) return;
+        if (a < b) { uint32_t t = a; a = b; b = t; }
+        uint32_t expected = a;
+        if (__hip_atomic_compare_exchange_strong(&parent[a], &expected, b, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+
+// Island of every contact (root of its dynamic body; kNone when it has none)
+// and the number of contacts per island.
+__global__ __launch_bounds__(kThreads) void kIslandOf(const uint4* __restrict__ meta, uint32_t m, uint32_t* parent,
+                                                      uint32_t* __restrict__ islandOf, uint32_t* __restrict__ size) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= m) return;
+    const uint4 mt = meta[k];
+    uint32_t root = kNone;
+    if (mt.z & kDynA) root = ufFind(parent, mt.x);
+    else if (mt.z & kDynB) root = ufFind(parent, mt.y);
+    islandOf[k] = root;
+    if (root != kNone) atomicAdd(&size[root], 1u);
+}
+
+constexpr uint32_t kTinyIsland = 64;
+
+// Splits contacts (in colour order) into small islands (sorted by island,
+// colour order kept) and the big-island list (colour order kept).
+__global__ __launch_bounds__(kThreads) void kIslandClassify(const uint32_t* __restrict__ islandOf,
+                                                            const uint32_t* __restrict__ size, uint32_t m,
+                                                            uint32_t limit, uint32_t* __restrict__ bigFlag,
+                                                            uint64_t* __restrict__ smallKeys,
+                                                            uint32_t* __restrict__ smallVals,
+                                                            uint32_t* __restrict__ smallCounter, int mb, int nb) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t root = k < m ? islandOf[k] : kNone;
+    const bool big = root != kNone && size[root] > limit;
+    const bool small = root != kNone && !big;
+    if (k < m) bigFlag[k] = big ? 1u : 0u;
+    uint32_t at = waveReserve(small ? 1u : 0u, smallCounter);
+    if (small) {
+        // Tiny islands (one wavefront's worth of contacts) sort first.
+        const uint64_t medium = size[root] > kTinyIsland ? 1u : 0u;
+        smallKeys[at] = (medium << (nb + mb)) | ((uint64_t)root << mb) | k;
+        smallVals[at] = k;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void kBigList(const uint32_t* __restrict__ bigFlag,
+                                                     const uint32_t* __restrict__ bigPos, uint32_t m,
+                                                     const uint32_t* __restrict__ color, uint32_t* __restrict__ bigIdx,
+                                                     uint32_t* __restrict__ bigColorCount) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= m || !bigFlag[k]) return;
+    bigIdx[bigPos[k]] = k;
+    atomicAdd(&bigColorCount[color[k]], 1u);
+}
+
+// Run heads of the island-sorted small list: a run is a maximal stretch of
+// one island and one colour.
+__global__ __launch_bounds__(kThreads) void kRunHeads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
+                                                      uint32_t ms, const uint32_t* __restrict__ color,
+                                                      uint32_t* __restrict__ runHead, uint32_t* __restrict__ islandHead,
+                                                      int mb, int nb, uint32_t* __restrict__ tinyIslands) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= ms) return;
+    bool newIsland = j == 0 || (keys[j] >> mb) != (keys[j - 1] >> mb);
+    bool newRun = newIsland || color[idx[j]] != color[idx[j - 1]];
+    runHead[j] = newRun ? 1u : 0u;
+    islandHead[j] = newIsland ? 1u : 0u;
+    if (newIsland && (keys[j] >> (nb + mb)) == 0) atomicAdd(tinyIslands, 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void kRunTables(const uint32_t* __restrict__ runHead,
+                                                       const uint32_t* __restrict__ islandHead,
+                                                       const uint32_t* __restrict__ runPos,
+                                                       const uint32_t* __restrict__ islandPos, uint32_t ms,
+                                                       uint32_t* __restrict__ runStart,
+                                                       uint32_t* __restrict__ islandFirstRun) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= ms) return;
+    if (runHead[j]) runStart[runPos[j]] = j;
+    if (islandHead[j]) islandFirstRun[islandPos[j]] = runPos[j];
+}
+
+__global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t ms, uint32_t* islandFirstRun,
+                              const uint32_t* islands, const uint32_t* tiny, uint32_t* ranges) {
+    runStart[*runs] = ms;
+    islandFirstRun[*islands] = *runs;
+    ranges[0] = 0; ranges[1] = *tiny;         // tiny islands: one wavefront each
+    ranges[2] = *tiny; ranges[3] = *islands;  // the rest: one workgroup each
+}
+
+// Small islands, one workgroup each: every sweep of every colour of the
+// island, barrier between colours (the island shares no dynamic body with
+// any other contact, so this is the colour order exactly).
+// Islands [first, last) of the island list, kBlock threads each (one
+// wavefront for tiny islands).
+template <int kMode, int kBlock>
+__global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const uint32_t* __restrict__ idx,
+                                                        const uint32_t* __restrict__ runStart,
+                                                        const uint32_t* __restrict__ islandFirstRun,
+                                                        const uint32_t* __restrict__ islandRange, int sweeps) {
+    const uint32_t first = islandRange[0], last = islandRange[1];
+    for (uint32_t isl = first + blockIdx.x; isl < last; isl += gridDim.x) {
+        const uint32_t r0 = islandFirstRun[isl], r1 = islandFirstRun[isl + 1];
+        for (int s = 0; s < sweeps; ++s)
+            for (uint32_t r = r0; r < r1; ++r) {
+                const uint32_t j0 = runStart[r], j1 = runStart[r + 1];
+                for (uint32_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
+                    const uint32_t k = idx[j];
+                    if (kMode == 2) warmRow(sa, k);
+                    else solveRow<kMode == 1>(sa, k);
+                }
+                __syncthreads();
+            }
+    }
+}
+
+
+// Tiny islands (at most 64 contacts), one wavefront each: every lane owns one
+// contact and keeps its rows and impulses in registers for all sweeps; the
+// island's bodies live in LDS (each body once, found by scanning the
+// wavefront's endpoint list), so a colour hand-off costs an LDS round trip
+// instead of global memory.  Same per-body update order as the colour sweep.
+template <int kMode>  // 0 two-axis, 1 cone-block, 2 warm start
+__global__ __launch_bounds__(64) void kTinyIslands(SolveArrays sa, const uint32_t* __restrict__ idx,
+                                                   const uint32_t* __restrict__ runStart,
+                                                   const uint32_t* __restrict__ islandFirstRun,
+                                                   const uint32_t* __restrict__ islandRange, int sweeps) {
+    __shared__ uint32_t bodyId[128];
+    __shared__ float4 bodyVel[128], bodyAng[128];
+    const uint32_t first = islandRange[0], last = islandRange[1];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t isl = first + blockIdx.x; isl < last; isl += gridDim.x) {
+        const uint32_t r0 = islandFirstRun[isl], r1 = islandFirstRun[isl + 1];
+        const uint32_t j0 = runStart[r0], j1 = runStart[r1];
+        const bool active = j0 + t < j1;
+        uint32_t k = 0, run = 0;
+        uint4 meta = make_uint4(kNone, kNone, kSkip, 0);
+        if (active) {
+            k = idx[j0 + t];
+            meta = sa.meta[k];
+            for (uint32_t r = r0; r < r1; ++r)
+                if (runStart[r] <= j0 + t) run = r - r0;
+        }
+        bodyId[2 * t] = active && (meta.z & kDynA) ? meta.x : kNone;
+        bodyId[2 * t + 1] = active && (meta.z & kDynB) ? meta.y : kNone;
+        __syncthreads();
+        // Canonical slot of each endpoint: the first slot holding the body.
+        uint32_t slotA = 2 * t, slotB = 2 * t + 1;
+        if (active) {
+            for (uint32_t q = 0; q < 2 * t + 2; ++q) {
+                if (q < slotA && bodyId[q] == bodyId[2 * t]) slotA = q;
+                if (q < slotB && bodyId[q] == bodyId[2 * t + 1]) slotB = q;
+            }
+            if (slotA == 2 * t && bodyId[slotA] != kNone) {
+                bodyVel[slotA] = sa.vel[meta.x];
+                bodyAng[slotA] = sa.ang[meta.x];
+            }
+            if (slotB == 2 * t + 1 && bodyId[slotB] != kNone) {
+                bodyVel[slotB] = sa.vel[meta.y];
+                bodyAng[slotB] = sa.ang[meta.y];
+            }
+        }
+        RowRegs R;
+        R.flags = meta.z;
+        float4 fa = make_float4(0, 0, 0, 1), fb = fa, iia = make_float4(0, 0, 0, 0), iib = iia;
+        float4 i0 = make_float4(0, 0, 0, 0), i1 = i0;
+        if (active) {
+            for (int q = 0; q < 8; ++q) R.r[q] = sa.rows[8 * (size_t)k + q];
+            fa = sa.frame[meta.x]; iia = sa.invI[meta.x];
+            fb = sa.frame[meta.y]; iib = sa.invI[meta.y];
+            i0 = sa.imp[2 * k]; i1 = sa.imp[2 * k + 1];
+        }
+        float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
+        // Static endpoints keep their (unchanging) global velocities.
+        const V3 vStaticA = active && !(meta.z & kDynA) ? xyz(sa.vel[meta.x]) : V3{};
+        const V3 wStaticA = active && !(meta.z & kDynA) ? xyz(sa.ang[meta.x]) : V3{};
+        const V3 vStaticB = active && !(meta.z & kDynB) ? xyz(sa.vel[meta.y]) : V3{};
+        const V3 wStaticB = active && !(meta.z & kDynB) ? xyz(sa.ang[meta.y]) : V3{};
+        __syncthreads();
+        const uint32_t runs = r1 - r0;
+        for (int sw = 0; sw < sweeps; ++sw)
+            for (uint32_t r = 0; r < runs; ++r) {
+                if (active && run == r) {
+                    const bool dynA = meta.z & kDynA, dynB = meta.z & kDynB;
+                    V3 vA = dynA ? xyz(bodyVel[slotA]) : vStaticA, wA = dynA ? xyz(bodyAng[slotA]) : wStaticA;
+                    V3 vB = dynB ? xyz(bodyVel[slotB]) : vStaticB, wB = dynB ? xyz(bodyAng[slotB]) : wStaticB;
+                    if (kMode == 2) {
+                        if (!(meta.z & kSensorRow) &&
+                            !(cN <= 0.0f && cT1 == 0.0f && cT2 == 0.0f && cR1 == 0.0f && cR2 == 0.0f && cS == 0.0f)) {
+                            const V3 ra = xyz(R.r[0]), rb = xyz(R.r[1]), n = xyz(R.r[2]), t1 = xyz(R.r[3]), t2 = xyz(R.r[4]);
+                            V3 imp = n * cN + t1 * cT1 + t2 * cT2;
+                            V3 ang = t1 * cR1 + t2 * cR2 + n * cS;
+                            if (dynA) {
+                                vA = vA + imp * iia.w;
+                                wA += frameMul(fa, iia, cross(ra, imp), meta.z & kRotA);
+                                wA += frameMul(fa, iia, ang, meta.z & kRotA);
+                            }
+                            if (dynB) {
+                                vB = vB - imp * iib.w;
+                                wB -= frameMul(fb, iib, cross(rb, imp), meta.z & kRotB);
+                                wB -= frameMul(fb, iib, ang, meta.z & kRotB);
+                            }
+                        }
+                    } else if (!(meta.z & kSkip)) {
+                        solveRegs<kMode == 1>(R, fa, iia, fb, iib, vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
+                    }
+                    if (dynA) { bodyVel[slotA] = make_float4(vA.x, vA.y, vA.z, 0.0f); bodyAng[slotA] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
+                    if (dynB) { bodyVel[slotB] = make_float4(vB.x, vB.y, vB.z, 0.0f); bodyAng[slotB] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
+                }
+                __syncthreads();
+            }
+        if (active) {
+            if (slotA == 2 * t && bodyId[slotA] != kNone) { sa.vel[meta.x] = bodyVel[slotA]; sa.ang[meta.x] = bodyAng[slotA]; }
+            if (slotB == 2 * t + 1 && bodyId[slotB] != kNone) { sa.vel[meta.y] = bodyVel[slotB]; sa.ang[meta.y] = bodyAng[slotB]; }
+            sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
+            sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
+        }
+        __syncthreads();
+    }
 }
 
 }  // namespace vxk

@@ -108,6 +108,13 @@ struct vxhip_ctx {
     Buf<uint8_t> ccdFlag;
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta;
+    // island partition of the colour-ordered contacts (one step)
+    Buf<uint32_t> uf, islandOf, islandSize, bigFlag, bigPos, bigIdx, bigColorCount, bigColorStart, colorSolve;
+    Buf<uint64_t> smallKeys, smallKeysTmp;
+    Buf<uint32_t> smallIdx, smallIdxTmp, runHead, islandHead, runPos, islandPos, runStart, islandFirstRun;
+    std::vector<uint32_t> bigColorStartHost;
+    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, tinyIslandCount = 0;
+    uint32_t islandLimit = 4096;
     Buf<uint32_t> ccdDone;
     Buf<float4> ccdGap;
     Buf<uint8_t> temp;
@@ -128,7 +135,7 @@ struct vxhip_ctx {
     std::vector<hipEvent_t> stageEv;     // one per stage boundary
     std::vector<hipEvent_t> launchEv;    // pairs around colour-solve launches
     uint32_t launchUsed = 0;
-    std::vector<uint32_t> launchContacts;
+    std::vector<uint64_t> launchContacts;
     vxhip_timings timings{};
 };
 
@@ -137,7 +144,7 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cCount = 24
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cTiny = 22, cRanges = 24, cCount = 28
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -167,7 +174,7 @@ void solveLaunchBegin(vxhip_ctx* c) {
     HIPCK(hipEventRecord(c->launchEv[c->launchUsed], c->stream));
 }
 
-void solveLaunchEnd(vxhip_ctx* c, uint32_t contacts) {
+void solveLaunchEnd(vxhip_ctx* c, uint64_t contacts) {
     if (!c->profile) return;
     HIPCK(hipEventRecord(c->launchEv[c->launchUsed + 1], c->stream));
     c->launchUsed += 2;
@@ -214,6 +221,10 @@ __global__ void kFill32(uint32_t* p, uint32_t n, uint32_t v) {
 __global__ void kIota(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i < n) p[i] = i;
+}
+__global__ void kColorOfSolve(const uint32_t* color, const uint32_t* order, uint32_t m, uint32_t* out) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m) out[k] = color[order[k]];
 }
 __global__ void kColorKeys(const uint32_t* color, uint32_t m, uint64_t* keys, uint32_t* vals) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
@@ -416,39 +427,115 @@ uint32_t maxColorSize(const std::vector<uint32_t>& start) {
     return mx;
 }
 
-void warmStartColors(vxhip_ctx* c, vx_contact* solve, uint32_t K) {
-    for (uint32_t col = 0; col < K; ++col) {
-        uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
-        if (cnt) kWarmStartColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col);
+
+// Island partition of the colour-ordered contacts (needs this step's row
+// flags): small islands are solved whole by one workgroup each, big ones by
+// per-colour launches over the big-island list.
+void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
+    hipStream_t s = c->stream;
+    uint32_t* k = c->counters.p;
+    const uint32_t n = c->n;
+    c->uf.reserve(n); c->islandSize.reserve(n);
+    c->islandOf.reserve(m); c->bigFlag.reserve(m); c->bigPos.reserve(m + 1); c->bigIdx.reserve(m);
+    c->bigColorCount.reserve(K + 1); c->bigColorStart.reserve(K + 2); c->colorSolve.reserve(m);
+    c->smallKeys.reserve(m); c->smallKeysTmp.reserve(m); c->smallIdx.reserve(m); c->smallIdxTmp.reserve(m);
+    c->runHead.reserve(m); c->islandHead.reserve(m); c->runPos.reserve(m + 1); c->islandPos.reserve(m + 1);
+    c->runStart.reserve(m + 1); c->islandFirstRun.reserve(m + 1);
+    kResetCounters<<<1, 64, 0, s>>>(k);
+    c->tinyIslandCount = 0;
+    kIota<<<blocks(n), kThreads, 0, s>>>(c->uf.p, n);
+    fill32(c, c->islandSize.p, n, 0);
+    fill32(c, c->bigColorCount.p, K + 1, 0);
+    kColorOfSolve<<<blocks(m), kThreads, 0, s>>>(c->color.p, c->order.p, m, c->colorSolve.p);
+    kIslandUnion<<<blocks(m), kThreads, 0, s>>>(sa.meta, m, c->uf.p);
+    kIslandOf<<<blocks(m), kThreads, 0, s>>>(sa.meta, m, c->uf.p, c->islandOf.p, c->islandSize.p);
+    const int mb = vxhost::bitsFor(m - 1) ? vxhost::bitsFor(m - 1) : 1;
+    kIslandClassify<<<blocks(m), kThreads, 0, s>>>(c->islandOf.p, c->islandSize.p, m, c->islandLimit, c->bigFlag.p,
+                                                   c->smallKeys.p, c->smallIdx.p, k + cSmall, mb, bodyBits(n));
+    HIPCK(vxhost::exclusiveScan(c->bigFlag.p, c->bigPos.p, m, k + cBig, tempFor(c, vxhost::scanTempBytes(m)), s));
+    kBigList<<<blocks(m), kThreads, 0, s>>>(c->bigFlag.p, c->bigPos.p, m, c->colorSolve.p, c->bigIdx.p,
+                                            c->bigColorCount.p);
+    HIPCK(vxhost::exclusiveScan(c->bigColorCount.p, c->bigColorStart.p, K, c->bigColorStart.p + K,
+                                tempFor(c, vxhost::scanTempBytes(K)), s));
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    const uint32_t ms = c->hostCounters[cSmall];
+    c->smallCount = ms;
+    c->bigCount = c->hostCounters[cBig];
+    c->bigColorStartHost.resize(K + 1);
+    HIPCK(hipMemcpyAsync(c->bigColorStartHost.data(), c->bigColorStart.p, (K + 1) * sizeof(uint32_t),
+                         hipMemcpyDeviceToHost, s));
+    c->islandCount = 0;
+    if (ms) {
+        HIPCK(vxhost::radixSort(c->smallKeys.p, c->smallIdx.p, c->smallKeysTmp.p, c->smallIdxTmp.p, ms,
+                                1 + bodyBits(n) + mb, tempFor(c, vxhost::radixTempBytes(ms)), s));
+        kRunHeads<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->smallIdx.p, ms, c->colorSolve.p, c->runHead.p,
+                                                  c->islandHead.p, mb, bodyBits(n), k + cTiny);
+        HIPCK(vxhost::exclusiveScan(c->runHead.p, c->runPos.p, ms, k + cRuns, tempFor(c, vxhost::scanTempBytes(ms)), s));
+        HIPCK(vxhost::exclusiveScan(c->islandHead.p, c->islandPos.p, ms, k + cIslands,
+                                    tempFor(c, vxhost::scanTempBytes(ms)), s));
+        kRunTables<<<blocks(ms), kThreads, 0, s>>>(c->runHead.p, c->islandHead.p, c->runPos.p, c->islandPos.p, ms,
+                                                   c->runStart.p, c->islandFirstRun.p);
+        kRunSentinels<<<1, 1, 0, s>>>(c->runStart.p, k + cRuns, ms, c->islandFirstRun.p, k + cIslands, k + cTiny,
+                                      k + cRanges);
+        HIPCK(hipGetLastError());
+        readCounters(c);
+        c->islandCount = c->hostCounters[cIslands];
+        c->tinyIslandCount = c->hostCounters[cTiny];
+    } else {
+        HIPCK(hipStreamSynchronize(s));
     }
+}
+
+// One pass (warm start, or 2*I sweeps) over every contact: small islands in
+// one launch, big islands colour by colour.
+template <int kMode>
+void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
+    hipStream_t s = c->stream;
+    const uint32_t tiny = c->tinyIslandCount, medium = c->islandCount - c->tinyIslandCount;
+    if (c->islandCount) solveLaunchBegin(c);
+    if (tiny)
+        kTinyIslands<kMode><<<std::min<uint32_t>(tiny, 16384), 64, 0, s>>>(
+            sa, c->smallIdx.p, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges, sweeps);
+    if (medium)
+        kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(medium, 8192), kThreads, 0, s>>>(
+            sa, c->smallIdx.p, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
+    if (c->islandCount) solveLaunchEnd(c, (uint64_t)c->smallCount * sweeps);
+    if (c->bigCount)
+        for (int it = 0; it < sweeps; ++it)
+            for (uint32_t col = 0; col < K; ++col) {
+                uint32_t cnt = c->bigColorStartHost[col + 1] - c->bigColorStartHost[col];
+                if (!cnt) continue;
+                solveLaunchBegin(c);
+                kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, s>>>(sa, c->bigIdx.p, c->bigColorStart.p, col);
+                solveLaunchEnd(c, cnt);
+            }
     HIPCK(hipGetLastError());
 }
 
-// One substep's velocity solve: compact body arrays, per-substep rows, then
-// 2*I sweeps of per-colour launches (backend_cuda.cu:661-683), velocities and
-// impulses written back.
-void solveSweeps(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx_solver_options& opt) {
+// One substep's velocity solve (backend_cuda.cu:652-689, 730-747): compact
+// body arrays, this substep's rows, warm start on the first substep, then
+// 2*I colour-ordered sweeps; velocities (and impulses after the last
+// substep) written back.
+void substepSolve(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx_solver_options& opt, bool first,
+                  bool last) {
     const uint32_t n = c->n, m = c->solveCount;
     hipStream_t s = c->stream;
     c->sVel.reserve(n); c->sAng.reserve(n); c->sInvI.reserve(n); c->sFrame.reserve(n);
     c->sRows.reserve(8 * (size_t)m); c->sImp.reserve(2 * (size_t)m); c->sMeta.reserve(m);
     SolveArrays sa{c->sVel.p, c->sAng.p, c->sInvI.p, c->sFrame.p, c->sRows.p, c->sMeta.p, c->sImp.p};
     kGatherSolveBodies<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, sa);
-    kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
+    if (first) kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
     kPrepareRows<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, solve, m, h, sa);
-    const bool cone = opt.frictionModel == VX_FRICTION_CONE_BLOCK;
+    if (first) {
+        prepareIslands(c, m, K, sa);
+        solvePass<2>(c, sa, K, 1);
+    }
     const int sweeps = 2 * opt.velocityIterations;
-    for (int it = 0; it < sweeps; ++it)
-        for (uint32_t col = 0; col < K; ++col) {
-            uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
-            if (!cnt) continue;
-            solveLaunchBegin(c);
-            if (cone) kSweepColor<true><<<blocks(cnt), kThreads, 0, s>>>(sa, c->colorStart.p, col);
-            else kSweepColor<false><<<blocks(cnt), kThreads, 0, s>>>(sa, c->colorStart.p, col);
-            solveLaunchEnd(c, cnt);
-        }
+    if (opt.frictionModel == VX_FRICTION_CONE_BLOCK) solvePass<1>(c, sa, K, sweeps);
+    else solvePass<0>(c, sa, K, sweeps);
     kScatterVelocities<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, sa);
-    kStoreImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
+    if (last) kStoreImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
     HIPCK(hipGetLastError());
 }
 
@@ -526,10 +613,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sSolve);
     for (int st = 0; st < S; ++st) {
         if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
-        if (m) {
-            if (st == 0) warmStartColors(c, c->solve.p, K);
-            solveSweeps(c, c->solve.p, K, h, cfg.options);
-        }
+        if (m) substepSolve(c, c->solve.p, K, h, cfg.options, st == 0, st == S - 1);
         kTransforms<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, h);
     }
     HIPCK(hipGetLastError());
@@ -558,6 +642,9 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->stats.ccdCandidates = c->hostCounters[cCcdExamined];
     c->stats.ccdRescued = c->hostCounters[cCcdRescued];
     c->stats.velocityIterations = c->lastIterations;
+    c->stats.islands = m ? c->islandCount : 0;
+    c->stats.islandContacts = m ? c->smallCount : 0;
+    c->stats.bigIslandContacts = m ? c->bigCount : 0;
 }
 
 template <class F>
@@ -627,6 +714,11 @@ void vxhip_destroy(vxhip_ctx* c) {
     c->temp.release(); c->counters.release(); c->ccdDone.release(); c->ccdGap.release();
     c->sVel.release(); c->sAng.release(); c->sInvI.release(); c->sFrame.release(); c->sRows.release(); c->sImp.release();
     c->sMeta.release();
+    for (Buf<uint32_t>* b : {&c->uf, &c->islandOf, &c->islandSize, &c->bigFlag, &c->bigPos, &c->bigIdx, &c->bigColorCount,
+                             &c->bigColorStart, &c->colorSolve, &c->smallIdx, &c->smallIdxTmp, &c->runHead,
+                             &c->islandHead, &c->runPos, &c->islandPos, &c->runStart, &c->islandFirstRun})
+        b->release();
+    c->smallKeys.release(); c->smallKeysTmp.release();
     for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
@@ -840,7 +932,6 @@ static void prepareSolve(vxhip_ctx* c, vx_contact* contacts, uint32_t m) {
     kGather<<<blocks(m), kThreads, 0, c->stream>>>(c->contacts.p, c->order.p, m, c->solve.p);
     c->numColors = K;
     c->solveCount = m;
-    warmStartColors(c, c->solve.p, K);
 }
 
 int vxhip_solve_velocities(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m, float dt,
@@ -853,7 +944,7 @@ int vxhip_solve_velocities(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact
         setBodies(c, bodies, n);
         if (warmStart) prepareSolve(c, contacts, m);
         else if (c->solveCount != m) fail(VXHIP_ERR_INVALID, "vxhip: solve without a warm-started contact set");
-        solveSweeps(c, c->solve.p, c->numColors, dt, *options);
+        substepSolve(c, c->solve.p, c->numColors, dt, *options, warmStart != 0, true);
         c->lastIterations = (uint32_t)(2 * options->velocityIterations);
         HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
         if (warmStart)
@@ -878,7 +969,7 @@ int vxhip_advance_substeps(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact
         const int S = substeps > 0 ? substeps : 1;
         for (int s = 0; s < S; ++s) {
             if (s > 0) kIntegrate<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, g, h);
-            if (m) solveSweeps(c, c->solve.p, c->numColors, h, *options);
+            if (m) substepSolve(c, c->solve.p, c->numColors, h, *options, s == 0, s == S - 1);
             kTransforms<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, h);
         }
         HIPCK(hipGetLastError());
