@@ -136,7 +136,7 @@ struct RowRegs {
 };
 
 template <bool kCone>
-__device__ inline void solveRegs(const RowRegs& R, float4 fa, float4 iia, float4 fb, float4 iib, V3& vA, V3& wA, V3& vB,
+__device__ __attribute__((always_inline)) inline void solveRegs(const RowRegs& R, float4 fa, float4 iia, float4 fb, float4 iib, V3& vA, V3& wA, V3& vB,
                                  V3& wB, float& cN, float& cT1, float& cT2, float& cR1, float& cR2, float& cS) {
     const uint32_t flags = R.flags;
     const float4 r0 = R.r[0], r1 = R.r[1], r2 = R.r[2], r3 = R.r[3], r4 = R.r[4], r5 = R.r[5], r6 = R.r[6], r7 = R.r[7];
@@ -235,7 +235,7 @@ __device__ inline void solveRegs(const RowRegs& R, float4 fa, float4 iia, float4
 
 // One contact on the compact global arrays.
 template <bool kCone>
-__device__ inline void solveRow(SolveArrays sa, uint32_t k) {
+__device__ __attribute__((always_inline)) inline void solveRow(SolveArrays sa, uint32_t k) {
     const uint4 meta = sa.meta[k];
     if (meta.z & kSkip) return;
     const uint32_t ia = meta.x, ib = meta.y;
@@ -266,7 +266,7 @@ namespace vxk {
 
 // Warm start of one contact on the compact arrays (narrowphase.h:80-124);
 // anchors come from the substep rows (same arithmetic as recomputing them).
-__device__ inline void warmRow(SolveArrays sa, uint32_t k) {
+__device__ __attribute__((always_inline)) inline void warmRow(SolveArrays sa, uint32_t k) {
     const uint4 meta = sa.meta[k];
     const uint32_t flags = meta.z;
     const float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
@@ -358,28 +358,124 @@ __global__ __launch_bounds__(kThreads) void kIslandOf(const uint4* __restrict__ 
     if (root != kNone) atomicAdd(&size[root], 1u);
 }
 
-constexpr uint32_t kTinyIsland = 64;
+// Islands of up to kLdsIsland contacts are packed, in root order, into
+// batches of about kBatch contacts (an island joins the batch its first
+// contact's offset falls in) whose dynamic bodies live in LDS while the batch
+// is solved.  A batch holds fewer than kBatch + kLdsIsland contacts and at
+// most kBatch islands, and an island of c contacts has at most c + 1 bodies,
+// so kLdsBodies slots always suffice.  Islands up to the big-island limit
+// get a batch of their own on global memory; bigger ones go colour by colour
+// over the whole device.
+constexpr uint32_t kBatch = 512;
+constexpr uint32_t kLdsIsland = 2048;
+constexpr uint32_t kLdsBodies = kBatch + kLdsIsland + kBatch;
 
-// Splits contacts (in colour order) into small islands (sorted by island,
-// colour order kept) and the big-island list (colour order kept).
+// Per island root: contact count if the island goes through LDS batches (0
+// otherwise) and a flag for the global-memory islands; counts islands.
+__global__ __launch_bounds__(kThreads) void kIslandKinds(const uint32_t* __restrict__ size, uint32_t n, uint32_t limit,
+                                                         uint32_t* __restrict__ ldsSize, uint32_t* __restrict__ globalFlag,
+                                                         uint32_t* __restrict__ islands) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t sz = i < n ? size[i] : 0u;
+    const bool lds = sz > 0 && sz <= kLdsIsland;
+    const bool global = sz > kLdsIsland && sz <= limit;
+    if (i < n) {
+        ldsSize[i] = lds ? sz : 0u;
+        globalFlag[i] = global ? 1u : 0u;
+    }
+    waveReserve(lds || global ? 1u : 0u, islands);
+}
+
+// Splits contacts (in colour order) into the batched list, keyed by
+// (batch, contact) so that a batch lists its contacts in colour order, and
+// the big-island list (colour order kept).  LDS batches take ids below
+// globalBase; global-memory islands follow, one batch each.
 __global__ __launch_bounds__(kThreads) void kIslandClassify(const uint32_t* __restrict__ islandOf,
-                                                            const uint32_t* __restrict__ size, uint32_t m,
-                                                            uint32_t limit, uint32_t* __restrict__ bigFlag,
+                                                            const uint32_t* __restrict__ size,
+                                                            const uint32_t* __restrict__ ldsOffset,
+                                                            const uint32_t* __restrict__ globalRank, uint32_t globalBase,
+                                                            uint32_t m, uint32_t limit, uint32_t* __restrict__ bigFlag,
                                                             uint64_t* __restrict__ smallKeys,
                                                             uint32_t* __restrict__ smallVals,
-                                                            uint32_t* __restrict__ smallCounter, int mb, int nb) {
+                                                            uint32_t* __restrict__ smallCounter, int mb) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     uint32_t root = k < m ? islandOf[k] : kNone;
-    const bool big = root != kNone && size[root] > limit;
+    const uint32_t sz = root != kNone ? size[root] : 0u;
+    const bool big = sz > limit;
     const bool small = root != kNone && !big;
     if (k < m) bigFlag[k] = big ? 1u : 0u;
     uint32_t at = waveReserve(small ? 1u : 0u, smallCounter);
     if (small) {
-        // Tiny islands (one wavefront's worth of contacts) sort first.
-        const uint64_t medium = size[root] > kTinyIsland ? 1u : 0u;
-        smallKeys[at] = (medium << (nb + mb)) | ((uint64_t)root << mb) | k;
+        const uint64_t batch = sz <= kLdsIsland ? ldsOffset[root] / kBatch : globalBase + globalRank[root];
+        smallKeys[at] = (batch << mb) | k;
         smallVals[at] = k;
     }
+}
+
+// Dynamic bodies of LDS-batched islands: rank within the island.
+__global__ __launch_bounds__(kThreads) void kBatchBodyCount(const vx_body* __restrict__ bodies, uint32_t n,
+                                                            uint32_t* parent, const uint32_t* __restrict__ ldsSize,
+                                                            uint32_t* __restrict__ bodyRoot,
+                                                            uint32_t* __restrict__ bodyRank,
+                                                            uint32_t* __restrict__ bodyCount) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    uint32_t root = kNone;
+    if (isDynamic(bodies[i])) {
+        root = ufFind(parent, i);
+        if (ldsSize[root] == 0) root = kNone;
+    }
+    bodyRoot[i] = root;
+    if (root != kNone) bodyRank[i] = atomicAdd(&bodyCount[root], 1u);
+}
+
+// Body range of each LDS batch (indexed by batch id).
+__global__ __launch_bounds__(kThreads) void kBatchBodyRange(const uint32_t* __restrict__ bodyCount,
+                                                            const uint32_t* __restrict__ bodyOffset,
+                                                            const uint32_t* __restrict__ ldsOffset, uint32_t n,
+                                                            uint32_t* __restrict__ rangeStart,
+                                                            uint32_t* __restrict__ rangeEnd) {
+    uint32_t r = blockIdx.x * kThreads + threadIdx.x;
+    if (r >= n || bodyCount[r] == 0) return;
+    const uint32_t b = ldsOffset[r] / kBatch;
+    atomicMin(&rangeStart[b], bodyOffset[r]);
+    atomicMax(&rangeEnd[b], bodyOffset[r] + bodyCount[r]);
+}
+
+// Body list of the batches and each body's LDS slot within its batch.
+__global__ __launch_bounds__(kThreads) void kBatchBodySlots(const uint32_t* __restrict__ bodyRoot,
+                                                            const uint32_t* __restrict__ bodyRank,
+                                                            const uint32_t* __restrict__ bodyOffset,
+                                                            const uint32_t* __restrict__ ldsOffset,
+                                                            const uint32_t* __restrict__ rangeStart, uint32_t n,
+                                                            uint32_t* __restrict__ bodyList, uint32_t* __restrict__ slotOf) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t root = bodyRoot[i];
+    if (root == kNone) return;
+    const uint32_t pos = bodyOffset[root] + bodyRank[i];
+    bodyList[pos] = i;
+    slotOf[i] = pos - rangeStart[ldsOffset[root] / kBatch];
+}
+
+// LDS slots of both endpoints of every batched contact (kNone's low half
+// where the endpoint is not a dynamic body of the batch).
+__global__ __launch_bounds__(kThreads) void kBatchContactSlots(const uint64_t* __restrict__ keys,
+                                                               const uint32_t* __restrict__ idx, uint32_t ms,
+                                                               const uint4* __restrict__ meta,
+                                                               const uint32_t* __restrict__ slotOf, uint32_t globalBase,
+                                                               int mb, uint32_t* __restrict__ slotPair) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= ms) return;
+    uint32_t sa = 0xffffu, sb = 0xffffu;
+    if ((keys[j] >> mb) < globalBase) {
+        const uint4 mt = meta[idx[j]];
+        if (!(mt.z & kSensorRow)) {
+            if (mt.z & kDynA) sa = slotOf[mt.x];
+            if (mt.z & kDynB) sb = slotOf[mt.y];
+        }
+    }
+    slotPair[j] = sa | (sb << 16);
 }
 
 __global__ __launch_bounds__(kThreads) void kBigList(const uint32_t* __restrict__ bigFlag,
@@ -392,46 +488,54 @@ __global__ __launch_bounds__(kThreads) void kBigList(const uint32_t* __restrict_
     atomicAdd(&bigColorCount[color[k]], 1u);
 }
 
-// Run heads of the island-sorted small list: a run is a maximal stretch of
-// one island and one colour.
+// Run heads of the batch-sorted list: a run is a maximal stretch of one
+// batch and one colour.  Counts the LDS batches.
 __global__ __launch_bounds__(kThreads) void kRunHeads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
                                                       uint32_t ms, const uint32_t* __restrict__ color,
-                                                      uint32_t* __restrict__ runHead, uint32_t* __restrict__ islandHead,
-                                                      int mb, int nb, uint32_t* __restrict__ tinyIslands) {
+                                                      uint32_t* __restrict__ runHead, uint32_t* __restrict__ batchHead,
+                                                      int mb, uint32_t globalBase, uint32_t* __restrict__ ldsBatches) {
     uint32_t j = blockIdx.x * kThreads + threadIdx.x;
     if (j >= ms) return;
-    bool newIsland = j == 0 || (keys[j] >> mb) != (keys[j - 1] >> mb);
-    bool newRun = newIsland || color[idx[j]] != color[idx[j - 1]];
+    bool newBatch = j == 0 || (keys[j] >> mb) != (keys[j - 1] >> mb);
+    bool newRun = newBatch || color[idx[j]] != color[idx[j - 1]];
     runHead[j] = newRun ? 1u : 0u;
-    islandHead[j] = newIsland ? 1u : 0u;
-    if (newIsland && (keys[j] >> (nb + mb)) == 0) atomicAdd(tinyIslands, 1u);
+    batchHead[j] = newBatch ? 1u : 0u;
+    if (newBatch && (keys[j] >> mb) < globalBase) atomicAdd(ldsBatches, 1u);
 }
 
-__global__ __launch_bounds__(kThreads) void kRunTables(const uint32_t* __restrict__ runHead,
-                                                       const uint32_t* __restrict__ islandHead,
+__global__ __launch_bounds__(kThreads) void kRunTables(const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ runHead,
+                                                       const uint32_t* __restrict__ batchHead,
                                                        const uint32_t* __restrict__ runPos,
-                                                       const uint32_t* __restrict__ islandPos, uint32_t ms,
+                                                       const uint32_t* __restrict__ batchPos, uint32_t ms, int mb,
+                                                       uint32_t globalBase, const uint32_t* __restrict__ rangeStart,
+                                                       const uint32_t* __restrict__ rangeEnd,
                                                        uint32_t* __restrict__ runStart,
-                                                       uint32_t* __restrict__ islandFirstRun) {
+                                                       uint32_t* __restrict__ batchFirstRun,
+                                                       uint2* __restrict__ batchBodies) {
     uint32_t j = blockIdx.x * kThreads + threadIdx.x;
     if (j >= ms) return;
     if (runHead[j]) runStart[runPos[j]] = j;
-    if (islandHead[j]) islandFirstRun[islandPos[j]] = runPos[j];
+    if (batchHead[j]) {
+        batchFirstRun[batchPos[j]] = runPos[j];
+        const uint32_t b = (uint32_t)(keys[j] >> mb);
+        batchBodies[batchPos[j]] = b < globalBase ? make_uint2(rangeStart[b], rangeEnd[b]) : make_uint2(0u, 0u);
+    }
 }
 
-__global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t ms, uint32_t* islandFirstRun,
-                              const uint32_t* islands, const uint32_t* tiny, uint32_t* ranges) {
+__global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t ms, uint32_t* batchFirstRun,
+                              const uint32_t* batches, const uint32_t* ldsBatches, uint32_t* ranges) {
     runStart[*runs] = ms;
-    islandFirstRun[*islands] = *runs;
-    ranges[0] = 0; ranges[1] = *tiny;         // tiny islands: one wavefront each
-    ranges[2] = *tiny; ranges[3] = *islands;  // the rest: one workgroup each
+    batchFirstRun[*batches] = *runs;
+    ranges[0] = 0; ranges[1] = *ldsBatches;         // LDS batches
+    ranges[2] = *ldsBatches; ranges[3] = *batches;  // global-memory islands
 }
 
-// Small islands, one workgroup each: every sweep of every colour of the
-// island, barrier between colours (the island shares no dynamic body with
-// any other contact, so this is the colour order exactly).
-// Islands [first, last) of the island list, kBlock threads each (one
-// wavefront for tiny islands).
+// Batches of small islands, one workgroup each: every sweep of every colour
+// present in the batch, barrier between colours.  The batch's islands share
+// no dynamic body with any other contact, and contacts of one colour share
+// none with each other, so this is the colour order exactly.  Batches
+// [first, last) of the batch list.
 template <int kMode, int kBlock>
 __global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const uint32_t* __restrict__ idx,
                                                         const uint32_t* __restrict__ runStart,
@@ -453,106 +557,213 @@ __global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const ui
     }
 }
 
+// ------------------------------------------------------ LDS batch sweeps --
+// Two lanes per contact: the even lane carries body A, the odd lane body B.
+// Each lane keeps its own body's velocity update and inertia products; the
+// quantities both sides need (relative velocity, impulses) are formed from
+// the same operands in the same order on both lanes after a lane swap, so
+// the two copies agree bit for bit and equal the one-lane sequence of
+// solveContact (narrowphase.h:183-365).  b -= x is computed as b + (-x),
+// which IEEE defines to be the same operation.
 
-// Tiny islands (at most 64 contacts), one wavefront each: every lane owns one
-// contact and keeps its rows and impulses in registers for all sweeps; the
-// island's bodies live in LDS (each body once, found by scanning the
-// wavefront's endpoint list), so a colour hand-off costs an LDS round trip
-// instead of global memory.  Same per-body update order as the colour sweep.
+__device__ inline float swapLane(float x) {  // lanes 2i <-> 2i+1
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+__device__ inline V3 swapLane(V3 v) { return {swapLane(v.x), swapLane(v.y), swapLane(v.z)}; }
+
 template <int kMode>  // 0 two-axis, 1 cone-block, 2 warm start
-__global__ __launch_bounds__(64) void kTinyIslands(SolveArrays sa, const uint32_t* __restrict__ idx,
-                                                   const uint32_t* __restrict__ runStart,
-                                                   const uint32_t* __restrict__ islandFirstRun,
-                                                   const uint32_t* __restrict__ islandRange, int sweeps) {
-    __shared__ uint32_t bodyId[128];
-    __shared__ float4 bodyVel[128], bodyAng[128];
-    const uint32_t first = islandRange[0], last = islandRange[1];
-    const uint32_t t = threadIdx.x;
-    for (uint32_t isl = first + blockIdx.x; isl < last; isl += gridDim.x) {
-        const uint32_t r0 = islandFirstRun[isl], r1 = islandFirstRun[isl + 1];
-        const uint32_t j0 = runStart[r0], j1 = runStart[r1];
-        const bool active = j0 + t < j1;
-        uint32_t k = 0, run = 0;
-        uint4 meta = make_uint4(kNone, kNone, kSkip, 0);
-        if (active) {
-            k = idx[j0 + t];
-            meta = sa.meta[k];
-            for (uint32_t r = r0; r < r1; ++r)
-                if (runStart[r] <= j0 + t) run = r - r0;
+__device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, uint32_t k, uint32_t slots, bool sideB,
+                                                              float* __restrict__ lv) {
+    const uint4 meta = sa.meta[k];
+    const uint32_t flags = meta.z;
+    if (kMode == 2 ? (flags & kSensorRow) != 0 : (flags & kSkip) != 0) return;
+    const uint32_t body = sideB ? meta.y : meta.x;
+    const bool dyn = flags & (sideB ? kDynB : kDynA);
+    const bool rot = flags & (sideB ? kRotB : kRotA);
+    const uint32_t slot = sideB ? slots >> 16 : slots & 0xffffu;
+    const float4* row = sa.rows + 8 * (size_t)k;
+    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4], r5 = row[5], r6 = row[6],
+                 r7 = row[7];
+    const float4 fr = sa.frame[body], ii = sa.invI[body];
+    const float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
+    V3 v, w;
+    if (dyn) {
+        v = V3{lv[slot], lv[kLdsBodies + slot], lv[2 * kLdsBodies + slot]};
+        w = V3{lv[3 * kLdsBodies + slot], lv[4 * kLdsBodies + slot], lv[5 * kLdsBodies + slot]};
+    } else {
+        v = xyz(sa.vel[body]);
+        w = xyz(sa.ang[body]);
+    }
+    const V3 r = sideB ? xyz(r1) : xyz(r0);
+    const V3 n = xyz(r2), t1 = xyz(r3), t2 = xyz(r4);
+    const float sim = ii.w;
+    auto I = [&](V3 x) { return frameMul(fr, ii, x, rot); };
+    // Adds (side A) or subtracts (side B) a linear and an angular change.
+    auto push = [&](V3 imp) {
+        if (!dyn) return;
+        const V3 dv = imp * sim;
+        const V3 dw = I(cross(r, imp));
+        v = v + (sideB ? V3{-dv.x, -dv.y, -dv.z} : dv);
+        w = w + (sideB ? V3{-dw.x, -dw.y, -dw.z} : dw);
+    };
+    auto pushAng = [&](V3 tq) {
+        if (!dyn) return;
+        const V3 dw = I(tq);
+        w = w + (sideB ? V3{-dw.x, -dw.y, -dw.z} : dw);
+    };
+    float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
+    if (kMode == 2) {
+        if (!(cN <= 0.0f && cT1 == 0.0f && cT2 == 0.0f && cR1 == 0.0f && cR2 == 0.0f && cS == 0.0f) && dyn) {
+            const V3 imp = n * cN + t1 * cT1 + t2 * cT2;
+            const V3 ang = t1 * cR1 + t2 * cR2 + n * cS;
+            const V3 dv = imp * sim;
+            v = v + (sideB ? V3{-dv.x, -dv.y, -dv.z} : dv);
+            const V3 dw1 = I(cross(r, imp));
+            w = w + (sideB ? V3{-dw1.x, -dw1.y, -dw1.z} : dw1);
+            const V3 dw2 = I(ang);
+            w = w + (sideB ? V3{-dw2.x, -dw2.y, -dw2.z} : dw2);
         }
-        bodyId[2 * t] = active && (meta.z & kDynA) ? meta.x : kNone;
-        bodyId[2 * t + 1] = active && (meta.z & kDynB) ? meta.y : kNone;
-        __syncthreads();
-        // Canonical slot of each endpoint: the first slot holding the body.
-        uint32_t slotA = 2 * t, slotB = 2 * t + 1;
-        if (active) {
-            for (uint32_t q = 0; q < 2 * t + 2; ++q) {
-                if (q < slotA && bodyId[q] == bodyId[2 * t]) slotA = q;
-                if (q < slotB && bodyId[q] == bodyId[2 * t + 1]) slotB = q;
+    } else {
+        const float kN = r0.w, k1 = r1.w, k2 = r2.w, targetBase = r3.w, lg = r4.w;
+        const float vn0 = r5.x, rest = r5.y, mu1 = r5.z, mu2 = r5.w;
+        const float rolling = r6.x, spinning = r6.y, kr1 = r6.z, kr2 = r6.w;
+        const float ks = r7.x, kN1 = r7.y, kN2 = r7.z, k12 = r7.w;
+        // Relative point velocity (A minus B), identical on both lanes.
+        auto relVel = [&]() {
+            const V3 mine = v + cross(w, r);
+            const V3 other = swapLane(mine);
+            return sideB ? other - mine : mine - other;
+        };
+        auto relAng = [&]() {
+            const V3 other = swapLane(w);
+            return sideB ? other - w : w - other;
+        };
+        const float vn = dot(relVel(), n);
+        float target = targetBase;
+        if (vn0 < -1.0f && (lg <= 0.0f || vn > 0.0f)) target = fmx(target, -rest * vn0);
+        if (kMode == 0) {
+            float jn = (target - vn) / kN;
+            const float nextN = fmx(cN + jn, 0.0f);
+            jn = nextN - cN;
+            cN = nextN;
+            push(n * jn);
+            if (k1 > 0.0f && k2 > 0.0f) {
+                const V3 rv = relVel();
+                const float o1 = cT1, o2 = cT2;
+                float n1 = o1 - dot(rv, t1) / k1;
+                float n2 = o2 - dot(rv, t2) / k2;
+                const float m1 = mu1 * cN, m2 = mu2 * cN;
+                if (m1 <= 0.0f) n1 = 0.0f;
+                if (m2 <= 0.0f) n2 = 0.0f;
+                const float el = (m1 > 0.0f ? n1 * n1 / (m1 * m1) : 0.0f) + (m2 > 0.0f ? n2 * n2 / (m2 * m2) : 0.0f);
+                if (el > 1.0f) { const float sc = 1.0f / sqrtf(el); n1 *= sc; n2 *= sc; }
+                cT1 = n1;
+                cT2 = n2;
+                push(t1 * (n1 - o1) + t2 * (n2 - o2));
             }
-            if (slotA == 2 * t && bodyId[slotA] != kNone) {
-                bodyVel[slotA] = sa.vel[meta.x];
-                bodyAng[slotA] = sa.ang[meta.x];
-            }
-            if (slotB == 2 * t + 1 && bodyId[slotB] != kNone) {
-                bodyVel[slotB] = sa.vel[meta.y];
-                bodyAng[slotB] = sa.ang[meta.y];
-            }
-        }
-        RowRegs R;
-        R.flags = meta.z;
-        float4 fa = make_float4(0, 0, 0, 1), fb = fa, iia = make_float4(0, 0, 0, 0), iib = iia;
-        float4 i0 = make_float4(0, 0, 0, 0), i1 = i0;
-        if (active) {
-            for (int q = 0; q < 8; ++q) R.r[q] = sa.rows[8 * (size_t)k + q];
-            fa = sa.frame[meta.x]; iia = sa.invI[meta.x];
-            fb = sa.frame[meta.y]; iib = sa.invI[meta.y];
-            i0 = sa.imp[2 * k]; i1 = sa.imp[2 * k + 1];
-        }
-        float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
-        // Static endpoints keep their (unchanging) global velocities.
-        const V3 vStaticA = active && !(meta.z & kDynA) ? xyz(sa.vel[meta.x]) : V3{};
-        const V3 wStaticA = active && !(meta.z & kDynA) ? xyz(sa.ang[meta.x]) : V3{};
-        const V3 vStaticB = active && !(meta.z & kDynB) ? xyz(sa.vel[meta.y]) : V3{};
-        const V3 wStaticB = active && !(meta.z & kDynB) ? xyz(sa.ang[meta.y]) : V3{};
-        __syncthreads();
-        const uint32_t runs = r1 - r0;
-        for (int sw = 0; sw < sweeps; ++sw)
-            for (uint32_t r = 0; r < runs; ++r) {
-                if (active && run == r) {
-                    const bool dynA = meta.z & kDynA, dynB = meta.z & kDynB;
-                    V3 vA = dynA ? xyz(bodyVel[slotA]) : vStaticA, wA = dynA ? xyz(bodyAng[slotA]) : wStaticA;
-                    V3 vB = dynB ? xyz(bodyVel[slotB]) : vStaticB, wB = dynB ? xyz(bodyAng[slotB]) : wStaticB;
-                    if (kMode == 2) {
-                        if (!(meta.z & kSensorRow) &&
-                            !(cN <= 0.0f && cT1 == 0.0f && cT2 == 0.0f && cR1 == 0.0f && cR2 == 0.0f && cS == 0.0f)) {
-                            const V3 ra = xyz(R.r[0]), rb = xyz(R.r[1]), n = xyz(R.r[2]), t1 = xyz(R.r[3]), t2 = xyz(R.r[4]);
-                            V3 imp = n * cN + t1 * cT1 + t2 * cT2;
-                            V3 ang = t1 * cR1 + t2 * cR2 + n * cS;
-                            if (dynA) {
-                                vA = vA + imp * iia.w;
-                                wA += frameMul(fa, iia, cross(ra, imp), meta.z & kRotA);
-                                wA += frameMul(fa, iia, ang, meta.z & kRotA);
-                            }
-                            if (dynB) {
-                                vB = vB - imp * iib.w;
-                                wB -= frameMul(fb, iib, cross(rb, imp), meta.z & kRotB);
-                                wB -= frameMul(fb, iib, ang, meta.z & kRotB);
-                            }
-                        }
-                    } else if (!(meta.z & kSkip)) {
-                        solveRegs<kMode == 1>(R, fa, iia, fb, iib, vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
-                    }
-                    if (dynA) { bodyVel[slotA] = make_float4(vA.x, vA.y, vA.z, 0.0f); bodyAng[slotA] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
-                    if (dynB) { bodyVel[slotB] = make_float4(vB.x, vB.y, vB.z, 0.0f); bodyAng[slotB] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
+        } else {
+            const float on = cN, o1 = cT1, o2 = cT2;
+            float nn = on, n1 = o1, n2 = o2;
+            if (k1 > 0.0f && k2 > 0.0f) {
+                const float det = kN * (k1 * k2 - k12 * k12) - kN1 * (kN1 * k2 - kN2 * k12) + kN2 * (kN1 * k12 - kN2 * k1);
+                if (fabsf(det) > 1e-12f) {
+                    const V3 rv = relVel();
+                    const float bn = target - dot(rv, n);
+                    const float b1 = -dot(rv, t1);
+                    const float b2 = -dot(rv, t2);
+                    const float dn = (bn * (k1 * k2 - k12 * k12) + b1 * (kN2 * k12 - kN1 * k2) + b2 * (kN1 * k12 - kN2 * k1)) / det;
+                    const float d1 = (bn * (kN2 * k12 - kN1 * k2) + b1 * (kN * k2 - kN2 * kN2) + b2 * (kN1 * kN2 - kN * k12)) / det;
+                    const float d2 = (bn * (kN1 * k12 - kN2 * k1) + b1 * (kN1 * kN2 - kN * k12) + b2 * (kN * k1 - kN1 * kN1)) / det;
+                    nn = fmx(on + dn, 0.0f);
+                    n1 = o1 + d1;
+                    n2 = o2 + d2;
                 }
-                __syncthreads();
             }
-        if (active) {
-            if (slotA == 2 * t && bodyId[slotA] != kNone) { sa.vel[meta.x] = bodyVel[slotA]; sa.ang[meta.x] = bodyAng[slotA]; }
-            if (slotB == 2 * t + 1 && bodyId[slotB] != kNone) { sa.vel[meta.y] = bodyVel[slotB]; sa.ang[meta.y] = bodyAng[slotB]; }
+            const float m1 = mu1 * nn, m2 = mu2 * nn;
+            if (m1 <= 0.0f) n1 = 0.0f;
+            if (m2 <= 0.0f) n2 = 0.0f;
+            const float el = (m1 > 0.0f ? n1 * n1 / (m1 * m1) : 0.0f) + (m2 > 0.0f ? n2 * n2 / (m2 * m2) : 0.0f);
+            if (el > 1.0f) { const float sc = 1.0f / sqrtf(el); n1 *= sc; n2 *= sc; }
+            cN = nn;
+            cT1 = n1;
+            cT2 = n2;
+            push(n * (nn - on) + t1 * (n1 - o1) + t2 * (n2 - o2));
+        }
+        if (rolling > 0.0f && kr1 > 0.0f && kr2 > 0.0f) {
+            const V3 wr = relAng();
+            const float o1 = cR1, o2 = cR2;
+            float n1 = o1 - dot(wr, t1) / kr1;
+            float n2 = o2 - dot(wr, t2) / kr2;
+            const float lim = rolling * cN;
+            const float mag = sqrtf(n1 * n1 + n2 * n2);
+            if (mag > lim && mag > 1e-9f) { const float sc = lim / mag; n1 *= sc; n2 *= sc; }
+            cR1 = n1;
+            cR2 = n2;
+            pushAng(t1 * (n1 - o1) + t2 * (n2 - o2));
+        }
+        if (spinning > 0.0f && ks > 0.0f) {
+            const V3 wr = relAng();
+            const float o = cS;
+            float nx = o - dot(wr, n) / ks;
+            const float lim = spinning * cN;
+            nx = fclamp(nx, -lim, lim);
+            cS = nx;
+            pushAng(n * (nx - o));
+        }
+        if (!sideB) {
             sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
             sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
+        }
+    }
+    if (dyn) {
+        lv[slot] = v.x; lv[kLdsBodies + slot] = v.y; lv[2 * kLdsBodies + slot] = v.z;
+        lv[3 * kLdsBodies + slot] = w.x; lv[4 * kLdsBodies + slot] = w.y; lv[5 * kLdsBodies + slot] = w.z;
+    }
+}
+
+// LDS batches, one workgroup each: the batch's dynamic bodies are loaded
+// into LDS, then every sweep of every colour present in the batch runs with
+// a barrier between colours (the batch's islands share no dynamic body with
+// any other contact, and contacts of one colour share none with each other,
+// so this is the colour order exactly), and the bodies are written back.
+template <int kMode>
+__global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ idx,
+                                                         const uint32_t* __restrict__ slotPair,
+                                                         const uint32_t* __restrict__ runStart,
+                                                         const uint32_t* __restrict__ batchFirstRun,
+                                                         const uint2* __restrict__ batchBodies,
+                                                         const uint32_t* __restrict__ bodyList,
+                                                         const uint32_t* __restrict__ batchRange, int sweeps,
+                                                         uint32_t* __restrict__ error) {
+    __shared__ float lv[6 * kLdsBodies];
+    const uint32_t first = batchRange[0], last = batchRange[1];
+    const uint32_t t = threadIdx.x;
+    const bool sideB = t & 1;
+    for (uint32_t q = first + blockIdx.x; q < last; q += gridDim.x) {
+        const uint2 bb = batchBodies[q];
+        const uint32_t nb = bb.y - bb.x;
+        if (nb > kLdsBodies) {  // excluded by the packing bounds; reported, never overrun
+            if (t == 0) atomicOr(error, 1u);
+            continue;
+        }
+        for (uint32_t i = t; i < nb; i += kThreads) {
+            const uint32_t body = bodyList[bb.x + i];
+            const float4 v = sa.vel[body], w = sa.ang[body];
+            lv[i] = v.x; lv[kLdsBodies + i] = v.y; lv[2 * kLdsBodies + i] = v.z;
+            lv[3 * kLdsBodies + i] = w.x; lv[4 * kLdsBodies + i] = w.y; lv[5 * kLdsBodies + i] = w.z;
+        }
+        __syncthreads();
+        const uint32_t r0 = batchFirstRun[q], r1 = batchFirstRun[q + 1];
+        for (int sw = 0; sw < sweeps; ++sw)
+            for (uint32_t r = r0; r < r1; ++r) {
+                const uint32_t j0 = runStart[r], j1 = runStart[r + 1];
+                for (uint32_t j = j0 + (t >> 1); j < j1; j += kThreads / 2) pairRow<kMode>(sa, idx[j], slotPair[j], sideB, lv);
+                __syncthreads();
+            }
+        for (uint32_t i = t; i < nb; i += kThreads) {
+            const uint32_t body = bodyList[bb.x + i];
+            sa.vel[body] = make_float4(lv[i], lv[kLdsBodies + i], lv[2 * kLdsBodies + i], 0.0f);
+            sa.ang[body] = make_float4(lv[3 * kLdsBodies + i], lv[4 * kLdsBodies + i], lv[5 * kLdsBodies + i], 0.0f);
         }
         __syncthreads();
     }

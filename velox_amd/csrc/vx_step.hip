@@ -109,11 +109,14 @@ struct vxhip_ctx {
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta;
     // island partition of the colour-ordered contacts (one step)
+    Buf<uint32_t> ldsSize, ldsOffset, globalFlag, globalRank, bodyRoot, bodyRank, bodyCount, bodyOffset, bodyList,
+        slotOf, rangeStart, rangeEnd, slotPair;
+    Buf<uint2> batchBodies;
     Buf<uint32_t> uf, islandOf, islandSize, bigFlag, bigPos, bigIdx, bigColorCount, bigColorStart, colorSolve;
     Buf<uint64_t> smallKeys, smallKeysTmp;
     Buf<uint32_t> smallIdx, smallIdxTmp, runHead, islandHead, runPos, islandPos, runStart, islandFirstRun;
     std::vector<uint32_t> bigColorStartHost;
-    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, tinyIslandCount = 0;
+    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, batchCount = 0, ldsBatchCount = 0;
     uint32_t islandLimit = 4096;
     Buf<uint32_t> ccdDone;
     Buf<float4> ccdGap;
@@ -144,7 +147,7 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cTiny = 22, cRanges = 24, cCount = 28
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cBatchError = 28, cCount = 32
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -204,9 +207,11 @@ void collectProfile(vxhip_ctx* c) {
     c->timings.steps += 1;
 }
 
-__global__ void kResetCounters(uint32_t* k) {
+// Resets the per-stage counters; the sticky error words from cBatchError on
+// are cleared only at the start of a step (keepSticky false).
+__global__ void kResetCounters(uint32_t* k, bool keepSticky = true) {
     uint32_t t = threadIdx.x;
-    if (t >= cCount) return;
+    if (t >= cCount || (keepSticky && t >= cBatchError)) return;
     uint32_t v = 0;
     if (t >= cBounds && t < cBounds + 3) v = 0x7fffffffu;       // min cell
     else if (t >= cBounds + 3 && t < cBounds + 6) v = 0x80000000u;  // max cell
@@ -435,23 +440,42 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
     hipStream_t s = c->stream;
     uint32_t* k = c->counters.p;
     const uint32_t n = c->n;
+    const uint32_t globalBase = m / kBatch + 1;  // above every LDS batch id
     c->uf.reserve(n); c->islandSize.reserve(n);
+    c->ldsSize.reserve(n); c->ldsOffset.reserve(n); c->globalFlag.reserve(n); c->globalRank.reserve(n);
+    c->bodyRoot.reserve(n); c->bodyRank.reserve(n); c->bodyCount.reserve(n); c->bodyOffset.reserve(n);
+    c->bodyList.reserve(n); c->slotOf.reserve(n); c->rangeStart.reserve(globalBase); c->rangeEnd.reserve(globalBase);
     c->islandOf.reserve(m); c->bigFlag.reserve(m); c->bigPos.reserve(m + 1); c->bigIdx.reserve(m);
     c->bigColorCount.reserve(K + 1); c->bigColorStart.reserve(K + 2); c->colorSolve.reserve(m);
     c->smallKeys.reserve(m); c->smallKeysTmp.reserve(m); c->smallIdx.reserve(m); c->smallIdxTmp.reserve(m);
     c->runHead.reserve(m); c->islandHead.reserve(m); c->runPos.reserve(m + 1); c->islandPos.reserve(m + 1);
-    c->runStart.reserve(m + 1); c->islandFirstRun.reserve(m + 1);
+    c->runStart.reserve(m + 1); c->islandFirstRun.reserve(m + 1); c->slotPair.reserve(m); c->batchBodies.reserve(m);
     kResetCounters<<<1, 64, 0, s>>>(k);
-    c->tinyIslandCount = 0;
     kIota<<<blocks(n), kThreads, 0, s>>>(c->uf.p, n);
     fill32(c, c->islandSize.p, n, 0);
+    fill32(c, c->bodyCount.p, n, 0);
+    fill32(c, c->rangeStart.p, globalBase, kNone);
+    fill32(c, c->rangeEnd.p, globalBase, 0);
     fill32(c, c->bigColorCount.p, K + 1, 0);
     kColorOfSolve<<<blocks(m), kThreads, 0, s>>>(c->color.p, c->order.p, m, c->colorSolve.p);
     kIslandUnion<<<blocks(m), kThreads, 0, s>>>(sa.meta, m, c->uf.p);
     kIslandOf<<<blocks(m), kThreads, 0, s>>>(sa.meta, m, c->uf.p, c->islandOf.p, c->islandSize.p);
+    kIslandKinds<<<blocks(n), kThreads, 0, s>>>(c->islandSize.p, n, c->islandLimit, c->ldsSize.p, c->globalFlag.p,
+                                                k + cIslands);
+    HIPCK(vxhost::exclusiveScan(c->ldsSize.p, c->ldsOffset.p, n, nullptr, tempFor(c, vxhost::scanTempBytes(n)), s));
+    HIPCK(vxhost::exclusiveScan(c->globalFlag.p, c->globalRank.p, n, nullptr, tempFor(c, vxhost::scanTempBytes(n)), s));
+    // LDS slots of the batched islands' dynamic bodies.
+    kBatchBodyCount<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->uf.p, c->ldsSize.p, c->bodyRoot.p,
+                                                   c->bodyRank.p, c->bodyCount.p);
+    HIPCK(vxhost::exclusiveScan(c->bodyCount.p, c->bodyOffset.p, n, nullptr, tempFor(c, vxhost::scanTempBytes(n)), s));
+    kBatchBodyRange<<<blocks(n), kThreads, 0, s>>>(c->bodyCount.p, c->bodyOffset.p, c->ldsOffset.p, n,
+                                                   c->rangeStart.p, c->rangeEnd.p);
+    kBatchBodySlots<<<blocks(n), kThreads, 0, s>>>(c->bodyRoot.p, c->bodyRank.p, c->bodyOffset.p, c->ldsOffset.p,
+                                                   c->rangeStart.p, n, c->bodyList.p, c->slotOf.p);
     const int mb = vxhost::bitsFor(m - 1) ? vxhost::bitsFor(m - 1) : 1;
-    kIslandClassify<<<blocks(m), kThreads, 0, s>>>(c->islandOf.p, c->islandSize.p, m, c->islandLimit, c->bigFlag.p,
-                                                   c->smallKeys.p, c->smallIdx.p, k + cSmall, mb, bodyBits(n));
+    kIslandClassify<<<blocks(m), kThreads, 0, s>>>(c->islandOf.p, c->islandSize.p, c->ldsOffset.p, c->globalRank.p,
+                                                   globalBase, m, c->islandLimit, c->bigFlag.p, c->smallKeys.p,
+                                                   c->smallIdx.p, k + cSmall, mb);
     HIPCK(vxhost::exclusiveScan(c->bigFlag.p, c->bigPos.p, m, k + cBig, tempFor(c, vxhost::scanTempBytes(m)), s));
     kBigList<<<blocks(m), kThreads, 0, s>>>(c->bigFlag.p, c->bigPos.p, m, c->colorSolve.p, c->bigIdx.p,
                                             c->bigColorCount.p);
@@ -462,26 +486,31 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
     const uint32_t ms = c->hostCounters[cSmall];
     c->smallCount = ms;
     c->bigCount = c->hostCounters[cBig];
+    c->islandCount = c->hostCounters[cIslands];
     c->bigColorStartHost.resize(K + 1);
     HIPCK(hipMemcpyAsync(c->bigColorStartHost.data(), c->bigColorStart.p, (K + 1) * sizeof(uint32_t),
                          hipMemcpyDeviceToHost, s));
-    c->islandCount = 0;
+    c->batchCount = c->ldsBatchCount = 0;
     if (ms) {
-        HIPCK(vxhost::radixSort(c->smallKeys.p, c->smallIdx.p, c->smallKeysTmp.p, c->smallIdxTmp.p, ms,
-                                1 + bodyBits(n) + mb, tempFor(c, vxhost::radixTempBytes(ms)), s));
+        const int bb = vxhost::bitsFor(globalBase + m);
+        HIPCK(vxhost::radixSort(c->smallKeys.p, c->smallIdx.p, c->smallKeysTmp.p, c->smallIdxTmp.p, ms, bb + mb,
+                                tempFor(c, vxhost::radixTempBytes(ms)), s));
+        kBatchContactSlots<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->smallIdx.p, ms, sa.meta, c->slotOf.p,
+                                                           globalBase, mb, c->slotPair.p);
         kRunHeads<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->smallIdx.p, ms, c->colorSolve.p, c->runHead.p,
-                                                  c->islandHead.p, mb, bodyBits(n), k + cTiny);
+                                                  c->islandHead.p, mb, globalBase, k + cLdsBatches);
         HIPCK(vxhost::exclusiveScan(c->runHead.p, c->runPos.p, ms, k + cRuns, tempFor(c, vxhost::scanTempBytes(ms)), s));
-        HIPCK(vxhost::exclusiveScan(c->islandHead.p, c->islandPos.p, ms, k + cIslands,
+        HIPCK(vxhost::exclusiveScan(c->islandHead.p, c->islandPos.p, ms, k + cBatches,
                                     tempFor(c, vxhost::scanTempBytes(ms)), s));
-        kRunTables<<<blocks(ms), kThreads, 0, s>>>(c->runHead.p, c->islandHead.p, c->runPos.p, c->islandPos.p, ms,
-                                                   c->runStart.p, c->islandFirstRun.p);
-        kRunSentinels<<<1, 1, 0, s>>>(c->runStart.p, k + cRuns, ms, c->islandFirstRun.p, k + cIslands, k + cTiny,
-                                      k + cRanges);
+        kRunTables<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->runHead.p, c->islandHead.p, c->runPos.p,
+                                                   c->islandPos.p, ms, mb, globalBase, c->rangeStart.p, c->rangeEnd.p,
+                                                   c->runStart.p, c->islandFirstRun.p, c->batchBodies.p);
+        kRunSentinels<<<1, 1, 0, s>>>(c->runStart.p, k + cRuns, ms, c->islandFirstRun.p, k + cBatches,
+                                      k + cLdsBatches, k + cRanges);
         HIPCK(hipGetLastError());
         readCounters(c);
-        c->islandCount = c->hostCounters[cIslands];
-        c->tinyIslandCount = c->hostCounters[cTiny];
+        c->batchCount = c->hostCounters[cBatches];
+        c->ldsBatchCount = c->hostCounters[cLdsBatches];
     } else {
         HIPCK(hipStreamSynchronize(s));
     }
@@ -492,15 +521,18 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
 template <int kMode>
 void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
     hipStream_t s = c->stream;
-    const uint32_t tiny = c->tinyIslandCount, medium = c->islandCount - c->tinyIslandCount;
-    if (c->islandCount) solveLaunchBegin(c);
-    if (tiny)
-        kTinyIslands<kMode><<<std::min<uint32_t>(tiny, 16384), 64, 0, s>>>(
-            sa, c->smallIdx.p, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges, sweeps);
-    if (medium)
-        kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(medium, 8192), kThreads, 0, s>>>(
-            sa, c->smallIdx.p, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
-    if (c->islandCount) solveLaunchEnd(c, (uint64_t)c->smallCount * sweeps);
+    if (c->batchCount) {
+        const uint32_t lds = c->ldsBatchCount, global = c->batchCount - c->ldsBatchCount;
+        solveLaunchBegin(c);
+        if (lds)
+            kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, s>>>(
+                sa, c->smallIdx.p, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
+                c->bodyList.p, c->counters.p + cRanges, sweeps, c->counters.p + cBatchError);
+        if (global)
+            kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, s>>>(
+                sa, c->smallIdx.p, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
+        solveLaunchEnd(c, (uint64_t)c->smallCount * sweeps);
+    }
     if (c->bigCount)
         for (int it = 0; it < sweeps; ++it)
             for (uint32_t col = 0; col < K; ++col) {
@@ -594,7 +626,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->prevPos.reserve(n); c->prevQ.reserve(n);
     uint32_t* k = c->counters.p;
     markStage(c, sBegin);
-    kResetCounters<<<1, 64, 0, s>>>(k);
+    kResetCounters<<<1, 64, 0, s>>>(k, false);
     kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
     HIPCK(hipGetLastError());
     markStage(c, sBroad);
@@ -634,6 +666,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sEnd);
     readCounters(c);
     if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: CCD dataflow did not converge");
+    if (c->hostCounters[cBatchError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: island batch exceeds its LDS capacity");
     collectProfile(c);
     c->stats.bodies = n;
     c->stats.pairs = np;

@@ -35,15 +35,18 @@ VXD V3& operator*=(V3& a, float s) { a.x *= s; a.y *= s; a.z *= s; return a; }
 
 // math.h:36-49: cross() orders its operands so a signed-zero result is canonical.
 VXD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-VXD float zeroSign(float v, int order) { return v != 0.0f ? v : (order > 0 ? 0.0f : -0.0f); }
+// The sign of an exactly-zero component is + when a > b lexicographically
+// (a component that differs decides; a NaN compares as not greater) or a == b,
+// and - otherwise.  Written with selects only: no branches in the hot loops.
 VXD V3 cross(V3 a, V3 b) {
-    int order = 1;
-    if (a.x != b.x) order = a.x > b.x ? 1 : -1;
-    else if (a.y != b.y) order = a.y > b.y ? 1 : -1;
-    else if (a.z != b.z) order = a.z > b.z ? 1 : -1;
-    return {zeroSign(a.y * b.z - a.z * b.y, order),
-            zeroSign(a.z * b.x - a.x * b.z, order),
-            zeroSign(a.x * b.y - a.y * b.x, order)};
+    const bool ex = a.x != b.x, ey = a.y != b.y, ez = a.z != b.z;
+    const bool lx = !(a.x > b.x), ly = !(a.y > b.y), lz = !(a.z > b.z);
+    const bool neg = (ex & lx) | (!ex & ey & ly) | (!ex & !ey & ez & lz);
+    const float zero = neg ? -0.0f : 0.0f;
+    const float x = a.y * b.z - a.z * b.y;
+    const float y = a.z * b.x - a.x * b.z;
+    const float z = a.x * b.y - a.y * b.x;
+    return {x != 0.0f ? x : zero, y != 0.0f ? y : zero, z != 0.0f ? z : zero};
 }
 VXD float lengthSq(V3 v) { return dot(v, v); }
 VXD float length(V3 v) { return sqrtf(lengthSq(v)); }
