@@ -94,7 +94,8 @@ def main():
         st = ctx.stats()
         print(f"[bench] step {done}/{args.steps} t={time.perf_counter() - t0:.2f}s contacts={st['contacts']} "
               f"colors={st['colors']} pairs={st['pairs']} ccd={st['ccdCandidates']}/{st['ccdRescued']} "
-              f"islands={st['islands']} island_contacts={st['islandContacts']} big={st['bigIslandContacts']}",
+              f"islands={st['islands']} island_contacts={st['islandContacts']} big={st['bigIslandContacts']} "
+              f"largest={st['largestIsland']} passes={st['maxBatchPasses']}",
               file=sys.stderr, flush=True)
     seconds = time.perf_counter() - t0
     last = ctx.stats()

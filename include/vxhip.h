@@ -66,7 +66,9 @@ typedef struct {
     uint32_t islands;          /* small contact islands solved whole by one workgroup */
     uint32_t islandContacts;   /* contacts in those islands */
     uint32_t bigIslandContacts;/* contacts solved colour by colour (islands above the size limit) */
-    uint32_t reserved[4];
+    uint32_t largestIsland;    /* contacts in the largest island */
+    uint32_t maxBatchPasses;   /* most solver passes (colour runs x chunks) one batch needs per sweep */
+    uint32_t reserved[2];
 } vxhip_stats;
 
 /* ---------------------------------------------------------------- setup -- */

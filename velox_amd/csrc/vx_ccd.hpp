@@ -184,17 +184,75 @@ __device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const 
 
 constexpr int kCcdThreads = 1024;
 
-// One (body << pb | pair) entry per dynamic body of every pair that produced
-// contacts; sorted, they give each pair its predecessor on each body.
-__global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restrict__ bodies,
+// Pairs that can see a rewound body.  A rescue only happens at a flagged
+// pair or at a pair one of whose dynamic bodies an earlier rescue moved, so
+// every pair that can do anything is flagged or follows (in pair order) such
+// a pair on a shared dynamic body.  The closure is built by alternating two
+// passes until nothing changes: each relevant pair lowers firstRel of its
+// dynamic bodies, and a pair behind a body's firstRel becomes relevant.
+// Every other pair sees exactly the state the parallel check saw.
+__global__ __launch_bounds__(kThreads) void kCcdFlagsToRel(const uint8_t* __restrict__ flag, uint32_t np,
+                                                           uint32_t* __restrict__ rel) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p < np) rel[p] = flag[p] ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void kCcdRelMark(const vx_body* __restrict__ bodies,
+                                                        const uint64_t* __restrict__ pairs,
+                                                        const uint32_t* __restrict__ rel, uint32_t np, int nb,
+                                                        uint32_t* __restrict__ firstRel) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= np || !rel[p]) return;
+    uint32_t i, j;
+    ccdBodies(bodies, pairs[p], nb, i, j);
+    if (isDynamic(bodies[i])) atomicMin(&firstRel[i], p);
+    if (isDynamic(bodies[j])) atomicMin(&firstRel[j], p);
+}
+
+__global__ __launch_bounds__(kThreads) void kCcdRelGrow(const vx_body* __restrict__ bodies,
                                                         const uint64_t* __restrict__ pairs,
                                                         const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                        const uint32_t* __restrict__ firstRel, uint32_t* __restrict__ rel,
+                                                        uint32_t* __restrict__ changed) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= np || rel[p] || !counts[p]) return;
+    uint32_t i, j;
+    ccdBodies(bodies, pairs[p], nb, i, j);
+    const bool behind = (isDynamic(bodies[i]) && firstRel[i] < p) || (isDynamic(bodies[j]) && firstRel[j] < p);
+    if (behind) {
+        rel[p] = 1u;
+        atomicAdd(changed, 1u);
+    }
+}
+
+// Fallback when the closure has not settled after a few rounds: every pair
+// that produced contacts.
+__global__ __launch_bounds__(kThreads) void kCcdRelAll(const uint32_t* __restrict__ counts, uint32_t np,
+                                                       uint32_t* __restrict__ rel) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p < np) rel[p] = counts[p] ? 1u : 0u;
+}
+
+// Relevant pairs listed in increasing pair order.
+__global__ __launch_bounds__(kThreads) void kCcdRelList(const uint32_t* __restrict__ rel,
+                                                        const uint32_t* __restrict__ relPos, uint32_t np,
+                                                        uint32_t* __restrict__ list) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p < np && rel[p]) list[relPos[p]] = p;
+}
+
+// One (body << pb | pair) entry per dynamic body of every relevant pair;
+// sorted, they give each pair its predecessor on each body.
+__global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restrict__ bodies,
+                                                        const uint64_t* __restrict__ pairs,
+                                                        const uint32_t* __restrict__ list, uint32_t nr, int nb,
                                                         int pbits, uint64_t* __restrict__ keys,
                                                         uint32_t* __restrict__ vals, uint32_t* __restrict__ counter) {
-    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
     bool s0 = false, s1 = false;
-    uint32_t i = 0, j = 0;
-    if (p < np && counts[p]) {
+    uint32_t i = 0, j = 0, p = 0;
+    if (t < nr) {
+        p = list[t];
         ccdBodies(bodies, pairs[p], nb, i, j);
         s0 = isDynamic(bodies[i]);
         s1 = isDynamic(bodies[j]);
@@ -204,32 +262,36 @@ __global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restric
     if (s1) { keys[at] = ((uint64_t)j << pbits) | p; vals[at] = (p << 1) | 1u; }
 }
 
-// The net in reference order by dataflow across workgroups: a pair waits for
-// the previous pairs on its dynamic bodies (one atomic word each: done bit and
-// a "moved" bit per body), re-examines itself only when flagged or when one of
-// its bodies was rewound earlier, and passes the moved bits on.  Body data is
-// published with release/acquire at device scope only along chains that
-// carry a move; all other hand-offs are relaxed.  Lanes walk their pairs in
-// increasing order and the grid is sized to be co-resident, so the lowest
-// unfinished pair is always running and ready; a spin budget bounds waits.
+// The net in reference order by dataflow across workgroups, over the
+// relevant pairs only: a pair waits for the previous relevant pairs on its
+// dynamic bodies (one atomic word each: done bit and a "moved" bit per body),
+// re-examines itself only when flagged or when one of its bodies was rewound
+// earlier, and passes the moved bits on.  Body data is published with
+// release/acquire at device scope only along chains that carry a move; all
+// other hand-offs are relaxed.  Consecutive list entries go to different
+// wavefronts (entry t to wavefront t mod W), because a rescue is long
+// divergent GJK work and the flagged pairs sit close together in the list;
+// each lane walks its entries in increasing order and the grid is sized to be
+// co-resident, so the lowest unfinished pair is always running and ready; a
+// spin budget bounds waits.
 constexpr int kCcdBlocks = 128;
 
 __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const vx_vec3* __restrict__ prevPos,
                                                          const vx_quat* __restrict__ prevQ, Soup soup, float dt,
                                                          const uint64_t* __restrict__ pairs,
-                                                         const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                         const uint32_t* __restrict__ list, uint32_t nr, int nb,
                                                          const uint8_t* __restrict__ flag,
                                                          const float4* __restrict__ endGap,
-                                                         const uint32_t* __restrict__ flagged,
                                                          const uint32_t* __restrict__ pred, uint32_t* word,
                                                          uint32_t* __restrict__ examined,
                                                          uint32_t* __restrict__ rescued, uint32_t* __restrict__ error) {
-    if (*flagged == 0) return;
-    const uint32_t stride = gridDim.x * kThreads;
-    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t waves = gridDim.x * (kThreads / 64);
+    const uint32_t wave = blockIdx.x * (kThreads / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint32_t stride = 64 * waves;
+    uint32_t t = wave + waves * lane;
     uint32_t spins = 0;
-    while (p < np) {
-        if (!counts[p]) { p += stride; continue; }
+    while (t < nr) {
+        const uint32_t p = list[t];
         const uint32_t pa = pred[2 * p], pb = pred[2 * p + 1];
         const uint32_t wa = pa == kNone ? 1u : __hip_atomic_load(&word[pa >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint32_t wb = pb == kNone ? 1u : __hip_atomic_load(&word[pb >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -248,7 +310,7 @@ __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const 
             const uint32_t out = 1u | ((movedA || r) ? 2u : 0u) | ((movedB || r) ? 4u : 0u);
             if (out != 1u) __hip_atomic_store(&word[p], out, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             else __hip_atomic_store(&word[p], out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            p += stride;
+            t += stride;
             spins = 0;
         } else if (++spins > (1u << 24)) {
             atomicOr(error, 1u);

@@ -19,9 +19,13 @@ struct SolveArrays {
     float4* ang;      // per body: angular velocity
     float4* invI;     // per body: inverse principal inertia (xyz), solver inverse mass (w)
     float4* frame;    // per body: principal-inertia frame at the current orientation
+    // Per-contact arrays are indexed by the contact's solve position: batched
+    // contacts in batch order, then the big-island contacts in colour order.
     float4* rows;     // per contact: 8 float4 (see kPrepareRows)
     uint4* meta;      // per contact: a, b, flags
     float4* imp;      // per contact: 2 float4 (normal, t1, t2, roll1 | roll2, spin)
+    float4* side;     // per contact: 2 x (frame, inverse inertia, linear, angular velocity) of A then B
+    const uint32_t* pos;  // contact (colour order) -> solve position
 };
 
 __device__ inline V3 frameMul(float4 frame, float4 invI, V3 v, bool active) {  // body.h:220-228
@@ -52,18 +56,22 @@ __global__ __launch_bounds__(kThreads) void kScatterVelocities(vx_body* __restri
     bodies[i].angularVelocity = {w.x, w.y, w.z};
 }
 
-__global__ __launch_bounds__(kThreads) void kLoadImpulses(const vx_contact* __restrict__ cs, uint32_t m, float4* imp) {
+__global__ __launch_bounds__(kThreads) void kLoadImpulses(const vx_contact* __restrict__ cs, uint32_t m, float4* imp,
+                                                          const uint32_t* __restrict__ pos) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= m) return;
     const vx_contact& c = cs[k];
-    imp[2 * k] = make_float4(c.normalImpulse, c.tangentImpulse1, c.tangentImpulse2, c.rollingImpulse1);
-    imp[2 * k + 1] = make_float4(c.rollingImpulse2, c.spinningImpulse, 0.0f, 0.0f);
+    const uint32_t p = pos[k];
+    imp[2 * p] = make_float4(c.normalImpulse, c.tangentImpulse1, c.tangentImpulse2, c.rollingImpulse1);
+    imp[2 * p + 1] = make_float4(c.rollingImpulse2, c.spinningImpulse, 0.0f, 0.0f);
 }
 
-__global__ __launch_bounds__(kThreads) void kStoreImpulses(vx_contact* __restrict__ cs, uint32_t m, const float4* imp) {
+__global__ __launch_bounds__(kThreads) void kStoreImpulses(vx_contact* __restrict__ cs, uint32_t m, const float4* imp,
+                                                           const uint32_t* __restrict__ pos) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= m) return;
-    float4 a = imp[2 * k], b = imp[2 * k + 1];
+    const uint32_t p = pos[k];
+    float4 a = imp[2 * p], b = imp[2 * p + 1];
     cs[k].normalImpulse = a.x; cs[k].tangentImpulse1 = a.y; cs[k].tangentImpulse2 = a.z; cs[k].rollingImpulse1 = a.w;
     cs[k].rollingImpulse2 = b.x; cs[k].spinningImpulse = b.y;
 }
@@ -74,20 +82,38 @@ __global__ __launch_bounds__(kThreads) void kStoreImpulses(vx_contact* __restric
 //   r0 = ra, kN        r1 = rb, k1          r2 = n, k2         r3 = t1, targetBase
 //   r4 = t2, liveGap   r5 = vn0, e, mu1, mu2                   r6 = roll, spin, kr1, kr2
 //   r7 = ks, kN1, kN2, k12
+__device__ inline uint32_t rowFlags(const vx_body& a, const vx_body& b) {
+    uint32_t flags = 0;
+    if (isDynamic(a)) flags |= kDynA;
+    if (isDynamic(b)) flags |= kDynB;
+    if (isDynamic(a) && !rotationLocked(a)) flags |= kRotA;
+    if (isDynamic(b) && !rotationLocked(b)) flags |= kRotB;
+    if (isSensor(a) || isSensor(b)) flags |= kSkip | kSensorRow;
+    return flags;
+}
+
+// Endpoints and body flags of every contact in colour order (the island
+// pass runs on these before solve positions exist).
+__global__ __launch_bounds__(kThreads) void kPrepareMeta(const vx_body* __restrict__ bodies,
+                                                         const vx_contact* __restrict__ cs, uint32_t m,
+                                                         uint4* __restrict__ meta) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t a = cs[k].a, b = cs[k].b;
+    meta[k] = make_uint4(a, b, rowFlags(bodies[a], bodies[b]), 0);
+}
+
 __global__ __launch_bounds__(kThreads) void kPrepareRows(const vx_body* __restrict__ bodies, vx_contact* __restrict__ cs,
                                                          uint32_t m, float h, SolveArrays sa) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= m) return;
     const vx_contact c = cs[k];
     const vx_body a = bodies[c.a], b = bodies[c.b];
-    uint32_t flags = 0;
-    if (isDynamic(a)) flags |= kDynA;
-    if (isDynamic(b)) flags |= kDynB;
-    if (isDynamic(a) && !rotationLocked(a)) flags |= kRotA;
-    if (isDynamic(b) && !rotationLocked(b)) flags |= kRotB;
-    float4* row = sa.rows + 8 * (size_t)k;
-    if (isSensor(a) || isSensor(b)) {
-        sa.meta[k] = make_uint4(c.a, c.b, flags | kSkip | kSensorRow, 0);
+    uint32_t flags = rowFlags(a, b);
+    const uint32_t p = sa.pos[k];
+    float4* row = sa.rows + 8 * (size_t)p;
+    if (flags & kSensorRow) {
+        sa.meta[p] = make_uint4(c.a, c.b, flags, 0);
         return;
     }
     const float4 fa = sa.frame[c.a], fb = sa.frame[c.b], ia = sa.invI[c.a], ib = sa.invI[c.b];
@@ -123,7 +149,10 @@ __global__ __launch_bounds__(kThreads) void kPrepareRows(const vx_body* __restri
     row[5] = make_float4(c.vn0, c.restitution, c.friction1, c.friction2);
     row[6] = make_float4(c.rollingFriction, c.spinningFriction, kr1, kr2);
     row[7] = make_float4(ks, kN1, kN2, k12);
-    sa.meta[k] = make_uint4(c.a, c.b, flags, 0);
+    sa.meta[p] = make_uint4(c.a, c.b, flags, 0);
+    float4* sd = sa.side + 8 * (size_t)p;
+    sd[0] = fa; sd[1] = ia; sd[2] = sa.vel[c.a]; sd[3] = sa.ang[c.a];
+    sd[4] = fb; sd[5] = ib; sd[6] = sa.vel[c.b]; sd[7] = sa.ang[c.b];
 }
 
 __device__ inline V3 xyz(float4 v) { return {v.x, v.y, v.z}; }
@@ -298,14 +327,15 @@ __device__ __attribute__((always_inline)) inline void warmRow(SolveArrays sa, ui
     }
 }
 
-// One colour of the contacts listed in idx[colorStart[c] .. colorStart[c+1]).
+// One colour of the big-island contacts: solve positions
+// base + colorStart[c] .. base + colorStart[c+1].
 template <int kMode>  // 0 = two-axis sweep, 1 = cone-block sweep, 2 = warm start
-__global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, const uint32_t* __restrict__ idx,
+__global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, uint32_t base,
                                                            const uint32_t* __restrict__ colorStart, uint32_t color) {
     const uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
     const uint32_t stride = gridDim.x * kThreads;
     for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < count; j += stride) {
-        const uint32_t k = idx[first + j];
+        const uint32_t k = base + first + j;
         if (kMode == 2) warmRow(sa, k);
         else solveRow<kMode == 1>(sa, k);
     }
@@ -374,9 +404,10 @@ constexpr uint32_t kLdsBodies = kBatch + kLdsIsland + kBatch;
 // otherwise) and a flag for the global-memory islands; counts islands.
 __global__ __launch_bounds__(kThreads) void kIslandKinds(const uint32_t* __restrict__ size, uint32_t n, uint32_t limit,
                                                          uint32_t* __restrict__ ldsSize, uint32_t* __restrict__ globalFlag,
-                                                         uint32_t* __restrict__ islands) {
+                                                         uint32_t* __restrict__ islands, uint32_t* __restrict__ largest) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     const uint32_t sz = i < n ? size[i] : 0u;
+    if (sz) atomicMax(largest, sz);
     const bool lds = sz > 0 && sz <= kLdsIsland;
     const bool global = sz > kLdsIsland && sz <= limit;
     if (i < n) {
@@ -488,6 +519,16 @@ __global__ __launch_bounds__(kThreads) void kBigList(const uint32_t* __restrict_
     atomicAdd(&bigColorCount[color[k]], 1u);
 }
 
+// Solve position of every contact: batched contacts at their place in the
+// batch-sorted list, big-island contacts after them in colour order.
+__global__ __launch_bounds__(kThreads) void kSolvePositions(const uint32_t* __restrict__ idx, uint32_t ms,
+                                                            const uint32_t* __restrict__ bigIdx, uint32_t mb,
+                                                            uint32_t* __restrict__ pos) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j < ms) pos[idx[j]] = j;
+    else if (j < ms + mb) pos[bigIdx[j - ms]] = j;
+}
+
 // Run heads of the batch-sorted list: a run is a maximal stretch of one
 // batch and one colour.  Counts the LDS batches.
 __global__ __launch_bounds__(kThreads) void kRunHeads(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ idx,
@@ -523,6 +564,20 @@ __global__ __launch_bounds__(kThreads) void kRunTables(const uint64_t* __restric
     }
 }
 
+// Solver passes per sweep of each batch (a run of L contacts takes
+// ceil(L / (kThreads/2)) passes); the maximum over batches.
+__global__ __launch_bounds__(kThreads) void kBatchPasses(const uint32_t* __restrict__ runStart,
+                                                         const uint32_t* __restrict__ batchFirstRun,
+                                                         const uint32_t* __restrict__ batches,
+                                                         uint32_t* __restrict__ maxPasses) {
+    uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= *batches) return;
+    uint32_t passes = 0;
+    for (uint32_t r = batchFirstRun[q]; r < batchFirstRun[q + 1]; ++r)
+        passes += (runStart[r + 1] - runStart[r] + kThreads / 2 - 1) / (kThreads / 2);
+    atomicMax(maxPasses, passes);
+}
+
 __global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t ms, uint32_t* batchFirstRun,
                               const uint32_t* batches, const uint32_t* ldsBatches, uint32_t* ranges) {
     runStart[*runs] = ms;
@@ -537,8 +592,7 @@ __global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t
 // none with each other, so this is the colour order exactly.  Batches
 // [first, last) of the batch list.
 template <int kMode, int kBlock>
-__global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const uint32_t* __restrict__ idx,
-                                                        const uint32_t* __restrict__ runStart,
+__global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const uint32_t* __restrict__ runStart,
                                                         const uint32_t* __restrict__ islandFirstRun,
                                                         const uint32_t* __restrict__ islandRange, int sweeps) {
     const uint32_t first = islandRange[0], last = islandRange[1];
@@ -547,8 +601,7 @@ __global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const ui
         for (int s = 0; s < sweeps; ++s)
             for (uint32_t r = r0; r < r1; ++r) {
                 const uint32_t j0 = runStart[r], j1 = runStart[r + 1];
-                for (uint32_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
-                    const uint32_t k = idx[j];
+                for (uint32_t k = j0 + threadIdx.x; k < j1; k += kBlock) {
                     if (kMode == 2) warmRow(sa, k);
                     else solveRow<kMode == 1>(sa, k);
                 }
@@ -571,28 +624,49 @@ __device__ inline float swapLane(float x) {  // lanes 2i <-> 2i+1
 }
 __device__ inline V3 swapLane(V3 v) { return {swapLane(v.x), swapLane(v.y), swapLane(v.z)}; }
 
+// What one lane needs of one contact, all addressed by its solve position
+// (loaded one pass ahead of its use).
+struct PairItem {
+    uint4 meta;
+    uint32_t slots;
+    float4 r[8];
+    float4 frame, invI, vel, ang;  // this lane's body
+    float4 i0, i1;                 // accumulated impulses
+};
+
+__device__ __attribute__((always_inline)) inline PairItem loadPairItem(SolveArrays sa, const uint32_t* slotPair,
+                                                                      uint32_t j, bool sideB) {
+    PairItem it;
+    it.meta = sa.meta[j];
+    it.slots = slotPair[j];
+    const float4* row = sa.rows + 8 * (size_t)j;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) it.r[q] = row[q];
+    const float4* sd = sa.side + 8 * (size_t)j + (sideB ? 4 : 0);
+    it.frame = sd[0]; it.invI = sd[1]; it.vel = sd[2]; it.ang = sd[3];
+    it.i0 = sa.imp[2 * j];
+    it.i1 = sa.imp[2 * j + 1];
+    return it;
+}
+
 template <int kMode>  // 0 two-axis, 1 cone-block, 2 warm start
-__device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, uint32_t k, uint32_t slots, bool sideB,
+__device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, uint32_t j, PairItem& it, bool sideB,
                                                               float* __restrict__ lv) {
-    const uint4 meta = sa.meta[k];
-    const uint32_t flags = meta.z;
+    const uint32_t flags = it.meta.z;
     if (kMode == 2 ? (flags & kSensorRow) != 0 : (flags & kSkip) != 0) return;
-    const uint32_t body = sideB ? meta.y : meta.x;
     const bool dyn = flags & (sideB ? kDynB : kDynA);
     const bool rot = flags & (sideB ? kRotB : kRotA);
-    const uint32_t slot = sideB ? slots >> 16 : slots & 0xffffu;
-    const float4* row = sa.rows + 8 * (size_t)k;
-    const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4], r5 = row[5], r6 = row[6],
-                 r7 = row[7];
-    const float4 fr = sa.frame[body], ii = sa.invI[body];
-    const float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
+    const uint32_t slot = sideB ? it.slots >> 16 : it.slots & 0xffffu;
+    const float4 r0 = it.r[0], r1 = it.r[1], r2 = it.r[2], r3 = it.r[3], r4 = it.r[4], r5 = it.r[5], r6 = it.r[6],
+                 r7 = it.r[7];
+    const float4 fr = it.frame, ii = it.invI;
     V3 v, w;
     if (dyn) {
         v = V3{lv[slot], lv[kLdsBodies + slot], lv[2 * kLdsBodies + slot]};
         w = V3{lv[3 * kLdsBodies + slot], lv[4 * kLdsBodies + slot], lv[5 * kLdsBodies + slot]};
     } else {
-        v = xyz(sa.vel[body]);
-        w = xyz(sa.ang[body]);
+        v = xyz(it.vel);
+        w = xyz(it.ang);
     }
     const V3 r = sideB ? xyz(r1) : xyz(r0);
     const V3 n = xyz(r2), t1 = xyz(r3), t2 = xyz(r4);
@@ -611,7 +685,7 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
         const V3 dw = I(tq);
         w = w + (sideB ? V3{-dw.x, -dw.y, -dw.z} : dw);
     };
-    float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
+    float cN = it.i0.x, cT1 = it.i0.y, cT2 = it.i0.z, cR1 = it.i0.w, cR2 = it.i1.x, cS = it.i1.y;
     if (kMode == 2) {
         if (!(cN <= 0.0f && cT1 == 0.0f && cT2 == 0.0f && cR1 == 0.0f && cR2 == 0.0f && cS == 0.0f) && dyn) {
             const V3 imp = n * cN + t1 * cT1 + t2 * cT2;
@@ -710,10 +784,12 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
             cS = nx;
             pushAng(n * (nx - o));
         }
-        if (!sideB) {
-            sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
-            sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
-        }
+        // Both lanes store the (identical) impulses, so each lane's later
+        // reload of this contact follows its own store.
+        it.i0 = make_float4(cN, cT1, cT2, cR1);
+        it.i1 = make_float4(cR2, cS, 0.0f, 0.0f);
+        sa.imp[2 * j] = it.i0;
+        sa.imp[2 * j + 1] = it.i1;
     }
     if (dyn) {
         lv[slot] = v.x; lv[kLdsBodies + slot] = v.y; lv[2 * kLdsBodies + slot] = v.z;
@@ -726,9 +802,13 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
 // a barrier between colours (the batch's islands share no dynamic body with
 // any other contact, and contacts of one colour share none with each other,
 // so this is the colour order exactly), and the bodies are written back.
+// A pass is up to kThreads/2 contacts of one colour; each lane loads its
+// next pass's contact while it solves the current one.  A lane meets the
+// same contact again one sweep later, which is the next pass only when a
+// sweep is a single pass: then the impulses it just computed are carried
+// over instead of the prefetched (older) ones.
 template <int kMode>
-__global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ idx,
-                                                         const uint32_t* __restrict__ slotPair,
+__global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ slotPair,
                                                          const uint32_t* __restrict__ runStart,
                                                          const uint32_t* __restrict__ batchFirstRun,
                                                          const uint2* __restrict__ batchBodies,
@@ -739,6 +819,7 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
     const uint32_t first = batchRange[0], last = batchRange[1];
     const uint32_t t = threadIdx.x;
     const bool sideB = t & 1;
+    constexpr uint32_t kPass = kThreads / 2;
     for (uint32_t q = first + blockIdx.x; q < last; q += gridDim.x) {
         const uint2 bb = batchBodies[q];
         const uint32_t nb = bb.y - bb.x;
@@ -752,14 +833,35 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
             lv[i] = v.x; lv[kLdsBodies + i] = v.y; lv[2 * kLdsBodies + i] = v.z;
             lv[3 * kLdsBodies + i] = w.x; lv[4 * kLdsBodies + i] = w.y; lv[5 * kLdsBodies + i] = w.z;
         }
-        __syncthreads();
         const uint32_t r0 = batchFirstRun[q], r1 = batchFirstRun[q + 1];
-        for (int sw = 0; sw < sweeps; ++sw)
-            for (uint32_t r = r0; r < r1; ++r) {
-                const uint32_t j0 = runStart[r], j1 = runStart[r + 1];
-                for (uint32_t j = j0 + (t >> 1); j < j1; j += kThreads / 2) pairRow<kMode>(sa, idx[j], slotPair[j], sideB, lv);
-                __syncthreads();
+        // Pass cursor: sweep sw, run r, chunk start base, run end end.
+        int sw = 0;
+        uint32_t r = r0, base = runStart[r0], end = runStart[r0 + 1];
+        uint32_t j = base + (t >> 1);
+        PairItem cur;
+        if (j < end) cur = loadPairItem(sa, slotPair, j, sideB);
+        __syncthreads();
+        for (;;) {
+            int nsw = sw;
+            uint32_t nr = r, nbase = base + kPass, nend = end;
+            const bool runDone = nbase >= end;
+            if (runDone) {
+                nr = r + 1;
+                if (nr == r1) { nr = r0; ++nsw; }
+                nbase = runStart[nr];
+                nend = runStart[nr + 1];
             }
+            const bool more = nsw < sweeps;
+            const uint32_t nj = nbase + (t >> 1);
+            PairItem nxt;
+            if (more && nj < nend) nxt = loadPairItem(sa, slotPair, nj, sideB);
+            if (j < end) pairRow<kMode>(sa, j, cur, sideB, lv);
+            if (runDone) __syncthreads();
+            if (!more) break;
+            if (nj == j && kMode != 2) { nxt.i0 = cur.i0; nxt.i1 = cur.i1; }
+            sw = nsw; r = nr; base = nbase; end = nend; j = nj;
+            cur = nxt;
+        }
         for (uint32_t i = t; i < nb; i += kThreads) {
             const uint32_t body = bodyList[bb.x + i];
             sa.vel[body] = make_float4(lv[i], lv[kLdsBodies + i], lv[2 * kLdsBodies + i], 0.0f);

@@ -73,6 +73,10 @@ struct Buf {
         p = nullptr;
         cap = 0;
     }
+    Buf() = default;
+    Buf(const Buf&) = delete;
+    Buf& operator=(const Buf&) = delete;
+    ~Buf() { release(); }
 };
 
 inline uint32_t blocks(uint64_t n, int t = kThreads) { return (uint32_t)((n + t - 1) / t); }
@@ -106,8 +110,11 @@ struct vxhip_ctx {
     Buf<unsigned long long> maskAfter;
     Buf<int> nextAfter;
     Buf<uint8_t> ccdFlag;
+    Buf<uint32_t> ccdRel, ccdRelPos, ccdList, ccdFirstRel;
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
-    Buf<uint4> sMeta;
+    Buf<uint4> sMeta, sMetaK;
+    Buf<float4> sSide;
+    Buf<uint32_t> solvePos;
     // island partition of the colour-ordered contacts (one step)
     Buf<uint32_t> ldsSize, ldsOffset, globalFlag, globalRank, bodyRoot, bodyRank, bodyCount, bodyOffset, bodyList,
         slotOf, rangeStart, rangeEnd, slotPair;
@@ -116,7 +123,8 @@ struct vxhip_ctx {
     Buf<uint64_t> smallKeys, smallKeysTmp;
     Buf<uint32_t> smallIdx, smallIdxTmp, runHead, islandHead, runPos, islandPos, runStart, islandFirstRun;
     std::vector<uint32_t> bigColorStartHost;
-    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, batchCount = 0, ldsBatchCount = 0;
+    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, batchCount = 0, ldsBatchCount = 0, largestIsland = 0,
+             maxBatchPasses = 0;
     uint32_t islandLimit = 4096;
     Buf<uint32_t> ccdDone;
     Buf<float4> ccdGap;
@@ -147,7 +155,7 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cBatchError = 28, cCount = 32
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cCount = 40
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -207,11 +215,11 @@ void collectProfile(vxhip_ctx* c) {
     c->timings.steps += 1;
 }
 
-// Resets the per-stage counters; the sticky error words from cBatchError on
-// are cleared only at the start of a step (keepSticky false).
+// Resets the per-stage counters; the sticky error word cBatchError is
+// cleared only at the start of a step (keepSticky false).
 __global__ void kResetCounters(uint32_t* k, bool keepSticky = true) {
     uint32_t t = threadIdx.x;
-    if (t >= cCount || (keepSticky && t >= cBatchError)) return;
+    if (t >= cCount || (keepSticky && t == cBatchError)) return;
     uint32_t v = 0;
     if (t >= cBounds && t < cBounds + 3) v = 0x7fffffffu;       // min cell
     else if (t >= cBounds + 3 && t < cBounds + 6) v = 0x80000000u;  // max cell
@@ -436,7 +444,7 @@ uint32_t maxColorSize(const std::vector<uint32_t>& start) {
 // Island partition of the colour-ordered contacts (needs this step's row
 // flags): small islands are solved whole by one workgroup each, big ones by
 // per-colour launches over the big-island list.
-void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
+void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
     hipStream_t s = c->stream;
     uint32_t* k = c->counters.p;
     const uint32_t n = c->n;
@@ -458,10 +466,10 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
     fill32(c, c->rangeEnd.p, globalBase, 0);
     fill32(c, c->bigColorCount.p, K + 1, 0);
     kColorOfSolve<<<blocks(m), kThreads, 0, s>>>(c->color.p, c->order.p, m, c->colorSolve.p);
-    kIslandUnion<<<blocks(m), kThreads, 0, s>>>(sa.meta, m, c->uf.p);
-    kIslandOf<<<blocks(m), kThreads, 0, s>>>(sa.meta, m, c->uf.p, c->islandOf.p, c->islandSize.p);
+    kIslandUnion<<<blocks(m), kThreads, 0, s>>>(meta, m, c->uf.p);
+    kIslandOf<<<blocks(m), kThreads, 0, s>>>(meta, m, c->uf.p, c->islandOf.p, c->islandSize.p);
     kIslandKinds<<<blocks(n), kThreads, 0, s>>>(c->islandSize.p, n, c->islandLimit, c->ldsSize.p, c->globalFlag.p,
-                                                k + cIslands);
+                                                k + cIslands, k + cLargest);
     HIPCK(vxhost::exclusiveScan(c->ldsSize.p, c->ldsOffset.p, n, nullptr, tempFor(c, vxhost::scanTempBytes(n)), s));
     HIPCK(vxhost::exclusiveScan(c->globalFlag.p, c->globalRank.p, n, nullptr, tempFor(c, vxhost::scanTempBytes(n)), s));
     // LDS slots of the batched islands' dynamic bodies.
@@ -495,7 +503,7 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
         const int bb = vxhost::bitsFor(globalBase + m);
         HIPCK(vxhost::radixSort(c->smallKeys.p, c->smallIdx.p, c->smallKeysTmp.p, c->smallIdxTmp.p, ms, bb + mb,
                                 tempFor(c, vxhost::radixTempBytes(ms)), s));
-        kBatchContactSlots<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->smallIdx.p, ms, sa.meta, c->slotOf.p,
+        kBatchContactSlots<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->smallIdx.p, ms, meta, c->slotOf.p,
                                                            globalBase, mb, c->slotPair.p);
         kRunHeads<<<blocks(ms), kThreads, 0, s>>>(c->smallKeys.p, c->smallIdx.p, ms, c->colorSolve.p, c->runHead.p,
                                                   c->islandHead.p, mb, globalBase, k + cLdsBatches);
@@ -507,12 +515,17 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, SolveArrays sa) {
                                                    c->runStart.p, c->islandFirstRun.p, c->batchBodies.p);
         kRunSentinels<<<1, 1, 0, s>>>(c->runStart.p, k + cRuns, ms, c->islandFirstRun.p, k + cBatches,
                                       k + cLdsBatches, k + cRanges);
+        kBatchPasses<<<blocks(ms), kThreads, 0, s>>>(c->runStart.p, c->islandFirstRun.p, k + cLdsBatches, k + cPasses);
         HIPCK(hipGetLastError());
-        readCounters(c);
+    }
+    kSolvePositions<<<blocks(m), kThreads, 0, s>>>(c->smallIdx.p, ms, c->bigIdx.p, c->bigCount, c->solvePos.p);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    c->largestIsland = c->hostCounters[cLargest];
+    c->maxBatchPasses = ms ? c->hostCounters[cPasses] : 0;
+    if (ms) {
         c->batchCount = c->hostCounters[cBatches];
         c->ldsBatchCount = c->hostCounters[cLdsBatches];
-    } else {
-        HIPCK(hipStreamSynchronize(s));
     }
 }
 
@@ -526,11 +539,11 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
         solveLaunchBegin(c);
         if (lds)
             kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, s>>>(
-                sa, c->smallIdx.p, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
+                sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
                 c->bodyList.p, c->counters.p + cRanges, sweeps, c->counters.p + cBatchError);
         if (global)
             kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, s>>>(
-                sa, c->smallIdx.p, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
+                sa, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
         solveLaunchEnd(c, (uint64_t)c->smallCount * sweeps);
     }
     if (c->bigCount)
@@ -539,7 +552,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 uint32_t cnt = c->bigColorStartHost[col + 1] - c->bigColorStartHost[col];
                 if (!cnt) continue;
                 solveLaunchBegin(c);
-                kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, s>>>(sa, c->bigIdx.p, c->bigColorStart.p, col);
+                kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, s>>>(sa, c->smallCount, c->bigColorStart.p, col);
                 solveLaunchEnd(c, cnt);
             }
     HIPCK(hipGetLastError());
@@ -554,20 +567,24 @@ void substepSolve(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx
     const uint32_t n = c->n, m = c->solveCount;
     hipStream_t s = c->stream;
     c->sVel.reserve(n); c->sAng.reserve(n); c->sInvI.reserve(n); c->sFrame.reserve(n);
-    c->sRows.reserve(8 * (size_t)m); c->sImp.reserve(2 * (size_t)m); c->sMeta.reserve(m);
-    SolveArrays sa{c->sVel.p, c->sAng.p, c->sInvI.p, c->sFrame.p, c->sRows.p, c->sMeta.p, c->sImp.p};
+    c->sRows.reserve(8 * (size_t)m); c->sImp.reserve(2 * (size_t)m); c->sMeta.reserve(m); c->sMetaK.reserve(m);
+    c->sSide.reserve(8 * (size_t)m); c->solvePos.reserve(m);
+    SolveArrays sa{c->sVel.p, c->sAng.p, c->sInvI.p, c->sFrame.p, c->sRows.p, c->sMeta.p, c->sImp.p, c->sSide.p,
+                   c->solvePos.p};
     kGatherSolveBodies<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, sa);
-    if (first) kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
-    kPrepareRows<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, solve, m, h, sa);
     if (first) {
-        prepareIslands(c, m, K, sa);
-        solvePass<2>(c, sa, K, 1);
+        // Islands, batches and solve positions for this step's contact set.
+        kPrepareMeta<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, solve, m, c->sMetaK.p);
+        prepareIslands(c, m, K, c->sMetaK.p);
+        kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p, c->solvePos.p);
     }
+    kPrepareRows<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, solve, m, h, sa);
+    if (first) solvePass<2>(c, sa, K, 1);
     const int sweeps = 2 * opt.velocityIterations;
     if (opt.frictionModel == VX_FRICTION_CONE_BLOCK) solvePass<1>(c, sa, K, sweeps);
     else solvePass<0>(c, sa, K, sweeps);
     kScatterVelocities<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, sa);
-    if (last) kStoreImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p);
+    if (last) kStoreImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p, c->solvePos.p);
     HIPCK(hipGetLastError());
 }
 
@@ -584,7 +601,8 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     if (np == 0) return;
     hipStream_t s = c->stream;
     uint32_t* k = c->counters.p;
-    const int nb = bodyBits(c->n);
+    const uint32_t n = c->n;
+    const int nb = bodyBits(n);
     const int pbits = vxhost::bitsFor(np - 1) ? vxhost::bitsFor(np - 1) : 1;
     c->ccdFlag.reserve(np);
     c->ccdGap.reserve(np);
@@ -593,13 +611,34 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     HIPCK(hipGetLastError());
     readCounters(c);
     if (c->hostCounters[cCcdFlagged] == 0) return;
-    c->keys.reserve(2 * (size_t)np); c->keysTmp.reserve(2 * (size_t)np);
-    c->colorVals.reserve(2 * (size_t)np); c->colorVals2.reserve(2 * (size_t)np);
+    // Pairs that can see a rewound body (flagged pairs and what follows them
+    // on shared dynamic bodies).
+    c->ccdRel.reserve(np); c->ccdRelPos.reserve(np); c->ccdList.reserve(np); c->ccdFirstRel.reserve(n);
+    kCcdFlagsToRel<<<blocks(np), kThreads, 0, s>>>(c->ccdFlag.p, np, c->ccdRel.p);
+    fill32(c, c->ccdFirstRel.p, n, kNone);
+    bool settled = false;
+    for (int round = 0; round < 8 && !settled; ++round) {
+        kCcdRelMark<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdRel.p, np, nb, c->ccdFirstRel.p);
+        fill32(c, k + cCcdChanged, 1, 0);
+        kCcdRelGrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, c->ccdFirstRel.p,
+                                                    c->ccdRel.p, k + cCcdChanged);
+        HIPCK(hipGetLastError());
+        readCounters(c);
+        settled = c->hostCounters[cCcdChanged] == 0;
+    }
+    if (!settled) kCcdRelAll<<<blocks(np), kThreads, 0, s>>>(c->counts.p, np, c->ccdRel.p);
+    HIPCK(vxhost::exclusiveScan(c->ccdRel.p, c->ccdRelPos.p, np, k + cCcdRelevant, tempFor(c, vxhost::scanTempBytes(np)), s));
+    kCcdRelList<<<blocks(np), kThreads, 0, s>>>(c->ccdRel.p, c->ccdRelPos.p, np, c->ccdList.p);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    const uint32_t nr = c->hostCounters[cCcdRelevant];
+    c->keys.reserve(2 * (size_t)nr); c->keysTmp.reserve(2 * (size_t)nr);
+    c->colorVals.reserve(2 * (size_t)nr); c->colorVals2.reserve(2 * (size_t)nr);
     c->pred.reserve(2 * (size_t)np);
     c->ccdDone.reserve(np);
     fill32(c, c->pred.p, 2 * np, kNone);
     fill32(c, c->ccdDone.p, np, 0);
-    kCcdEntries<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, pbits, c->keys.p,
+    kCcdEntries<<<blocks(nr), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdList.p, nr, nb, pbits, c->keys.p,
                                                 c->colorVals.p, k + cEntries);
     HIPCK(hipGetLastError());
     readCounters(c);
@@ -607,9 +646,9 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + pbits,
                             tempFor(c, vxhost::radixTempBytes(e)), s));
     if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, pbits, c->pred.p);
-    const uint32_t grid = std::min<uint32_t>(blocks(np), kCcdBlocks);
-    kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->counts.p,
-                                           np, nb, c->ccdFlag.p, c->ccdGap.p, k + cCcdFlagged, c->pred.p, c->ccdDone.p,
+    const uint32_t grid = std::min<uint32_t>(blocks(nr, kThreads / 64), kCcdBlocks);  // one entry per wavefront first
+    if (nr) kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->ccdList.p,
+                                           nr, nb, c->ccdFlag.p, c->ccdGap.p, c->pred.p, c->ccdDone.p,
                                            k + cCcdExamined, k + cCcdRescued, k + cColorError);
     HIPCK(hipGetLastError());
 }
@@ -678,6 +717,8 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->stats.islands = m ? c->islandCount : 0;
     c->stats.islandContacts = m ? c->smallCount : 0;
     c->stats.bigIslandContacts = m ? c->bigCount : 0;
+    c->stats.largestIsland = m ? c->largestIsland : 0;
+    c->stats.maxBatchPasses = m ? c->maxBatchPasses : 0;
 }
 
 template <class F>
@@ -737,21 +778,7 @@ void vxhip_destroy(vxhip_ctx* c) {
     if (!c) return;
     hipError_t e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) std::fprintf(stderr, "vxhip cleanup: %s\n", hipGetErrorString(e));
-    c->bodies.release(); c->vertices.release(); c->hullPts.release(); c->indices.release(); c->triRefs.release();
-    c->hullFaces.release(); c->meshes.release(); c->nodes.release(); c->children.release(); c->prevPos.release();
-    c->prevQ.release(); c->lo.release(); c->hi.release(); c->keys.release(); c->keysTmp.release(); c->idx.release();
-    c->idxTmp.release(); c->oversize.release(); c->pairs.release(); c->pairVals.release(); c->counts.release();
-    c->offsets.release(); c->slots.release(); c->contacts.release(); c->solve.release(); c->prev.release();
-    c->colorVals.release(); c->colorVals2.release(); c->pred.release(); c->color.release(); c->colorCount.release();
-    c->order.release(); c->colorStart.release(); c->maskAfter.release(); c->nextAfter.release(); c->ccdFlag.release();
-    c->temp.release(); c->counters.release(); c->ccdDone.release(); c->ccdGap.release();
-    c->sVel.release(); c->sAng.release(); c->sInvI.release(); c->sFrame.release(); c->sRows.release(); c->sImp.release();
-    c->sMeta.release();
-    for (Buf<uint32_t>* b : {&c->uf, &c->islandOf, &c->islandSize, &c->bigFlag, &c->bigPos, &c->bigIdx, &c->bigColorCount,
-                             &c->bigColorStart, &c->colorSolve, &c->smallIdx, &c->smallIdxTmp, &c->runHead,
-                             &c->islandHead, &c->runPos, &c->islandPos, &c->runStart, &c->islandFirstRun})
-        b->release();
-    c->smallKeys.release(); c->smallKeysTmp.release();
+    // Device buffers are released by their destructors below.
     for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);

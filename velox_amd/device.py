@@ -31,7 +31,8 @@ class Stats(C.Structure):
     _fields_ = [("bodies", C.c_uint32), ("pairs", C.c_uint32), ("contacts", C.c_uint32), ("colors", C.c_uint32),
                 ("ccdCandidates", C.c_uint32), ("ccdRescued", C.c_uint32), ("velocityIterations", C.c_uint32),
                 ("oversize", C.c_uint32), ("cellSize", C.c_float), ("islands", C.c_uint32),
-                ("islandContacts", C.c_uint32), ("bigIslandContacts", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("islandContacts", C.c_uint32), ("bigIslandContacts", C.c_uint32),
+                ("largestIsland", C.c_uint32), ("maxBatchPasses", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
