@@ -30,12 +30,26 @@ HBM_PEAK_GBS = 8000.0
 SOLVE_BYTES_PER_VISIT = 96 + 24 + 2 * (28 + 32 + 24 + 24)
 
 
+def measured_traffic(kernel, bodies, steps):
+    """HBM bytes per launch of `kernel` on this workload, as measured by the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE passes (profiles/traffic.json),
+    or None when no such measurement exists for this configuration."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get(f"{kernel}|debris{bodies}|{steps}")
+    return None if entry is None else entry["bytes_per_launch"]
+
+
 def run_cpu_baseline(bodies, steps, workers):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
         return None
-    out = subprocess.run([exe, "bench", f"debris{bodies}", str(steps), str(workers), "1"], capture_output=True,
-                         text=True, timeout=900)
+    env = dict(os.environ, VXREF_PROGRESS="20")  # progress lines on stderr while the CPU runs
+    out = subprocess.run([exe, "bench", f"debris{bodies}", str(steps), str(workers), "1"], stdout=subprocess.PIPE,
+                         text=True, timeout=900, env=env)
     if out.returncode != 0:
         return None
     r = json.loads(out.stdout.strip().splitlines()[-1])
@@ -52,7 +66,7 @@ def main():
     ap.add_argument("--steps", type=int, default=120)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--bodies", type=int, default=50000)
-    ap.add_argument("--cpu-steps", type=int, default=30, help="reference CPU sample length (steps)")
+    ap.add_argument("--cpu-steps", type=int, default=120, help="reference CPU sample length (steps)")
     ap.add_argument("--cpu-workers", type=int, default=16, help="reference CPU worker threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
@@ -73,7 +87,7 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    from velox_amd.device import Context, Scene
+    from velox_amd.device import Context, Scene, Timings
     scene = Scene("debris", args.bodies)
     n = len(scene.bodies)
     ctx = Context()
@@ -103,6 +117,7 @@ def main():
 
     roofline = None
     timings = None
+    kernels = None
     if not args.no_profile:
         # Same trajectory again with per-launch HIP events on the library's stream.
         ctx.load_scene(scene)
@@ -111,16 +126,28 @@ def main():
         ctx.sync()
         timings = ctx.timings()
         ctx.profile(False)
-        if timings["solveKernelMs"] > 0:
-            visits = timings["solveContactVisits"]
-            launches = timings["solveKernelLaunches"]
-            avg_ms = timings["solveKernelMs"] / launches
-            bytes_per_launch = SOLVE_BYTES_PER_VISIT * visits / launches
+        kernels = {}
+        for kind, name in enumerate(Timings.KERNELS):
+            launches = timings["kernelLaunches"][kind]
+            if launches:
+                kernels[name] = {"ms_per_step": round(timings["kernelMs"][kind] / max(1, timings["steps"]), 4),
+                                 "launches": launches,
+                                 "avg_launch_us": round(timings["kernelMs"][kind] * 1e3 / launches, 3),
+                                 "contact_visits": timings["kernelVisits"][kind]}
+        # Roofline of the dominant solver kernel (largest total device time).
+        kind = max(range(len(Timings.KERNELS)), key=lambda k: timings["kernelMs"][k])
+        if timings["kernelLaunches"][kind]:
+            name = Timings.KERNELS[kind]
+            launches = timings["kernelLaunches"][kind]
+            avg_ms = timings["kernelMs"][kind] / launches
+            bytes_per_launch = SOLVE_BYTES_PER_VISIT * timings["kernelVisits"][kind] / launches
             achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-            roofline = {"kernel": "kSweepColor", "bound": "hbm", "achieved": round(achieved, 3),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                        "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches,
-                        "bytes_per_launch": round(bytes_per_launch, 1)}
+            roofline = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                        "traffic": measured_traffic(name, args.bodies, args.steps),
+                        "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches,
+                        "bytes_per_launch": round(bytes_per_launch, 1),
+                        "bytes_per_contact_visit": SOLVE_BYTES_PER_VISIT}
 
     cpu = None
     if not args.no_cpu:
@@ -163,7 +190,8 @@ def main():
     }
     if timings:
         line["stage_ms_per_step"] = {k[:-2]: round(v / max(1, timings["steps"]), 4)
-                                     for k, v in timings.items() if k.endswith("Ms") and k != "solveKernelMs"}
+                                     for k, v in timings.items() if k.endswith("Ms") and k != "kernelMs"}
+        line["solve_kernels"] = kernels
     print(json.dumps(line))
 
 

@@ -156,15 +156,22 @@ int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
 
 /* Stage timings measured with HIP events on the context's own stream while
  * profiling is enabled (accumulated over steps; StepStats timings,
- * world.cpp:4523-4575).  Every colour-solve launch is bracketed by its own
- * event pair, giving the solver kernel's device time and the contacts it
- * visited. */
+ * world.cpp:4523-4575).  Every contact-solve launch is bracketed by its own
+ * event pair, giving each solver kernel's device time, launch count and the
+ * contact visits (contacts x sweeps) it performed, indexed by VXHIP_KERNEL_*. */
+enum {
+    VXHIP_KERNEL_BATCH = 0,      /* kBatchSweeps, velocity sweeps (island batches in LDS) */
+    VXHIP_KERNEL_BATCH_WARM = 1, /* kBatchSweeps, warm start */
+    VXHIP_KERNEL_ISLAND = 2,     /* kIslandSweeps (one larger island per workgroup) */
+    VXHIP_KERNEL_COLOR = 3,      /* kSweepColorIdx (big islands, one launch per colour) */
+    VXHIP_KERNEL_COUNT = 4
+};
 typedef struct {
     uint32_t steps;
     float beginMs, broadMs, narrowMs, matchMs, colorMs, solveMs, ccdMs, positionMs, finishMs;
-    float solveKernelMs;           /* sum over colour-solve launches */
-    uint32_t solveKernelLaunches;
-    uint64_t solveContactVisits;   /* contacts processed by those launches */
+    float kernelMs[VXHIP_KERNEL_COUNT];
+    uint32_t kernelLaunches[VXHIP_KERNEL_COUNT];
+    uint64_t kernelVisits[VXHIP_KERNEL_COUNT];
 } vxhip_timings;
 
 /* Enables (non-zero) or disables per-stage event timing and resets it. */

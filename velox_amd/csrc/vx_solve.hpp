@@ -579,11 +579,13 @@ __global__ __launch_bounds__(kThreads) void kBatchPasses(const uint32_t* __restr
 }
 
 __global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t ms, uint32_t* batchFirstRun,
-                              const uint32_t* batches, const uint32_t* ldsBatches, uint32_t* ranges) {
+                              const uint32_t* batches, const uint32_t* ldsBatches, uint32_t* ranges,
+                              uint32_t* ldsContacts) {
     runStart[*runs] = ms;
     batchFirstRun[*batches] = *runs;
     ranges[0] = 0; ranges[1] = *ldsBatches;         // LDS batches
     ranges[2] = *ldsBatches; ranges[3] = *batches;  // global-memory islands
+    *ldsContacts = runStart[batchFirstRun[*ldsBatches]];
 }
 
 // Batches of small islands, one workgroup each: every sweep of every colour

@@ -123,7 +123,8 @@ struct vxhip_ctx {
     Buf<uint64_t> smallKeys, smallKeysTmp;
     Buf<uint32_t> smallIdx, smallIdxTmp, runHead, islandHead, runPos, islandPos, runStart, islandFirstRun;
     std::vector<uint32_t> bigColorStartHost;
-    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, batchCount = 0, ldsBatchCount = 0, largestIsland = 0,
+    uint32_t smallCount = 0, bigCount = 0, islandCount = 0, batchCount = 0, ldsBatchCount = 0, ldsContactCount = 0,
+             largestIsland = 0,
              maxBatchPasses = 0;
     uint32_t islandLimit = 4096;
     Buf<uint32_t> ccdDone;
@@ -147,6 +148,7 @@ struct vxhip_ctx {
     std::vector<hipEvent_t> launchEv;    // pairs around colour-solve launches
     uint32_t launchUsed = 0;
     std::vector<uint64_t> launchContacts;
+    std::vector<int> launchKind;
     vxhip_timings timings{};
 };
 
@@ -155,7 +157,7 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cCount = 40
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cCount = 40
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -185,11 +187,12 @@ void solveLaunchBegin(vxhip_ctx* c) {
     HIPCK(hipEventRecord(c->launchEv[c->launchUsed], c->stream));
 }
 
-void solveLaunchEnd(vxhip_ctx* c, uint64_t contacts) {
+void solveLaunchEnd(vxhip_ctx* c, int kind, uint64_t contacts) {
     if (!c->profile) return;
     HIPCK(hipEventRecord(c->launchEv[c->launchUsed + 1], c->stream));
     c->launchUsed += 2;
     c->launchContacts.push_back(contacts);
+    c->launchKind.push_back(kind);
 }
 
 void collectProfile(vxhip_ctx* c) {
@@ -206,12 +209,14 @@ void collectProfile(vxhip_ctx* c) {
     for (uint32_t i = 0; i < c->launchUsed; i += 2) {
         float ms = 0.0f;
         HIPCK(hipEventElapsedTime(&ms, c->launchEv[i], c->launchEv[i + 1]));
-        c->timings.solveKernelMs += ms;
-        c->timings.solveKernelLaunches += 1;
-        c->timings.solveContactVisits += c->launchContacts[i / 2];
+        const int kind = c->launchKind[i / 2];
+        c->timings.kernelMs[kind] += ms;
+        c->timings.kernelLaunches[kind] += 1;
+        c->timings.kernelVisits[kind] += c->launchContacts[i / 2];
     }
     c->launchUsed = 0;
     c->launchContacts.clear();
+    c->launchKind.clear();
     c->timings.steps += 1;
 }
 
@@ -514,7 +519,7 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
                                                    c->islandPos.p, ms, mb, globalBase, c->rangeStart.p, c->rangeEnd.p,
                                                    c->runStart.p, c->islandFirstRun.p, c->batchBodies.p);
         kRunSentinels<<<1, 1, 0, s>>>(c->runStart.p, k + cRuns, ms, c->islandFirstRun.p, k + cBatches,
-                                      k + cLdsBatches, k + cRanges);
+                                      k + cLdsBatches, k + cRanges, k + cLdsContacts);
         kBatchPasses<<<blocks(ms), kThreads, 0, s>>>(c->runStart.p, c->islandFirstRun.p, k + cLdsBatches, k + cPasses);
         HIPCK(hipGetLastError());
     }
@@ -523,9 +528,11 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
     readCounters(c);
     c->largestIsland = c->hostCounters[cLargest];
     c->maxBatchPasses = ms ? c->hostCounters[cPasses] : 0;
+    c->ldsContactCount = 0;
     if (ms) {
         c->batchCount = c->hostCounters[cBatches];
         c->ldsBatchCount = c->hostCounters[cLdsBatches];
+        c->ldsContactCount = c->hostCounters[cLdsContacts];
     }
 }
 
@@ -536,15 +543,21 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
     hipStream_t s = c->stream;
     if (c->batchCount) {
         const uint32_t lds = c->ldsBatchCount, global = c->batchCount - c->ldsBatchCount;
-        solveLaunchBegin(c);
-        if (lds)
+        if (lds) {
+            // Contacts visited: every contact of the LDS batches, once per sweep.
+            const uint64_t visits = (uint64_t)c->ldsContactCount * sweeps;
+            solveLaunchBegin(c);
             kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, s>>>(
                 sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
                 c->bodyList.p, c->counters.p + cRanges, sweeps, c->counters.p + cBatchError);
-        if (global)
+            solveLaunchEnd(c, kMode == 2 ? VXHIP_KERNEL_BATCH_WARM : VXHIP_KERNEL_BATCH, visits);
+        }
+        if (global) {
+            solveLaunchBegin(c);
             kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, s>>>(
                 sa, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
-        solveLaunchEnd(c, (uint64_t)c->smallCount * sweeps);
+            solveLaunchEnd(c, VXHIP_KERNEL_ISLAND, (uint64_t)(c->smallCount - c->ldsContactCount) * sweeps);
+        }
     }
     if (c->bigCount)
         for (int it = 0; it < sweeps; ++it)
@@ -553,7 +566,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 if (!cnt) continue;
                 solveLaunchBegin(c);
                 kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, s>>>(sa, c->smallCount, c->bigColorStart.p, col);
-                solveLaunchEnd(c, cnt);
+                solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
             }
     HIPCK(hipGetLastError());
 }
@@ -860,6 +873,7 @@ int vxhip_world_profile(vxhip_ctx* c, int enable) {
         c->timings = vxhip_timings{};
         c->launchUsed = 0;
         c->launchContacts.clear();
+        c->launchKind.clear();
     });
 }
 

@@ -247,61 +247,92 @@ VXD void solveTriangle(Simplex& s) {
     }
     float sum = va + vb + vc;
     if (sum < 1e-20f) {
-        int best = 0;
-        for (int i = 1; i < 3; ++i)
-            if (lengthSq(s.v[i].p) < lengthSq(s.v[best].p)) best = i;
-        s.v[0] = s.v[best]; s.count = 1; s.bary[0] = 1.0f;
+        // Nearest of the three vertices (first on ties).
+        const float l0 = lengthSq(s.v[0].p), l1 = lengthSq(s.v[1].p), l2 = lengthSq(s.v[2].p);
+        const int best01 = l1 < l0 ? 1 : 0;
+        const float lb = best01 ? l1 : l0;
+        if (l2 < lb) s.v[0] = s.v[2];
+        else if (best01) s.v[0] = s.v[1];
+        s.count = 1; s.bary[0] = 1.0f;
         return;
     }
     float inv = 1.0f / sum;
     s.count = 3; s.bary[0] = va * inv; s.bary[1] = vb * inv; s.bary[2] = vc * inv;
 }
 
+// Loops over simplex vertices are unrolled with a count guard so that every
+// index is a compile-time constant and the simplex stays in registers.
 VXD V3 simplexPoint(const Simplex& s) {
     V3 c{};
-    for (int i = 0; i < s.count; ++i) c += s.v[i].p * s.bary[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= s.count) break;
+        c += s.v[i].p * s.bary[i];
+    }
     return c;
+}
+
+VXD void simplexPush(Simplex& s, const SV& w) {
+    switch (s.count) {
+    case 0: s.v[0] = w; break;
+    case 1: s.v[1] = w; break;
+    case 2: s.v[2] = w; break;
+    default: s.v[3] = w; break;
+    }
+    ++s.count;
 }
 
 __device__ constexpr int kTetFaces[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
 
-// gjk.h:379-445
+// The face (i0, i1, i2) of the tetrahedron as a triangle simplex.
+template <int i0, int i1, int i2>
+VXD Simplex tetFace(const Simplex& s) {
+    Simplex tri;
+    tri.count = 3;
+    tri.v[0] = s.v[i0]; tri.v[1] = s.v[i1]; tri.v[2] = s.v[i2];
+    return tri;
+}
+
+// gjk.h:379-445 (faces visited in kTetFaces order)
 VXD bool solveTetrahedron(Simplex& s, float scale) {
     float volume = dot(s.v[1].p - s.v[0].p, cross(s.v[2].p - s.v[0].p, s.v[3].p - s.v[0].p));
     float floorV = 1e-10f * scale * scale * scale;
     if (fabsf(volume) < floorV) {
         Simplex best; float bestD = 1e30f;
-        for (int f = 0; f < 4; ++f) {
-            Simplex tri; tri.count = 3;
-            for (int k = 0; k < 3; ++k) tri.v[k] = s.v[kTetFaces[f][k]];
+        auto visit = [&](Simplex tri) {
             solveTriangle(tri);
             float d = lengthSq(simplexPoint(tri));
             if (d < bestD) { bestD = d; best = tri; }
-        }
+        };
+        visit(tetFace<0, 1, 2>(s));
+        visit(tetFace<0, 1, 3>(s));
+        visit(tetFace<0, 2, 3>(s));
+        visit(tetFace<1, 2, 3>(s));
         s = best;
         return false;
     }
     Simplex best; float bestD = 1e30f;
     bool inside = true;
     const float band = 3e-6f * scale;
-    for (int f = 0; f < 4; ++f) {
-        int i0 = kTetFaces[f][0], i1 = kTetFaces[f][1], i2 = kTetFaces[f][2];
-        V3 a = s.v[i0].p, b = s.v[i1].p, c = s.v[i2].p;
+    auto face = [&](const Simplex& tri, V3 opp) {
+        V3 a = tri.v[0].p, b = tri.v[1].p, c = tri.v[2].p;
         V3 n = cross(b - a, c - a);
         float nl = length(n);
-        if (nl < 1e-20f) continue;
-        V3 opp = s.v[6 - i0 - i1 - i2].p;
+        if (nl < 1e-20f) return;
         float od = dot(n, -a) / nl;
         float pd = dot(n, opp - a) / nl;
         if (od * pd < 0.0f || fabsf(od) <= band) {
             inside = false;
-            Simplex tri; tri.count = 3;
-            tri.v[0] = s.v[i0]; tri.v[1] = s.v[i1]; tri.v[2] = s.v[i2];
-            solveTriangle(tri);
-            float d = lengthSq(simplexPoint(tri));
-            if (d < bestD) { bestD = d; best = tri; }
+            Simplex t = tri;
+            solveTriangle(t);
+            float d = lengthSq(simplexPoint(t));
+            if (d < bestD) { bestD = d; best = t; }
         }
-    }
+    };
+    face(tetFace<0, 1, 2>(s), s.v[3].p);
+    face(tetFace<0, 1, 3>(s), s.v[2].p);
+    face(tetFace<0, 2, 3>(s), s.v[1].p);
+    face(tetFace<1, 2, 3>(s), s.v[0].p);
     if (!inside) s = best;
     return inside;
 }
@@ -312,6 +343,7 @@ VXD bool originInTet(const SV& a, const SV& b, const SV& c, const SV& d, float s
     float volume = dot(v[1] - v[0], cross(v[2] - v[0], v[3] - v[0]));
     float floorV = 1e-10f * scale * scale * scale;
     if (fabsf(volume) < floorV) return false;
+#pragma unroll
     for (int f = 0; f < 4; ++f) {
         int i0 = kTetFaces[f][0], i1 = kTetFaces[f][1], i2 = kTetFaces[f][2];
         V3 n = cross(v[i1] - v[i0], v[i2] - v[i0]);
@@ -326,7 +358,11 @@ VXD bool originInTet(const SV& a, const SV& b, const SV& c, const SV& d, float s
 VXD bool expandOverlapSimplex(const Convex& A, const Convex& B, Simplex& s) {
     SV cand[18];
     int count = 0;
-    for (int i = 0; i < s.count; ++i) cand[count++] = s.v[i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= s.count) break;
+        cand[count++] = s.v[i];
+    }
     constexpr float dirs[14][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
                                       {1, 1, 1}, {-1, 1, 1}, {1, -1, 1}, {1, 1, -1},
                                       {-1, -1, 1}, {-1, 1, -1}, {1, -1, -1}, {-1, -1, -1}};
@@ -487,16 +523,24 @@ VXD GjkResult gjkSeeded(const Convex& A, const Convex& B, V3 dir) {
                           : 1e-6f * sqrtf(distSq) + 1e-9f;
         if (progress < pf) break;
         bool dup = false;
-        for (int i = 0; i < s.count; ++i)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (i >= s.count) break;
             if (lengthSq(w.p - s.v[i].p) < dupDistSq(A, B)) { dup = true; break; }
+        }
         if (dup) break;
-        s.v[s.count++] = w;
+        simplexPush(s, w);
         if (s.count == 2) solveSegment(s);
         else if (s.count == 3) solveTriangle(s);
         else if (solveTetrahedron(s, scale)) { r.overlapping = true; break; }
     }
     V3 pa{}, pb{};
-    for (int i = 0; i < s.count; ++i) { pa += s.v[i].a * s.bary[i]; pb += s.v[i].b * s.bary[i]; }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (i >= s.count) break;
+        pa += s.v[i].a * s.bary[i];
+        pb += s.v[i].b * s.bary[i];
+    }
     if (r.overlapping) {
         if (s.count == 4 && epaPenetration(A, B, s, r)) return r;
         V3 n = A.position - B.position;

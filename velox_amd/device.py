@@ -104,11 +104,18 @@ def device_count():
 class Timings(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("beginMs", C.c_float), ("broadMs", C.c_float), ("narrowMs", C.c_float),
                 ("matchMs", C.c_float), ("colorMs", C.c_float), ("solveMs", C.c_float), ("ccdMs", C.c_float),
-                ("positionMs", C.c_float), ("finishMs", C.c_float), ("solveKernelMs", C.c_float),
-                ("solveKernelLaunches", C.c_uint32), ("solveContactVisits", C.c_uint64)]
+                ("positionMs", C.c_float), ("finishMs", C.c_float), ("kernelMs", C.c_float * 4),
+                ("kernelLaunches", C.c_uint32 * 4), ("kernelVisits", C.c_uint64 * 4)]
+
+    # Solver kernel families, in the order of the per-kernel arrays.
+    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorIdx")
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            d[name] = list(v) if name.startswith("kernel") else v
+        return d
 
 
 class Scene:
