@@ -43,6 +43,61 @@ def measured_traffic(kernel, bodies, steps):
     return None if entry is None else entry["bytes_per_launch"]
 
 
+def batched_leg(worlds, steps, sample, plain_steps):
+    """World-steps/s of one batched device step over `worlds` independent
+    worlds against the same worlds stepped one after another, each in its own
+    context (a uniform stride sample of them; every world's cost is its own,
+    so the sample's rate estimates the whole set's)."""
+    from velox_amd.device import Context, Scene
+    scene = Scene("batch", worlds)
+    dt = scene.dt
+    ctx = Context()
+    ctx.load_scene(scene)
+    ctx.step(dt, 1)  # warm-up (buffer growth)
+    ctx.sync()
+    ctx.load_scene(scene)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(dt, steps)
+    ctx.sync()
+    batched_s = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.close()
+    print(f"[bench] batched: {worlds} worlds, {len(scene.bodies)} bodies, {steps} steps in {batched_s:.3f}s",
+          file=sys.stderr, flush=True)
+    stride = max(1, worlds // max(1, sample))
+    chosen = list(range(0, worlds, stride))[:sample]
+    # Plain: each sampled world stepped on its own, world after world; one
+    # context is reused (loading a world replaces the previous one), and only
+    # the world's own steps are timed, after a warm-up step that sizes buffers.
+    plain = Context()
+    plain_s = 0.0
+    for w in chosen:
+        sub = scene.world(w)
+        plain.load_scene(sub)
+        plain.step(dt, 1)
+        plain.sync()
+        plain.load_scene(sub)
+        plain.sync()
+        t0 = time.perf_counter()
+        for _ in range(plain_steps):
+            plain.step(dt, 1)
+            plain.sync()
+        plain_s += time.perf_counter() - t0
+    plain.close()
+    print(f"[bench] plain: {len(chosen)} worlds x {plain_steps} steps in {plain_s:.3f}s", file=sys.stderr, flush=True)
+    batched_rate = worlds * steps / batched_s
+    plain_rate = len(chosen) * plain_steps / plain_s
+    return {"metric": "batched vs plain world-steps/s, same GPU", "worlds": worlds, "bodies": int(len(scene.bodies)),
+            "batched_world_steps_per_s": round(batched_rate, 1), "plain_world_steps_per_s": round(plain_rate, 1),
+            "ratio": round(batched_rate / plain_rate, 2), "batched_steps": steps,
+            "batched_ms_per_step": round(batched_s * 1e3 / steps, 3), "contacts_last_step": st["contacts"],
+            "plain": f"{len(chosen)} of {worlds} worlds (every {stride}th), each loaded alone and stepped "
+                     f"{plain_steps} times with a sync per world-step (timed), after an untimed warm-up step",
+            "workload": "log-uniform 8..512 bodies per world (box towers, sphere piles, capsule rows) on one plane "
+                        "each, dt=1/60, 4 substeps, 8 iterations"}
+
+
 def run_cpu_baseline(bodies, steps, workers):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_bench")
     if not os.path.exists(exe):
@@ -69,6 +124,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=120, help="reference CPU sample length (steps)")
     ap.add_argument("--cpu-workers", type=int, default=16, help="reference CPU worker threads")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched-worlds leg")
+    ap.add_argument("--worlds", type=int, default=4096, help="batched leg: number of worlds")
+    ap.add_argument("--batched-steps", type=int, default=20, help="batched leg: timed steps of the batched world")
+    ap.add_argument("--plain-sample", type=int, default=512,
+                    help="batched leg: worlds stepped one by one, each in its own context (uniform stride sample)")
+    ap.add_argument("--plain-steps", type=int, default=3, help="batched leg: timed steps per plain world")
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--progress", type=int, default=25, help="steps between progress lines")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -168,6 +229,10 @@ def main():
                    "gpu_same_sample": round(gpu_sample, 1)}
     ctx.close()
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(args.worlds, args.batched_steps, args.plain_sample, args.plain_steps)
+
     line = {
         "metric": "body-steps/s @50k mixed bodies",
         "value": round(value, 1),
@@ -187,6 +252,7 @@ def main():
                    "colors_last_step": last["colors"], "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     if timings:
         line["stage_ms_per_step"] = {k[:-2]: round(v / max(1, timings["steps"]), 4)

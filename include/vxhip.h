@@ -131,6 +131,15 @@ uint32_t vxhip_last_velocity_iterations(const vxhip_ctx* ctx);
 int vxhip_world_load(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx_soup* soup,
                      const vxhip_config* config);
 
+/* Batched worlds (no reference counterpart: the batched form of the device
+ * step).  After vxhip_world_load, declares that body i belongs to world
+ * worldOf[i] < worldCount; bodies of different worlds never interact, and
+ * each world evolves exactly as it would loaded alone (the broad phase keys
+ * cells by world and pairs oversize bodies only within their world; every
+ * later stage is per pair, per body chain or per island).  worldCount = 1
+ * returns to a single world.  Reset by vxhip_world_load. */
+int vxhip_world_set_worlds(vxhip_ctx* ctx, const uint32_t* worldOf, uint32_t worldCount);
+
 /* Advances the resident world `steps` times by dt: the device form of
  * World::stepImpl's physics (world.cpp:3975-4397 without events, sleeping,
  * joints or host callbacks): orientation renormalisation, integration, broad

@@ -10,6 +10,7 @@
 // (examples/fuzz_demo.cpp:23-31).  Orientations are uniform (Shoemake's
 // subgroup algorithm); normals use Box-Muller.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <vector>
@@ -258,6 +259,84 @@ inline Scene terrain(uint32_t side = 24, float cell = 0.5f, float amp = 1.0f, ui
         BodySpec b = d.bodies[i];
         b.pos[1] += 1.5f * amp;
         s.bodies.push_back(b);
+    }
+    return s;
+}
+
+// Batched worlds: `worlds` independent worlds, each on its own static plane
+// (all at the same coordinates; a world id separates them).  A world holds
+// n_w = round(exp(U(ln 8, ln 512))) dynamic bodies built from seeded
+// structures on a 4 m site grid: box towers of height U{3..12} (unit boxes,
+// small yaw jitter), sphere piles (r ~ U(0.2, 0.4), a loose cluster dropped
+// from 1-3 m) and capsule rows (r ~ U(0.15, 0.25), half-height U(0.3, 0.6),
+// lying side by side).  Density 1000, friction 0.6, restitution 0.
+inline Scene batchWorlds(uint32_t worlds = 4096, uint32_t seed = kSeedBatch) {
+    Scene s;
+    s.worlds = worlds;
+    Rng r(seed);
+    const float pi = 3.14159265f;
+    for (uint32_t w = 0; w < worlds; ++w) {
+        BodySpec pl = plane(0.6f, 0.0f);
+        pl.world = w;
+        s.bodies.push_back(pl);
+        const uint32_t target = (uint32_t)std::lround(std::exp(r.range(std::log(8.0f), std::log(512.0f))));
+        uint32_t placed = 0, site = 0;
+        while (placed < target) {
+            const float sx = 4.0f * (float)(site % 8) - 14.0f, sz = 4.0f * (float)(site / 8) - 14.0f;
+            ++site;
+            const uint32_t left = target - placed;
+            const int kind = r.pick(3);
+            auto spec = [&](Kind k) {
+                BodySpec b;
+                b.kind = k;
+                b.friction = 0.6f;
+                b.restitution = 0.0f;
+                b.world = w;
+                return b;
+            };
+            if (kind == 0) {  // box tower
+                const uint32_t h = std::min<uint32_t>(left, 3u + (uint32_t)r.pick(10));
+                for (uint32_t k = 0; k < h; ++k) {
+                    BodySpec b = spec(Box);
+                    b.half[0] = b.half[1] = b.half[2] = 0.5f;
+                    b.mass = 1000.0f;
+                    b.pos[0] = sx + r.range(-0.02f, 0.02f);
+                    b.pos[1] = 0.5f + 1.002f * (float)k;
+                    b.pos[2] = sz + r.range(-0.02f, 0.02f);
+                    const float yaw = r.range(-0.1f, 0.1f);
+                    b.quat[0] = 0.0f; b.quat[1] = std::sin(0.5f * yaw); b.quat[2] = 0.0f; b.quat[3] = std::cos(0.5f * yaw);
+                    s.bodies.push_back(b);
+                }
+                placed += h;
+            } else if (kind == 1) {  // sphere pile
+                const uint32_t m = std::min<uint32_t>(left, 8u + (uint32_t)r.pick(25));
+                for (uint32_t k = 0; k < m; ++k) {
+                    BodySpec b = spec(Sphere);
+                    b.radius = r.range(0.2f, 0.4f);
+                    b.mass = 1000.0f * (4.0f / 3.0f) * pi * b.radius * b.radius * b.radius;
+                    b.pos[0] = sx + 0.85f * (float)(k % 3) - 0.85f + r.range(-0.05f, 0.05f);
+                    b.pos[1] = 1.0f + 0.85f * (float)(k / 9) + r.range(0.0f, 0.1f);
+                    b.pos[2] = sz + 0.85f * (float)((k / 3) % 3) - 0.85f + r.range(-0.05f, 0.05f);
+                    s.bodies.push_back(b);
+                }
+                placed += m;
+            } else {  // capsule row, axes along x, side by side along z
+                const uint32_t m = std::min<uint32_t>(left, 4u + (uint32_t)r.pick(4));
+                const float rad = r.range(0.15f, 0.25f), hh = r.range(0.3f, 0.6f);
+                for (uint32_t k = 0; k < m; ++k) {
+                    BodySpec b = spec(Capsule);
+                    b.radius = rad;
+                    b.halfHeight = hh;
+                    b.mass = 1000.0f * (pi * rad * rad * 2.0f * hh + (4.0f / 3.0f) * pi * rad * rad * rad);
+                    b.pos[0] = sx;
+                    b.pos[1] = rad + 0.01f;
+                    b.pos[2] = sz - 1.8f + (2.0f * rad + 0.002f) * (float)k;
+                    b.quat[0] = 0.0f; b.quat[1] = 0.0f; b.quat[2] = std::sin(0.25f * pi); b.quat[3] = std::cos(0.25f * pi);
+                    s.bodies.push_back(b);
+                }
+                placed += m;
+            }
+        }
     }
     return s;
 }

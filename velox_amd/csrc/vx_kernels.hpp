@@ -14,7 +14,8 @@ struct Grid {           // dense cell grid of the broad phase
     float cell;         // cell edge
     int minx, miny, minz;
     uint32_t nx, ny, nz;
-    uint64_t ncells;    // also the key of oversize bodies
+    uint64_t ncells;    // cells of one world's grid
+    uint64_t sentinel;  // key of oversize bodies: worlds * ncells
 };
 
 __device__ inline float4 f4(V3 v, float w = 0.0f) { return make_float4(v.x, v.y, v.z, w); }
@@ -129,10 +130,14 @@ __global__ __launch_bounds__(kThreads) void kCellBounds(const vx_body* __restric
     atomicMax(&bounds[3], cx); atomicMax(&bounds[4], cy); atomicMax(&bounds[5], cz);
 }
 
+// Batched worlds share one grid: a body's key is world * ncells + cell, so
+// neighbourhood searches never leave the body's own world.
 __global__ __launch_bounds__(kThreads) void kCellKeys(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
                                                       const float4* __restrict__ hi, uint32_t n, Grid g,
+                                                      const uint32_t* __restrict__ worldOf,
                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ idx,
-                                                      uint32_t* __restrict__ oversize, uint32_t* __restrict__ oversizeCount) {
+                                                      uint32_t* __restrict__ oversize, uint32_t* __restrict__ oversizeCount,
+                                                      uint32_t* __restrict__ worldOversize) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
     idx[i] = i;
@@ -140,14 +145,17 @@ __global__ __launch_bounds__(kThreads) void kCellKeys(const vx_body* __restrict_
     float ext = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
     const uint8_t shape = bodies[i].shape;
     if (ext > g.cell || shape == VX_SHAPE_PLANE || shape == VX_SHAPE_MESH) {
-        keys[i] = g.ncells;
+        keys[i] = g.sentinel;
         uint32_t slot = atomicAdd(oversizeCount, 1u);
         oversize[slot] = i;
+        if (worldOf) atomicAdd(&worldOversize[worldOf[i]], 1u);
         return;
     }
     int cx, cy, cz;
     cellOf(l, h, 1.0f / g.cell, cx, cy, cz);
-    keys[i] = ((uint64_t)(cx - g.minx) * g.ny + (uint64_t)(cy - g.miny)) * g.nz + (uint64_t)(cz - g.minz);
+    const uint64_t world = worldOf ? worldOf[i] : 0u;
+    keys[i] = world * g.ncells +
+              ((uint64_t)(cx - g.minx) * g.ny + (uint64_t)(cy - g.miny)) * g.nz + (uint64_t)(cz - g.minz);
 }
 
 __device__ inline bool pairAllowed(const vx_body& a, const vx_body& b) {
@@ -201,7 +209,9 @@ __device__ inline uint32_t gridVisit(const vx_body* __restrict__ bodies, const f
                                      const float4* __restrict__ hi, const uint64_t* __restrict__ keys,
                                      const uint32_t* __restrict__ idx, uint32_t n, const Grid& g, uint32_t s, int nb,
                                      uint64_t* out, uint32_t at, uint32_t cap) {
-    const uint64_t key = keys[s];
+    const uint64_t wkey = keys[s];
+    const uint64_t base = (wkey / g.ncells) * g.ncells;  // the body's world
+    const uint64_t key = wkey - base;
     const uint32_t i = idx[s];
     const uint32_t z = (uint32_t)(key % g.nz);
     const uint32_t y = (uint32_t)((key / g.nz) % g.ny);
@@ -215,7 +225,7 @@ __device__ inline uint32_t gridVisit(const vx_body* __restrict__ bodies, const f
             if ((dy < 0 && y == 0) || (uint32_t)(y + dy) >= g.ny) continue;
             for (int dz = -1; dz <= 1; ++dz) {
                 if ((dz < 0 && z == 0) || (uint32_t)(z + dz) >= g.nz) continue;
-                const uint64_t nk = ((uint64_t)(x + dx) * g.ny + (y + dy)) * g.nz + (z + dz);
+                const uint64_t nk = base + ((uint64_t)(x + dx) * g.ny + (y + dy)) * g.nz + (z + dz);
                 for (uint32_t t = lowerBound(keys, n, nk); t < n && keys[t] == nk; ++t) {
                     const uint32_t j = idx[t];
                     if (j <= i) continue;
@@ -238,7 +248,7 @@ __global__ __launch_bounds__(kThreads) void kGridPairs(const vx_body* __restrict
                                                        uint64_t* __restrict__ pairs, uint32_t* __restrict__ pairCount,
                                                        uint32_t cap) {
     uint32_t s = blockIdx.x * kThreads + threadIdx.x;
-    const bool valid = s < n && keys[s] != g.ncells;
+    const bool valid = s < n && keys[s] != g.sentinel;
     uint32_t cnt = valid ? gridVisit<false>(bodies, lo, hi, keys, idx, n, g, s, nb, nullptr, 0, 0) : 0u;
     uint32_t at = waveReserve(cnt, pairCount);
     if (valid && cnt) gridVisit<true>(bodies, lo, hi, keys, idx, n, g, s, nb, pairs, at, cap);
@@ -272,6 +282,55 @@ __global__ __launch_bounds__(kThreads) void kOversizePairs(const vx_body* __rest
     }
     uint32_t at = waveReserve(hit ? 1u : 0u, pairCount);
     if (hit && at < cap) pairs[at] = key;
+}
+
+// Batched worlds: oversize bodies listed per world, then each body tested
+// against its own world's oversize bodies only (the same tests and the same
+// duplicate rule as kOversizePairs).
+__global__ __launch_bounds__(kThreads) void kOversizeByWorld(const uint32_t* __restrict__ oversize, uint32_t oversizeCount,
+                                                             const uint32_t* __restrict__ worldOf,
+                                                             const uint32_t* __restrict__ worldStart,
+                                                             uint32_t* __restrict__ worldFill,
+                                                             uint32_t* __restrict__ byWorld) {
+    uint32_t o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= oversizeCount) return;
+    const uint32_t i = oversize[o], w = worldOf[i];
+    byWorld[worldStart[w] + atomicAdd(&worldFill[w], 1u)] = i;
+}
+
+__global__ __launch_bounds__(kThreads) void kOversizePairsWorld(const vx_body* __restrict__ bodies,
+                                                                const float4* __restrict__ lo,
+                                                                const float4* __restrict__ hi,
+                                                                const uint32_t* __restrict__ worldOf,
+                                                                const uint32_t* __restrict__ worldStart,
+                                                                const uint32_t* __restrict__ byWorld, uint32_t n,
+                                                                Grid g, int nb, uint64_t* __restrict__ pairs,
+                                                                uint32_t* __restrict__ pairCount, uint32_t cap) {
+    const uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t first = 0, last = 0;
+    float4 lj{}, hj{};
+    bool jOver = false;
+    if (j < n) {
+        const uint32_t w = worldOf[j];
+        first = worldStart[w];
+        last = worldStart[w + 1];
+        lj = lo[j];
+        hj = hi[j];
+        const float extJ = fmaxf(hj.x - lj.x, fmaxf(hj.y - lj.y, hj.z - lj.z));
+        jOver = oversized(bodies[j], extJ, g.cell);
+    }
+    // Lanes of a wavefront walk their own lists; emission is reserved per hit.
+    for (uint32_t o = first; o < last; ++o) {
+        const uint32_t i = byWorld[o];
+        const float4 li = lo[i], hiI = hi[i];
+        const bool hit = j != i && !(jOver && j < i) &&
+                         hiI.x >= lj.x && li.x <= hj.x && hiI.y >= lj.y && li.y <= hj.y && hiI.z >= lj.z &&
+                         li.z <= hj.z && pairAllowed(bodies[i], bodies[j]);
+        if (hit) {
+            const uint32_t at = atomicAdd(pairCount, 1u);
+            if (at < cap) pairs[at] = packPair(bodies, i, j, nb);
+        }
+    }
 }
 
 // ----------------------------------------------------------- narrow phase --

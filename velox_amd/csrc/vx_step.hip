@@ -111,6 +111,10 @@ struct vxhip_ctx {
     Buf<int> nextAfter;
     Buf<uint8_t> ccdFlag;
     Buf<uint32_t> ccdRel, ccdRelPos, ccdList, ccdFirstRel;
+    // Batched worlds: world id per body (bodies of different worlds never
+    // interact) and the per-world oversize lists of the broad phase.
+    Buf<uint32_t> worldOf, worldOversize, worldStart, worldFill, oversizeByWorld;
+    uint32_t worldCount = 1, worldN = 0;
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta, sMetaK;
     Buf<float4> sSide;
@@ -329,12 +333,29 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
         g.nx = g.ny = g.nz = 1;
     }
     g.ncells = (uint64_t)g.nx * g.ny * g.nz;
+    const bool batched = c->worldCount > 1 && c->worldN == n;
+    const uint32_t W = batched ? c->worldCount : 1u;
+    g.sentinel = (uint64_t)W * g.ncells;
     c->stats.cellSize = g.cell;
     c->stats.oversize = oversizeCount;
-    kCellKeys<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, g, c->keys.p, c->idx.p, c->oversize.p,
-                                             k + cOversize);
-    HIPCK(vxhost::radixSort(c->keys.p, c->idx.p, c->keysTmp.p, c->idxTmp.p, n, vxhost::bitsFor(g.ncells),
+    if (batched) {
+        c->worldOversize.reserve(W + 1); c->worldStart.reserve(W + 1); c->worldFill.reserve(W);
+        c->oversizeByWorld.reserve(n);
+        fill32(c, c->worldOversize.p, W, 0);
+    }
+    kCellKeys<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, g, batched ? c->worldOf.p : nullptr,
+                                             c->keys.p, c->idx.p, c->oversize.p, k + cOversize,
+                                             batched ? c->worldOversize.p : nullptr);
+    HIPCK(vxhost::radixSort(c->keys.p, c->idx.p, c->keysTmp.p, c->idxTmp.p, n, vxhost::bitsFor(g.sentinel),
                             tempFor(c, vxhost::radixTempBytes(n)), s));
+    if (batched && oversizeCount) {
+        HIPCK(vxhost::exclusiveScan(c->worldOversize.p, c->worldStart.p, W, c->worldStart.p + W,
+                                    tempFor(c, vxhost::scanTempBytes(W)), s));
+        fill32(c, c->worldFill.p, W, 0);
+        kOversizeByWorld<<<blocks(oversizeCount), kThreads, 0, s>>>(c->oversize.p, oversizeCount, c->worldOf.p,
+                                                                    c->worldStart.p, c->worldFill.p,
+                                                                    c->oversizeByWorld.p);
+    }
     const int nb = bodyBits(n);
     if (c->pairCap == 0) c->pairCap = 32 * n + 1024;
     for (;;) {
@@ -342,7 +363,11 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
         kResetCounters<<<1, 64, 0, s>>>(k);  // only cPairs matters here; others re-derived
         kGridPairs<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->keys.p, c->idx.p, n, g, nb,
                                                   c->pairs.p, k + cPairs, c->pairCap);
-        if (oversizeCount)
+        if (oversizeCount && batched)
+            kOversizePairsWorld<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->worldOf.p,
+                                                               c->worldStart.p, c->oversizeByWorld.p, n, g, nb,
+                                                               c->pairs.p, k + cPairs, c->pairCap);
+        else if (oversizeCount)
             kOversizePairs<<<blocks((uint64_t)oversizeCount * n), kThreads, 0, s>>>(
                 c->bodies.p, c->lo.p, c->hi.p, c->oversize.p, oversizeCount, n, g, nb, c->pairs.p, k + cPairs, c->pairCap);
         HIPCK(hipGetLastError());
@@ -808,6 +833,8 @@ int vxhip_world_load(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_s
         setBodies(c, bodies, n);
         c->prevCount = 0;
         c->solveCount = 0;
+        c->worldCount = 1;
+        c->worldN = 0;
         HIPCK(hipStreamSynchronize(c->stream));
         c->loaded = true;
     });
@@ -819,6 +846,23 @@ int vxhip_world_step(vxhip_ctx* c, float dt, uint32_t steps) {
         if (!(dt >= 0.0f) || dt > 1e30f) fail(VXHIP_ERR_INVALID, "vxhip: timestep must be finite and non-negative");
         if (dt == 0.0f) return;
         for (uint32_t i = 0; i < steps; ++i) worldStep(c, dt);
+    });
+}
+
+int vxhip_world_set_worlds(vxhip_ctx* c, const uint32_t* worldOf, uint32_t worldCount) {
+    return guarded([&] {
+        if (!c || !c->loaded) fail(VXHIP_ERR_INVALID, "vxhip: no world loaded");
+        if (worldCount == 0 || (worldCount > 1 && !worldOf)) fail(VXHIP_ERR_INVALID, "vxhip: invalid world ids");
+        if (worldCount > 1) {
+            for (uint32_t i = 0; i < c->n; ++i)
+                if (worldOf[i] >= worldCount) fail(VXHIP_ERR_INVALID, "vxhip: world id out of range");
+            c->worldOf.reserve(c->n);
+            HIPCK(hipMemcpyAsync(c->worldOf.p, worldOf, (size_t)c->n * sizeof(uint32_t), hipMemcpyHostToDevice,
+                                 c->stream));
+            HIPCK(hipStreamSynchronize(c->stream));
+        }
+        c->worldCount = worldCount;
+        c->worldN = c->n;
     });
 }
 

@@ -54,6 +54,7 @@ SIGNATURES = {
     "vxhip_last_velocity_iterations": (_u32, [_vp]),
     "vxhip_world_load": (_i32, [_vp, _vp, _u32, _vp, _vp]),
     "vxhip_world_step": (_i32, [_vp, _f32, _u32]),
+    "vxhip_world_set_worlds": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_sync": (_i32, [_vp]),
     "vxhip_world_download_bodies": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_upload_bodies": (_i32, [_vp, _vp, _u32]),
@@ -163,6 +164,16 @@ class Scene:
         finally:
             lib.vxhip_scene_free(h)
 
+    def world(self, w):
+        """World w of a batched scene as a scene of its own (same geometry
+        arrays, the world's bodies in their batched order)."""
+        sub = object.__new__(Scene)
+        sub.bodies = np.ascontiguousarray(self.bodies[self.world_of == w])
+        sub.worlds = 1
+        sub.world_of = np.zeros(len(sub.bodies), np.uint32)
+        sub.gravity, sub.substeps, sub.dt, sub.soup = self.gravity, self.substeps, self.dt, self.soup
+        return sub
+
 
 class SoupView(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("vertexCount", C.c_uint32), ("indices", C.c_void_p),
@@ -197,7 +208,17 @@ class Context:
 
     # -- resident world --------------------------------------------------
     def load_scene(self, scene, options=None):
-        return self.load(scene.bodies, scene.soup, gravity=scene.gravity, substeps=scene.substeps, options=options)
+        self.load(scene.bodies, scene.soup, gravity=scene.gravity, substeps=scene.substeps, options=options)
+        if getattr(scene, "worlds", 1) > 1:
+            self.set_worlds(scene.world_of, scene.worlds)
+        return self
+
+    def set_worlds(self, world_of, worlds):
+        """Batched worlds: body i belongs to world world_of[i]."""
+        world_of = np.ascontiguousarray(world_of, dtype=np.uint32)
+        assert len(world_of) == self.n
+        check(library().vxhip_world_set_worlds(self._h, ptr(world_of), int(worlds)))
+        return self
 
     def load(self, bodies, soup, gravity=(0.0, -9.81, 0.0), substeps=4, options=None):
         self._soup = soup
