@@ -125,6 +125,7 @@ def main():
     ap.add_argument("--cpu-workers", type=int, default=16, help="reference CPU worker threads")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched-worlds leg")
+    ap.add_argument("--batched-only", action="store_true", help="run only the batched-worlds leg (profiling)")
     ap.add_argument("--worlds", type=int, default=4096, help="batched leg: number of worlds")
     ap.add_argument("--batched-steps", type=int, default=20, help="batched leg: timed steps of the batched world")
     ap.add_argument("--plain-sample", type=int, default=512,
@@ -149,6 +150,10 @@ def main():
         del src, dst
 
     from velox_amd.device import Context, Scene, Timings
+    if args.batched_only:
+        print(json.dumps({"batched": batched_leg(args.worlds, args.batched_steps, args.plain_sample,
+                                                 args.plain_steps)}))
+        return
     scene = Scene("debris", args.bodies)
     n = len(scene.bodies)
     ctx = Context()
