@@ -50,6 +50,29 @@ def test_batched_equals_each_world_alone(ctx, steps):
         alone.close()
 
 
+def test_batched_large_equals_each_world_alone(ctx):
+    """256 worlds: the contact set is large enough for the device-wide colouring
+    path, each world alone takes the single-workgroup one."""
+    from velox_amd.device import Context
+    sc = _worlds(count=256)
+    steps = 3
+    print(f"batched 256 worlds, {len(sc.bodies)} bodies, {steps} steps", flush=True)
+    ctx.load_scene(sc)
+    ctx.step(sc.dt, steps)
+    assert ctx.stats()["contacts"] > (1 << 14)
+    batched = ctx.bodies()
+    alone = Context()
+    try:
+        for w in range(0, sc.worlds, 8):
+            sub = sc.world(w)
+            alone.load_scene(sub)
+            alone.step(sc.dt, steps)
+            got = batched[sc.world_of == w]
+            assert bits_equal(body_state(got), body_state(alone.bodies())), f"world {w} differs after {steps} steps"
+    finally:
+        alone.close()
+
+
 def test_batched_world_matches_oracle(ctx):
     sc = _worlds()
     steps = 10

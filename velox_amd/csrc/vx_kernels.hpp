@@ -519,6 +519,65 @@ __global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __r
     }
 }
 
+// The same greedy colouring by dataflow across the whole device, for large
+// contact sets: lanes walk contacts with a grid-wide stride, wait for their
+// predecessors with relaxed device-scope loads, and take one device-scope
+// acquire before reading the predecessors' masks; a colour is published with
+// a device-scope release.  The grid is sized to be co-resident, so the lowest
+// uncoloured contact is always running and ready.
+constexpr int kColorBlocks = 256;
+
+__global__ __launch_bounds__(kThreads) void kGreedyColorDevice(const vx_body* __restrict__ bodies,
+                                                               const vx_contact* __restrict__ cs, uint32_t m,
+                                                               const uint32_t* __restrict__ pred,
+                                                               unsigned long long* __restrict__ maskAfter,
+                                                               int* __restrict__ nextAfter, uint32_t* color,
+                                                               uint32_t* __restrict__ colorCount,
+                                                               uint32_t* __restrict__ numColors,
+                                                               uint32_t* __restrict__ error) {
+    const uint32_t stride = gridDim.x * kThreads;
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    uint32_t spins = 0;
+    while (k < m) {
+        const uint32_t pa = pred[2 * k], pb = pred[2 * k + 1];
+        const bool readyA = pa == kNone || __hip_atomic_load(&color[pa >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kNone;
+        const bool readyB = pb == kNone || __hip_atomic_load(&color[pb >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kNone;
+        if (readyA && readyB) {
+            if (pa != kNone || pb != kNone) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            const vx_contact& c = cs[k];
+            const bool sa = !constrainedSide(bodies, c, 0), sb = !constrainedSide(bodies, c, 1);
+            unsigned long long maskA = 0, maskB = 0;
+            int nextA = 64, nextB = 64;
+            if (!sa && pa != kNone) { maskA = maskAfter[pa]; nextA = nextAfter[pa]; }
+            if (!sb && pb != kNone) { maskB = maskAfter[pb]; nextB = nextAfter[pb]; }
+            const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
+            int col;
+            if (~used) {
+                col = __ffsll((long long)~used) - 1;
+            } else {
+                int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
+                col = na > nb2 ? na : nb2;
+                if (!sa) nextA = col + 1;
+                if (!sb) nextB = col + 1;
+            }
+            if (col < 64) {
+                if (!sa) maskA |= 1ull << col;
+                if (!sb) maskB |= 1ull << col;
+            }
+            maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
+            maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
+            atomicAdd(&colorCount[col], 1u);
+            atomicMax(numColors, (uint32_t)col + 1u);
+            __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            k += stride;
+            spins = 0;
+        } else if (++spins > (1u << 26)) {
+            atomicOr(error, 1u);
+            break;
+        }
+    }
+}
+
 // ----------------------------------------------------------------- solver --
 __global__ __launch_bounds__(kThreads) void kGather(const vx_contact* __restrict__ in, const uint32_t* __restrict__ order,
                                                     uint32_t m, vx_contact* __restrict__ out) {

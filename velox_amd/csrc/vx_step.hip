@@ -413,6 +413,10 @@ uint32_t narrowPhase(vxhip_ctx* c, uint32_t np, float dt) {
 }
 
 // Greedy coloring of c->contacts[0..m) in order; colours into c->color, returns K.
+// Contact sets up to this size are coloured by one workgroup (workgroup-scope
+// hand-offs); larger ones across the device.
+constexpr uint32_t kColorSingleLimit = 1u << 14;
+
 uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     hipStream_t s = c->stream;
     uint32_t* k = c->counters.p;
@@ -437,8 +441,13 @@ uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + cb,
                             tempFor(c, vxhost::radixTempBytes(e)), s));
     if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, cb, c->pred.p);
-    kGreedyColor<<<1, kColorThreads, 0, s>>>(c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p, c->color.p,
-                                             c->colorCount.p, k + cColors, k + cColorError);
+    if (m <= kColorSingleLimit)
+        kGreedyColor<<<1, kColorThreads, 0, s>>>(c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p,
+                                                 c->color.p, c->colorCount.p, k + cColors, k + cColorError);
+    else
+        kGreedyColorDevice<<<std::min<uint32_t>(blocks(m), kColorBlocks), kThreads, 0, s>>>(
+            c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p, c->color.p, c->colorCount.p, k + cColors,
+            k + cColorError);
     HIPCK(hipGetLastError());
     readCounters(c);
     if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: coloring did not converge");
