@@ -49,12 +49,29 @@ struct Convex {                 // gjk.h:13-29
         case CHull: {
             V3 d = rotateInv(orientation, dir);
             uint32_t best = 0;
-            float bestDot = dot(d, v3(hullPts[0]));
-            for (uint32_t i = 1; i < hullCount; ++i) {
-                float t = dot(d, v3(hullPts[i]));
-                if (t > bestDot) { bestDot = t; best = i; }
+            V3 bestP = v3(hullPts[0]);
+            float bestDot = dot(d, bestP);
+            // Eight points per step, loaded together (one memory round trip
+            // instead of eight), compared in index order: the first maximum
+            // wins as in the one-at-a-time scan.
+            uint32_t i = 1;
+            for (; i + 8 <= hullCount; i += 8) {
+                V3 p[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) p[q] = v3(hullPts[i + q]);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    float t = dot(d, p[q]);
+                    if (t > bestDot) { bestDot = t; best = i + q; bestP = p[q]; }
+                }
             }
-            return rotate(orientation, v3(hullPts[best]));
+            for (; i < hullCount; ++i) {
+                V3 p = v3(hullPts[i]);
+                float t = dot(d, p);
+                if (t > bestDot) { bestDot = t; best = i; bestP = p; }
+            }
+            (void)best;
+            return rotate(orientation, bestP);
         }
         case CCylinder: {
             V3 d = rotateInv(orientation, dir);
