@@ -140,6 +140,12 @@ int vxhip_world_load(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx
  * returns to a single world.  Reset by vxhip_world_load. */
 int vxhip_world_set_worlds(vxhip_ctx* ctx, const uint32_t* worldOf, uint32_t worldCount);
 
+/* Solver tuning (no reference counterpart): islands of more than `limit`
+ * contacts are solved colour by colour over the whole device instead of by
+ * one workgroup per island batch (default 4096; results are identical either
+ * way, only the schedule changes). */
+int vxhip_world_set_island_limit(vxhip_ctx* ctx, uint32_t limit);
+
 /* Advances the resident world `steps` times by dt: the device form of
  * World::stepImpl's physics (world.cpp:3975-4397 without events, sleeping,
  * joints or host callbacks): orientation renormalisation, integration, broad

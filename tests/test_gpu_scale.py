@@ -46,6 +46,30 @@ def test_device_step_matches_oracle(ctx, name, param, steps, every):
     print(f"[gpu] {name}: {ctx.stats()}", flush=True)
 
 
+@pytest.mark.parametrize("limit", [4, 64])
+def test_big_island_path_matches_oracle(ctx, limit):
+    """Islands above `limit` contacts are solved colour by colour over the
+    whole device; the trajectory must stay bitwise equal to the oracle."""
+    from velox_amd.device import Context, Scene
+    sc = Scene("pile", 600)
+    steps = 30
+    print(f"pile 600, island limit {limit}, {steps} steps", flush=True)
+    c = Context()
+    try:
+        c.load_scene(sc)
+        c.set_island_limit(limit)
+        ow = OracleWorld(sc.bodies, sc.soup, dt=sc.dt, substeps=sc.substeps, gravity=sc.gravity)
+        for step in range(steps):
+            c.step(sc.dt, 1)
+            ow.step(1)
+            got = c.bodies()
+            assert bits_equal(body_state(got), body_state(ow.bodies)), \
+                f"step {step + 1}: {_report(body_state(got), body_state(ow.bodies))}"
+        assert c.stats()["bigIslandContacts"] > 0
+    finally:
+        c.close()
+
+
 def test_full_size_invariants(ctx):
     """50,000 bodies, 60 steps: finite state, no body below the plane by more
     than the speculative slop, colouring valid (no two contacts of a colour

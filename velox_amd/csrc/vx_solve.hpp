@@ -408,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void kIslandKinds(const uint32_t* __restr
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     const uint32_t sz = i < n ? size[i] : 0u;
     if (sz) atomicMax(largest, sz);
-    const bool lds = sz > 0 && sz <= kLdsIsland;
+    const bool lds = sz > 0 && sz <= kLdsIsland && sz <= limit;
     const bool global = sz > kLdsIsland && sz <= limit;
     if (i < n) {
         ldsSize[i] = lds ? sz : 0u;

@@ -875,6 +875,13 @@ int vxhip_world_set_worlds(vxhip_ctx* c, const uint32_t* worldOf, uint32_t world
     });
 }
 
+int vxhip_world_set_island_limit(vxhip_ctx* c, uint32_t limit) {
+    return guarded([&] {
+        if (!c || limit == 0) fail(VXHIP_ERR_INVALID, "vxhip: invalid island limit");
+        c->islandLimit = limit;
+    });
+}
+
 int vxhip_world_sync(vxhip_ctx* c) {
     return guarded([&] { HIPCK(hipStreamSynchronize(c->stream)); });
 }

@@ -55,6 +55,7 @@ SIGNATURES = {
     "vxhip_world_load": (_i32, [_vp, _vp, _u32, _vp, _vp]),
     "vxhip_world_step": (_i32, [_vp, _f32, _u32]),
     "vxhip_world_set_worlds": (_i32, [_vp, _vp, _u32]),
+    "vxhip_world_set_island_limit": (_i32, [_vp, _u32]),
     "vxhip_world_sync": (_i32, [_vp]),
     "vxhip_world_download_bodies": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_upload_bodies": (_i32, [_vp, _vp, _u32]),
@@ -109,7 +110,7 @@ class Timings(C.Structure):
                 ("kernelLaunches", C.c_uint32 * 4), ("kernelVisits", C.c_uint64 * 4)]
 
     # Solver kernel families, in the order of the per-kernel arrays.
-    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorIdx")
+    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorIdx<0>")
 
     def as_dict(self):
         d = {}
@@ -211,6 +212,11 @@ class Context:
         self.load(scene.bodies, scene.soup, gravity=scene.gravity, substeps=scene.substeps, options=options)
         if getattr(scene, "worlds", 1) > 1:
             self.set_worlds(scene.world_of, scene.worlds)
+        return self
+
+    def set_island_limit(self, limit):
+        """Islands above `limit` contacts are solved colour by colour over the device."""
+        check(library().vxhip_world_set_island_limit(self._h, int(limit)))
         return self
 
     def set_worlds(self, world_of, worlds):
