@@ -524,11 +524,24 @@ int trace(const std::string& out, const std::string& which, int steps) {
 
 }  // namespace
 
+// Long full-configuration trajectories, one case per call (minutes each).
+int fixture(const std::string& out, const std::string& name) {
+    mkdir(out.c_str(), 0755);
+    if (name == "lattice1000") {  // the 1,000-box lattice over its full 300 steps
+        trajectory(out, name, vxscene::lattice(), {1, 100, 200, 300});
+        return 0;
+    }
+    std::fprintf(stderr, "unknown fixture %s\n", name.c_str());
+    return 2;
+}
+
 int main(int argc, char** argv) {
     std::string mode = argc > 1 ? argv[1] : "";
+    if (mode == "fixture" && argc > 3) return fixture(argv[2], argv[3]);
     if (mode == "trace" && argc > 4) return trace(argv[2], argv[3], std::atoi(argv[4]));
     if (mode == "fixtures" && argc > 2) return fixtures(argv[2]);
     if (mode == "bench" && argc > 4) return bench(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argc > 5 ? std::atoi(argv[5]) : 1);
-    std::fprintf(stderr, "usage: %s fixtures <outdir> | bench <scene> <steps> <workers> [warmup]\n", argv[0]);
+    std::fprintf(stderr, "usage: %s fixtures <outdir> | fixture <outdir> <case> | bench <scene> <steps> <workers> [warmup]\n",
+                 argv[0]);
     return 2;
 }

@@ -24,6 +24,9 @@ def main():
         if os.path.isdir(path):
             shutil.rmtree(path)
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixtures", HERE], cwd=ROOT)
+    # The full 300-step 1,000-box lattice (several minutes of strict CPU).
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "lattice1000"],
+                          cwd=ROOT)
 
 
 if __name__ == "__main__":

@@ -31,7 +31,7 @@ def test_integration_matches_reference(ctx):
     assert bits_equal(bodies, want), _report(bodies, want)
 
 
-@pytest.mark.parametrize("name", ["freeflight", "lattice64", "lattice64_cone", "pile", "terrain"])
+@pytest.mark.parametrize("name", ["freeflight", "lattice64", "lattice64_cone", "pile", "terrain", "lattice1000"])
 def test_trajectory_matches_reference(ctx, name):
     case = load_case(name)
     meta = case["meta"]
