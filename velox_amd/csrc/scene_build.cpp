@@ -379,6 +379,7 @@ vxscene::Scene vxSceneByName(const std::string& name, uint32_t param, bool& ok) 
     if (name == "terrain") return vxscene::terrain(24, 0.5f, 1.0f, param ? param : 96);
     if (name == "freeflight") return vxscene::freeFlight(param ? param : 128);
     if (name == "batch") return vxscene::batchWorlds(param ? param : 4096);
+    if (name == "ccdstress") return vxscene::ccdStress();
     // The exact descriptions behind the golden trajectory fixtures.
     if (name == "fixture-pile") return vxscene::pile(160, 5.0f, 6.0f);
     if (name == "fixture-terrain") return vxscene::terrain(16, 0.5f, 0.8f, 40);

@@ -341,4 +341,83 @@ inline Scene batchWorlds(uint32_t worlds = 4096, uint32_t seed = kSeedBatch) {
     return s;
 }
 
+// CCD stress: 4,096 spheres (r = 0.05 m, density 1000) fired at 2 km/s in a
+// 10-degree cone at a wall of 100 x 100 stacked unit boxes; 256 unit boxes
+// spinning at 1.5 km/s tip speed (1732 rad/s about random axes) above a
+// 256 x 256 heightfield at 0.5 m spacing whose heights are five octaves of
+// seeded sinusoidal noise scaled to 3 m amplitude.
+inline Scene ccdStress(uint32_t seed = kSeedCcd) {
+    Scene s;
+    Rng r(seed);
+    const float pi = 3.14159265f;
+    HeightfieldSpec hf;
+    const uint32_t side = 256;
+    hf.width = hf.depth = side;
+    hf.cell = 0.5f;
+    hf.origin[0] = hf.origin[2] = -0.5f * hf.cell * (float)(side - 1);
+    hf.origin[1] = 0.0f;
+    hf.heights.assign((size_t)side * side, 0.0f);
+    float phase[5][2], freq = 0.05f, amp = 1.0f, total = 0.0f;
+    for (int o = 0; o < 5; ++o) { phase[o][0] = r.range(0.0f, 2.0f * pi); phase[o][1] = r.range(0.0f, 2.0f * pi); }
+    for (int o = 0; o < 5; ++o, freq *= 2.0f, amp *= 0.5f) {
+        total += amp;
+        for (uint32_t z = 0; z < side; ++z)
+            for (uint32_t x = 0; x < side; ++x) {
+                const float wx = hf.origin[0] + hf.cell * (float)x, wz = hf.origin[2] + hf.cell * (float)z;
+                hf.heights[(size_t)z * side + x] += amp * std::sin(freq * wx + phase[o][0]) * std::cos(freq * wz + phase[o][1]);
+            }
+    }
+    for (float& h : hf.heights) h *= 3.0f / total;
+    s.heightfields.push_back(hf);
+    BodySpec terrain;
+    terrain.kind = Heightfield;
+    terrain.mass = 0.0f;
+    terrain.friction = 0.6f;
+    terrain.restitution = 0.0f;
+    s.bodies.push_back(terrain);
+    auto box = [&](float x, float y, float z) {
+        BodySpec b;
+        b.kind = Box;
+        b.half[0] = b.half[1] = b.half[2] = 0.5f;
+        b.mass = 1000.0f;
+        b.pos[0] = x; b.pos[1] = y; b.pos[2] = z;
+        b.friction = 0.6f;
+        b.restitution = 0.0f;
+        return b;
+    };
+    for (int j = 0; j < 100; ++j)  // the wall, x in [20, 21]
+        for (int k = 0; k < 100; ++k) s.bodies.push_back(box(20.5f, 4.5f + 1.002f * (float)j, -49.5f + 1.002f * (float)k));
+    const float cone = 10.0f * pi / 180.0f;
+    for (int j = 0; j < 64; ++j)  // bullets
+        for (int k = 0; k < 64; ++k) {
+            BodySpec b;
+            b.kind = Sphere;
+            b.radius = 0.05f;
+            b.mass = 1000.0f * (4.0f / 3.0f) * pi * 0.05f * 0.05f * 0.05f;
+            b.pos[0] = -10.0f;
+            b.pos[1] = 10.0f + 80.0f * (float)j / 63.0f;
+            b.pos[2] = -40.0f + 80.0f * (float)k / 63.0f;
+            // Uniform direction in the cone about +x.
+            const float cosT = 1.0f - r.next() * (1.0f - std::cos(cone)), sinT = std::sqrt(1.0f - cosT * cosT);
+            const float phi = r.range(0.0f, 2.0f * pi);
+            b.vel[0] = 2000.0f * cosT;
+            b.vel[1] = 2000.0f * sinT * std::cos(phi);
+            b.vel[2] = 2000.0f * sinT * std::sin(phi);
+            b.friction = 0.6f;
+            b.restitution = 0.0f;
+            s.bodies.push_back(b);
+        }
+    const float omega = 1500.0f / 0.8660254f;
+    for (int j = 0; j < 16; ++j)  // spinning boxes above the terrain
+        for (int k = 0; k < 16; ++k) {
+            BodySpec b = box(-50.0f + 3.0f * (float)j, 6.0f + r.range(0.0f, 2.0f), -24.0f + 3.0f * (float)k);
+            r.quat(b.quat);
+            float ax = r.normal(), ay = r.normal(), az = r.normal();
+            const float l = std::sqrt(ax * ax + ay * ay + az * az);
+            b.angVel[0] = omega * ax / l; b.angVel[1] = omega * ay / l; b.angVel[2] = omega * az / l;
+            s.bodies.push_back(b);
+        }
+    return s;
+}
+
 }  // namespace vxscene
