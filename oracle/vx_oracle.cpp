@@ -313,7 +313,6 @@ uint32_t ccdNet(vx_body* bodies, const vx_vec3* prevPos, const vx_quat* prevQ, c
             t = fmn(1.0f, t + g.gap / bound);
         }
         if (t >= 1.0f) continue;
-        if (getenv("VXO_DEBUG")) fprintf(stderr, "ccd rescue %u %u end=%g t=%.9g n=%g %g %g\n", i, j, end.gap, t, n.x, n.y, n.z);
         ++rescued;
         a.position = raw(pa0 + da * t);
         a.orientation = raw(qa0);

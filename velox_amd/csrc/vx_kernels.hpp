@@ -591,27 +591,6 @@ __global__ __launch_bounds__(kThreads) void kScatterBack(const vx_contact* __res
     if (k < m) out[order[k]] = in[k];
 }
 
-// Writes the solved bodies of one contact back: only the rows the solver
-// changes (velocities; warm start also refreshes nothing else).
-__device__ inline void storeVel(vx_body* bodies, uint32_t i, const vx_body& b) {
-    bodies[i].velocity = b.velocity;
-    bodies[i].angularVelocity = b.angularVelocity;
-}
-
-// One colour: contacts touch disjoint dynamic bodies (backend_cuda.cu:301-311).
-__global__ __launch_bounds__(kThreads) void kWarmStartColor(vx_body* __restrict__ bodies, vx_contact* __restrict__ cs,
-                                                            const uint32_t* __restrict__ colorStart, uint32_t color) {
-    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
-    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= count) return;
-    vx_contact c = cs[first + k];
-    vx_body a = bodies[c.a], b = bodies[c.b];
-    if (isSensor(a) || isSensor(b)) return;
-    warmStart(a, b, c);
-    if (isDynamic(a)) storeVel(bodies, c.a, a);
-    if (isDynamic(b)) storeVel(bodies, c.b, b);
-    cs[first + k].point = c.point;
-}
 
 // world.cpp:4364-4397, one colour at a time.
 __global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,

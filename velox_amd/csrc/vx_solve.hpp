@@ -281,14 +281,6 @@ __device__ __attribute__((always_inline)) inline void solveRow(SolveArrays sa, u
     sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
 }
 
-template <bool kCone>
-__global__ __launch_bounds__(kThreads) void kSweepColor(SolveArrays sa, const uint32_t* __restrict__ colorStart,
-                                                        uint32_t color) {
-    const uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
-    const uint32_t stride = gridDim.x * kThreads;
-    for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < count; k += stride) solveRow<kCone>(sa, first + k);
-}
-
 }  // namespace vxk
 
 namespace vxk {
