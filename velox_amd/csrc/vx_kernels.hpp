@@ -207,8 +207,8 @@ __device__ inline uint32_t waveReserve(uint32_t count, uint32_t* counter) {
 template <bool kWrite>
 __device__ inline uint32_t gridVisit(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
                                      const float4* __restrict__ hi, const uint64_t* __restrict__ keys,
-                                     const uint32_t* __restrict__ idx, uint32_t n, const Grid& g, uint32_t s, int nb,
-                                     uint64_t* out, uint32_t at, uint32_t cap) {
+                                     const uint32_t* __restrict__ idx, const uint2* __restrict__ cellRange, uint32_t n,
+                                     const Grid& g, uint32_t s, int nb, uint64_t* out, uint32_t at, uint32_t cap) {
     const uint64_t wkey = keys[s];
     const uint64_t base = (wkey / g.ncells) * g.ncells;  // the body's world
     const uint64_t key = wkey - base;
@@ -226,7 +226,17 @@ __device__ inline uint32_t gridVisit(const vx_body* __restrict__ bodies, const f
             for (int dz = -1; dz <= 1; ++dz) {
                 if ((dz < 0 && z == 0) || (uint32_t)(z + dz) >= g.nz) continue;
                 const uint64_t nk = base + ((uint64_t)(x + dx) * g.ny + (y + dy)) * g.nz + (z + dz);
-                for (uint32_t t = lowerBound(keys, n, nk); t < n && keys[t] == nk; ++t) {
+                uint32_t t0, t1;
+                if (cellRange) {
+                    const uint2 r = cellRange[nk];
+                    t0 = r.x;
+                    t1 = r.y;
+                } else {
+                    t0 = lowerBound(keys, n, nk);
+                    t1 = t0;
+                    while (t1 < n && keys[t1] == nk) ++t1;
+                }
+                for (uint32_t t = t0; t < t1; ++t) {
                     const uint32_t j = idx[t];
                     if (j <= i) continue;
                     const float4 lj = lo[j], hj = hi[j];
@@ -244,14 +254,28 @@ __device__ inline uint32_t gridVisit(const vx_body* __restrict__ bodies, const f
 
 __global__ __launch_bounds__(kThreads) void kGridPairs(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
                                                        const float4* __restrict__ hi, const uint64_t* __restrict__ keys,
-                                                       const uint32_t* __restrict__ idx, uint32_t n, Grid g, int nb,
+                                                       const uint32_t* __restrict__ idx,
+                                                       const uint2* __restrict__ cellRange, uint32_t n, Grid g, int nb,
                                                        uint64_t* __restrict__ pairs, uint32_t* __restrict__ pairCount,
                                                        uint32_t cap) {
     uint32_t s = blockIdx.x * kThreads + threadIdx.x;
     const bool valid = s < n && keys[s] != g.sentinel;
-    uint32_t cnt = valid ? gridVisit<false>(bodies, lo, hi, keys, idx, n, g, s, nb, nullptr, 0, 0) : 0u;
+    uint32_t cnt = valid ? gridVisit<false>(bodies, lo, hi, keys, idx, cellRange, n, g, s, nb, nullptr, 0, 0) : 0u;
     uint32_t at = waveReserve(cnt, pairCount);
-    if (valid && cnt) gridVisit<true>(bodies, lo, hi, keys, idx, n, g, s, nb, pairs, at, cap);
+    if (valid && cnt) gridVisit<true>(bodies, lo, hi, keys, idx, cellRange, n, g, s, nb, pairs, at, cap);
+}
+
+// Sorted-key run of every occupied cell ([start, end) into the sorted
+// arrays); empty cells keep (0, 0).  Replaces the binary searches of the
+// neighbourhood walk when the cell count is small enough for a table.
+__global__ __launch_bounds__(kThreads) void kCellRanges(const uint64_t* __restrict__ keys, uint32_t n, uint64_t sentinel,
+                                                        uint2* __restrict__ cellRange) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= n) return;
+    const uint64_t key = keys[t];
+    if (key == sentinel) return;
+    if (t == 0 || keys[t - 1] != key) cellRange[key].x = t;
+    if (t + 1 == n || keys[t + 1] != key) cellRange[key].y = t + 1;
 }
 
 // Oversize bodies (planes, meshes, huge sweeps) against every body

@@ -115,6 +115,7 @@ struct vxhip_ctx {
     // interact) and the per-world oversize lists of the broad phase.
     Buf<uint32_t> worldOf, worldOversize, worldStart, worldFill, oversizeByWorld;
     uint32_t worldCount = 1, worldN = 0;
+    Buf<uint2> cellRange;  // per-cell run of the sorted keys (small grids)
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta, sMetaK;
     Buf<float4> sSide;
@@ -305,6 +306,8 @@ int bodyBits(uint32_t n) { return vxhost::bitsFor(n ? n - 1 : 0) ? vxhost::bitsF
 
 // --------------------------------------------------------------- stages --
 // Broad phase: fills c->pairs (sorted packed contact keys), returns count.
+constexpr uint64_t kCellTableMax = 1ull << 22;
+
 uint32_t broadPhase(vxhip_ctx* c, float dt) {
     const uint32_t n = c->n;
     hipStream_t s = c->stream;
@@ -356,13 +359,22 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
                                                                     c->worldStart.p, c->worldFill.p,
                                                                     c->oversizeByWorld.p);
     }
+    // Cell table for grids of up to kCellTableMax cells; larger grids (many
+    // batched worlds) use binary searches.
+    const bool table = g.sentinel <= kCellTableMax;
+    if (table) {
+        c->cellRange.reserve(g.sentinel);
+        fill32(c, (uint32_t*)c->cellRange.p, (uint32_t)(2 * g.sentinel), 0);
+        kCellRanges<<<blocks(n), kThreads, 0, s>>>(c->keys.p, n, g.sentinel, c->cellRange.p);
+    }
     const int nb = bodyBits(n);
     if (c->pairCap == 0) c->pairCap = 32 * n + 1024;
     for (;;) {
         c->pairs.reserve(c->pairCap);
         kResetCounters<<<1, 64, 0, s>>>(k);  // only cPairs matters here; others re-derived
-        kGridPairs<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->keys.p, c->idx.p, n, g, nb,
-                                                  c->pairs.p, k + cPairs, c->pairCap);
+        kGridPairs<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->keys.p, c->idx.p,
+                                                  table ? c->cellRange.p : nullptr, n, g, nb, c->pairs.p, k + cPairs,
+                                                  c->pairCap);
         if (oversizeCount && batched)
             kOversizePairsWorld<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->worldOf.p,
                                                                c->worldStart.p, c->oversizeByWorld.p, n, g, nb,
