@@ -178,6 +178,18 @@ typedef struct {
     uint8_t reserved;
 } vx_solver_options;
 
+/* ContactEvent (world.h:65-71) with dense body indices in place of BodyId
+ * handles (the host maps indices to handles), 44 B.  type: 0 Begin,
+ * 1 Persist, 2 End (ContactEventType, world.h:46). */
+typedef struct {
+    uint32_t a, b;
+    float point[3];
+    float normal[3];
+    float impulse;
+    uint32_t type;
+    uint32_t sensor;
+} vx_contact_event;
+
 /* Raw-pointer view of the flat static geometry (MeshSoupView, backend.h:25-34)
  * plus the element counts a device copy needs. */
 typedef struct {
@@ -200,6 +212,7 @@ static_assert(offsetof(vx_body, ccdTuning) == 156, "Body layout");
 static_assert(offsetof(vx_body, shape) == 232, "Body layout");
 static_assert(offsetof(vx_body, compoundCount) == 296, "Body layout");
 static_assert(sizeof(vx_contact) == 128, "Contact layout");
+static_assert(sizeof(vx_contact_event) == 44, "ContactEvent record");
 static_assert(offsetof(vx_contact, spinningImpulse) == 116, "Contact layout");
 static_assert(sizeof(vx_mesh) == 52, "Mesh layout");
 static_assert(sizeof(vx_bvh_node) == 32, "BvhNode layout");

@@ -166,6 +166,17 @@ int vxhip_world_upload_bodies(vxhip_ctx* ctx, const vx_body* in, uint32_t n);
 /* Contacts solved by the last step, in color order (World's contacts_). */
 int vxhip_world_download_contacts(vxhip_ctx* ctx, vx_contact* out, uint32_t cap, uint32_t* count);
 
+/* Contact events of the last step (World::contactEvents, world.cpp:4399-4485):
+ * one Begin or Persist event per active pair in pair-key order, then End
+ * events for pairs active in the previous step only, in that step's key order.
+ * Bodies are dense indices; the host maps them to BodyId handles.
+ * VXHIP_ERR_CAPACITY with *count set when cap is short. */
+int vxhip_world_download_events(vxhip_ctx* ctx, vx_contact_event* out, uint32_t cap, uint32_t* count);
+
+/* Enables (default) or disables contact events in vxhip_world_step; either
+ * call restarts the Begin/Persist history. */
+int vxhip_world_set_events(vxhip_ctx* ctx, int enable);
+
 /* Counters of the last step (StepStats subset, world.h:175-208). */
 int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
 

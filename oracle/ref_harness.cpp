@@ -303,6 +303,26 @@ void writeMeta(const std::string& dir, const World& w, float dt, const std::vect
 // Reference World trajectory with CUDA-backend semantics: initial records,
 // soup, and body states after each listed step (plus the solved contacts of
 // the first listed step, in color order).
+// Contact events of the last step (world.cpp:4399-4485) as fixed 44-byte
+// records: dense body indices, point, normal, impulse, type, sensor.
+struct EventRecord { uint32_t a, b; float point[3], normal[3], impulse; uint32_t type, sensor; };
+static_assert(sizeof(EventRecord) == 44, "event record");
+std::vector<EventRecord> eventRecords(const World& w) {
+    std::vector<EventRecord> out;
+    for (const ContactEvent& e : w.events_) {
+        EventRecord r{};
+        r.a = e.a.slot();  // no body is ever removed here: slot == dense index
+        r.b = e.b.slot();
+        r.point[0] = e.point.x; r.point[1] = e.point.y; r.point[2] = e.point.z;
+        r.normal[0] = e.normal.x; r.normal[1] = e.normal.y; r.normal[2] = e.normal.z;
+        r.impulse = e.impulse;
+        r.type = (uint32_t)e.type;
+        r.sensor = e.sensor ? 1u : 0u;
+        out.push_back(r);
+    }
+    return out;
+}
+
 void trajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene,
                 const std::vector<int>& steps, bool coneBlock = false) {
     std::string dir = out + "/" + name;
@@ -322,6 +342,7 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
     for (int target : steps) {
         while (done < target) { w.step(scene.dt); ++done; }
         writeRecords(dir + "/state_" + std::to_string(target) + ".npy", states(w.bodies_));
+        writeRecords(dir + "/events_" + std::to_string(target) + ".npy", eventRecords(w));
         colorsJson += (target == steps[0] ? "" : ", ") + std::to_string(w.contacts_.size());
         kJson += (target == steps[0] ? "" : ", ") + std::to_string(static_cast<CudaSemantics*>(w.backend_.get())->colors);
         if (!wroteContacts) { writeRecords(dir + "/contacts_" + std::to_string(target) + ".npy", w.contacts_); wroteContacts = true; }

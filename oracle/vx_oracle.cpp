@@ -458,4 +458,84 @@ int vxo_world_step(vx_body* bodies, uint32_t n, const vx_soup* s, const vxo_conf
     return (int)m;
 }
 
+// Contact events of one step (world.cpp:4399-4485) from the step's contacts
+// as the solver left them (colour order, accumulated impulses).  prevKeys /
+// *prevKeyCount: the previous step's sorted unique pair keys, replaced on
+// return.  prevContacts: the previous step's contacts (canonical order), the
+// source of the representative of a pair whose bodies both went inert.
+// Returns the event count (events beyond cap are counted, not written).
+uint32_t vxo_contact_events(const vx_body* bodies, uint32_t n, const vx_contact* cs, uint32_t m, float dt,
+                            uint64_t* prevKeys, uint32_t* prevKeyCount, uint32_t keyCap,
+                            const vx_contact* prevContacts, uint32_t prevContactCount,
+                            vx_contact_event* out, uint32_t cap) {
+    auto canonicalKey = [](uint32_t a, uint32_t b) {
+        const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+        return (uint64_t)lo << 32 | hi;
+    };
+    std::vector<std::pair<uint64_t, uint32_t>> active;
+    for (uint32_t k = 0; k < m; ++k) {
+        const vx_contact& c = cs[k];
+        const vx_body &a = bodies[c.a], &b = bodies[c.b];
+        const bool sensor = isSensor(a) || isSensor(b);
+        const float lg = liveGap(a, b, c);
+        const bool swept = sensor && c.vn0 < 0.0f && c.gap <= -c.vn0 * dt + 1e-3f;
+        if (c.normalImpulse <= 1e-6f && lg > 1e-3f && !swept) continue;
+        active.emplace_back(canonicalKey(c.a, c.b), k);
+    }
+    std::sort(active.begin(), active.end());
+    std::vector<uint64_t> prev(prevKeys, prevKeys + *prevKeyCount), keys;
+    std::vector<vx_contact_event> ev;
+    auto event = [&](uint32_t lo, uint32_t hi, uint32_t type) {
+        vx_contact_event e{};
+        e.a = lo; e.b = hi; e.type = type;
+        e.sensor = (isSensor(bodies[lo]) || isSensor(bodies[hi])) ? 1u : 0u;
+        return e;
+    };
+    auto setRep = [](vx_contact_event& e, const vx_contact& c, uint32_t lo) {
+        e.impulse = c.normalImpulse;
+        e.point[0] = c.point.x; e.point[1] = c.point.y; e.point[2] = c.point.z;
+        // Narrow-phase normals point from B to A; events use the canonical
+        // low-to-high direction (world.cpp:4449-4451).
+        e.normal[0] = c.a == lo ? -c.normal.x : c.normal.x;
+        e.normal[1] = c.a == lo ? -c.normal.y : c.normal.y;
+        e.normal[2] = c.a == lo ? -c.normal.z : c.normal.z;
+    };
+    for (size_t i = 0; i < active.size();) {
+        const uint64_t key = active[i].first;
+        const uint32_t lo = (uint32_t)(key >> 32), hi = (uint32_t)key;
+        const bool persisted = std::binary_search(prev.begin(), prev.end(), key);
+        vx_contact_event e = event(lo, hi, persisted ? 1u : 0u);
+        bool set = false;
+        for (; i < active.size() && active[i].first == key; ++i) {
+            const vx_contact& c = cs[active[i].second];
+            if (!set || c.normalImpulse > e.impulse) { set = true; setRep(e, c, lo); }
+        }
+        keys.push_back(key);
+        ev.push_back(e);
+    }
+    for (uint64_t key : prev) {
+        if (std::binary_search(keys.begin(), keys.end(), key)) continue;
+        const uint32_t lo = (uint32_t)(key >> 32), hi = (uint32_t)key;
+        if (lo >= n || hi >= n) continue;
+        const bool inertLo = isStatic(bodies[lo]) || asleepFully(bodies[lo]);
+        const bool inertHi = isStatic(bodies[hi]) || asleepFully(bodies[hi]);
+        if (inertLo && inertHi) {
+            vx_contact_event e = event(lo, hi, 1u);
+            for (uint32_t k = 0; k < prevContactCount; ++k)
+                if (canonicalKey(prevContacts[k].a, prevContacts[k].b) == key) { setRep(e, prevContacts[k], lo); break; }
+            keys.push_back(key);
+            ev.push_back(e);
+            continue;
+        }
+        ev.push_back(event(lo, hi, 2u));
+    }
+    std::sort(keys.begin(), keys.end());
+    keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+    const uint32_t kc = (uint32_t)std::min<size_t>(keys.size(), keyCap);
+    for (uint32_t k = 0; k < kc; ++k) prevKeys[k] = keys[k];
+    *prevKeyCount = kc;
+    for (uint32_t k = 0; k < ev.size() && k < cap; ++k) out[k] = ev[k];
+    return (uint32_t)ev.size();
+}
+
 }  // extern "C"

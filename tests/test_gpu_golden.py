@@ -3,7 +3,7 @@ C ABI of libvxhip.  Needs an MI355X."""
 import numpy as np
 import pytest
 
-from vxfix import as_bodies, as_contacts, as_state, bits_equal, body_state, contact_bytes, default_options, load_case
+from vxfix import as_bodies, as_events, as_contacts, as_state, bits_equal, body_state, contact_bytes, default_options, load_case
 
 pytestmark = pytest.mark.gpu
 
@@ -50,6 +50,12 @@ def test_trajectory_matches_reference(ctx, name):
         assert bits_equal(got, want), f"{name} step {target}: " + _report(got, want)
         assert st["contacts"] == meta["contacts_at"][k]
         assert st["colors"] == meta["colors"][k]
+        if f"events_{target}" in case:
+            want_e = as_events(case[f"events_{target}"])
+            got_e = ctx.events()
+            assert len(got_e) == len(want_e), f"{name} step {target}: {len(got_e)} events, want {len(want_e)}"
+            assert np.array_equal(got_e.view(np.uint8), np.ascontiguousarray(want_e).view(np.uint8)), \
+                f"{name} step {target} events"
         if f"contacts_{target}" in case:
             want_c = as_contacts(case[f"contacts_{target}"])
             assert np.array_equal(contact_bytes(ctx.contacts()), contact_bytes(want_c))

@@ -43,6 +43,9 @@ def test_device_step_matches_oracle(ctx, name, param, steps, every):
             got = body_state(ctx.bodies())
             want = body_state(ref.bodies)
             assert bits_equal(got, want), f"{name} step {s}: " + _report(got, want)
+            got_e, want_e = ctx.events(), ref.last_events
+            assert len(got_e) == len(want_e), (name, s, len(got_e), len(want_e))
+            assert np.array_equal(got_e.view(np.uint8), np.ascontiguousarray(want_e).view(np.uint8)), (name, s)
     print(f"[gpu] {name}: {ctx.stats()}", flush=True)
 
 

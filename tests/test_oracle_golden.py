@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from vxfix import (CONTACT, OracleWorld, as_bodies, as_contacts, as_state, bits_equal, body_state,
+from vxfix import (as_events, CONTACT, OracleWorld, as_bodies, as_contacts, as_state, bits_equal, body_state,
                    contact_bytes, default_options, load_case, oracle, ptr)
 
 
@@ -56,6 +56,10 @@ def _run_trajectory(name, cone=False):
         assert bits_equal(got, want), f"{name} step {target}: " + _diff_report(got, want)
         assert w.stats.contacts == meta["contacts_at"][k], (name, target)
         assert w.stats.colors == meta["colors"][k], (name, target)
+        if f"events_{target}" in case:
+            want_e = as_events(case[f"events_{target}"])
+            assert np.array_equal(np.ascontiguousarray(w.last_events).view(np.uint8),
+                                  np.ascontiguousarray(want_e).view(np.uint8)), f"{name} step {target} events"
         if f"contacts_{target}" in case:
             want_c = as_contacts(case[f"contacts_{target}"])
             got_c = w.last_contacts

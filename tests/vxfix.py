@@ -47,6 +47,16 @@ def as_contacts(raw):
     return np.ascontiguousarray(raw).view(CONTACT).reshape(-1).copy()
 
 
+# vx_contact_event (include/vx_records.h), 44 B
+EVENT = np.dtype([("a", np.uint32), ("b", np.uint32), ("point", np.float32, 3), ("normal", np.float32, 3),
+                  ("impulse", np.float32), ("type", np.uint32), ("sensor", np.uint32)])
+assert EVENT.itemsize == 44
+
+
+def as_events(raw):
+    return np.ascontiguousarray(raw).view(EVENT).reshape(-1)
+
+
 def body_state(bodies):
     s = np.zeros(len(bodies), STATE)
     s["p"], s["q"], s["v"], s["w"] = bodies["position"], bodies["orientation"], bodies["velocity"], \
@@ -101,6 +111,7 @@ def oracle():
             "vxo_ccd": (u32, [vp, vp, vp, vp, u32, vp, f32]),
             "vxo_gap_at": (f32, [vp, u32, u32, vp, f32, vp]),
             "vxo_world_step": (i32, [vp, u32, vp, vp, f32, vp, vp, u32, vp, u32, vp]),
+            "vxo_contact_events": (u32, [vp, u32, vp, u32, f32, vp, vp, u32, vp, u32, vp, u32]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(lib, name)
@@ -125,14 +136,25 @@ class OracleWorld:
         self.contacts = np.zeros(cap, CONTACT)
         self.stats = OracleStats()
         self.last_contacts = self.contacts[:0]
+        self.event_keys = np.zeros(cap, np.uint64)
+        self.event_key_count = C.c_uint32(0)
+        self.events = np.zeros(2 * cap, EVENT)
+        self.last_events = self.events[:0]
 
     def step(self, n=1):
         lib = oracle()
         for _ in range(n):
+            prev_contacts = self.prev[:self.prev_count.value].copy()
             m = lib.vxo_world_step(ptr(self.bodies), len(self.bodies), C.byref(self._view), C.byref(self.cfg),
                                    self.dt, ptr(self.prev), C.byref(self.prev_count), len(self.prev),
                                    ptr(self.contacts), len(self.contacts), C.byref(self.stats))
             if m > len(self.contacts):
                 raise RuntimeError("oracle contact capacity exceeded")
             self.last_contacts = self.contacts[:m].copy()
+            ne = lib.vxo_contact_events(ptr(self.bodies), len(self.bodies), ptr(self.last_contacts), m, self.dt,
+                                        ptr(self.event_keys), C.byref(self.event_key_count), len(self.event_keys),
+                                        ptr(prev_contacts), len(prev_contacts), ptr(self.events), len(self.events))
+            if ne > len(self.events):
+                raise RuntimeError("oracle event capacity exceeded")
+            self.last_events = self.events[:ne].copy()
         return self

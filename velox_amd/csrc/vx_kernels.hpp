@@ -616,6 +616,188 @@ __global__ __launch_bounds__(kThreads) void kScatterBack(const vx_contact* __res
 }
 
 
+// ------------------------------------------------------------ contact events --
+// world.cpp:4399-4485.  Contacts in solve order (as the solver left them) are
+// keyed by their canonical pair (lo << nb | hi); inactive contacts get the
+// key `inactive` so that a stable sort puts them last and keeps, within a
+// pair, the solve order that picks the representative on ties.
+__global__ __launch_bounds__(kThreads) void kEventKeys(const vx_body* __restrict__ bodies,
+                                                       const vx_contact* __restrict__ cs, uint32_t m, float dt, int nb,
+                                                       uint64_t inactive, uint64_t* __restrict__ keys,
+                                                       uint32_t* __restrict__ vals, uint32_t* __restrict__ activeCount) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    bool act = false;
+    if (k < m) {
+        const vx_contact& c = cs[k];
+        const vx_body &a = bodies[c.a], &b = bodies[c.b];
+        const bool sensor = isSensor(a) || isSensor(b);
+        const float lg = liveGap(a, b, c);
+        const bool swept = sensor && c.vn0 < 0.0f && c.gap <= -c.vn0 * dt + 1e-3f;
+        act = !(c.normalImpulse <= 1e-6f && lg > 1e-3f && !swept);
+        const uint32_t lo = c.a < c.b ? c.a : c.b, hi = c.a < c.b ? c.b : c.a;
+        keys[k] = act ? ((uint64_t)lo << nb) | hi : inactive;
+        vals[k] = k;
+    }
+    waveReserve(act ? 1u : 0u, activeCount);
+}
+
+__device__ inline uint32_t lowerBound64(const uint64_t* keys, uint32_t n, uint64_t key) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (keys[mid] < key) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline vx_contact_event eventOf(const vx_body* bodies, uint32_t lo, uint32_t hi, uint32_t type) {
+    vx_contact_event e;
+    e.a = lo; e.b = hi;
+    e.point[0] = e.point[1] = e.point[2] = 0.0f;
+    e.normal[0] = e.normal[1] = e.normal[2] = 0.0f;
+    e.impulse = 0.0f;
+    e.type = type;
+    e.sensor = (isSensor(bodies[lo]) || isSensor(bodies[hi])) ? 1u : 0u;
+    return e;
+}
+
+// Representative contact of a pair: the narrow-phase normal points from B to
+// A, events from the high body to the low one (world.cpp:4443-4452).
+__device__ inline void eventRep(vx_contact_event& e, const vx_contact& c, uint32_t lo) {
+    e.impulse = c.normalImpulse;
+    e.point[0] = c.point.x; e.point[1] = c.point.y; e.point[2] = c.point.z;
+    e.normal[0] = c.a == lo ? -c.normal.x : c.normal.x;
+    e.normal[1] = c.a == lo ? -c.normal.y : c.normal.y;
+    e.normal[2] = c.a == lo ? -c.normal.z : c.normal.z;
+}
+
+__global__ __launch_bounds__(kThreads) void kEventRunHeads(const uint64_t* __restrict__ keys,
+                                                           const uint32_t* __restrict__ activeCount, uint32_t m,
+                                                           uint32_t* __restrict__ head) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= m) return;
+    head[t] = t < *activeCount && (t == 0 || keys[t - 1] != keys[t]) ? 1u : 0u;
+}
+
+// One event per active pair, in pair-key order: Begin, or Persist when the
+// pair was active in the previous step; the representative is the contact
+// with the largest normal impulse (the first in solve order on ties).
+__global__ __launch_bounds__(kThreads) void kEventRuns(const vx_body* __restrict__ bodies,
+                                                       const vx_contact* __restrict__ cs,
+                                                       const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ vals,
+                                                       const uint32_t* __restrict__ head,
+                                                       const uint32_t* __restrict__ runPos,
+                                                       const uint32_t* __restrict__ activeCount, uint32_t m, int nb,
+                                                       const uint64_t* __restrict__ prevKeys, uint32_t np,
+                                                       vx_contact_event* __restrict__ events,
+                                                       uint64_t* __restrict__ curKeys) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= m || !head[t]) return;
+    const uint32_t na = *activeCount;
+    const uint64_t key = keys[t];
+    const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
+    const uint32_t at = lowerBound64(prevKeys, np, key);
+    vx_contact_event e = eventOf(bodies, lo, hi, at < np && prevKeys[at] == key ? 1u : 0u);
+    bool set = false;
+    for (uint32_t u = t; u < na && keys[u] == key; ++u) {
+        const vx_contact& c = cs[vals[u]];
+        if (!set || c.normalImpulse > e.impulse) { set = true; eventRep(e, c, lo); }
+    }
+    events[runPos[t]] = e;
+    curKeys[runPos[t]] = key;
+}
+
+// Pairs active in the previous step and not in this one: End events, after
+// the active pairs, in previous-key order.  A pair whose bodies are both
+// inert (static or fully asleep) is flagged instead (the exact pass below).
+__global__ __launch_bounds__(kThreads) void kEventEnds(const vx_body* __restrict__ bodies, uint32_t n, int nb,
+                                                       const uint64_t* __restrict__ prevKeys, uint32_t np,
+                                                       const uint64_t* __restrict__ curKeys,
+                                                       const uint32_t* __restrict__ runs, uint32_t* __restrict__ endFlag,
+                                                       uint32_t* __restrict__ inertCount) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= np) return;
+    uint32_t flag = 0;
+    {
+        const uint64_t key = prevKeys[i];
+        const uint32_t r = *runs;
+        const uint32_t at = lowerBound64(curKeys, r, key);
+        if (!(at < r && curKeys[at] == key)) {
+            const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
+            if (lo < n && hi < n) {
+                const bool inert = (isStatic(bodies[lo]) || asleepFully(bodies[lo])) &&
+                                   (isStatic(bodies[hi]) || asleepFully(bodies[hi]));
+                if (inert) atomicAdd(inertCount, 1u);
+                flag = 1;
+            }
+        }
+    }
+    endFlag[i] = flag;
+}
+
+__global__ __launch_bounds__(kThreads) void kEventEndWrite(const vx_body* __restrict__ bodies, int nb,
+                                                           const uint64_t* __restrict__ prevKeys, uint32_t prevCap,
+                                                           const uint32_t* __restrict__ endFlag,
+                                                           const uint32_t* __restrict__ endPos,
+                                                           const uint32_t* __restrict__ runs,
+                                                           const uint32_t* __restrict__ inertCount,
+                                                           vx_contact_event* __restrict__ events) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= prevCap || !endFlag[i] || *inertCount) return;
+    const uint64_t key = prevKeys[i];
+    const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
+    events[*runs + endPos[i]] = eventOf(bodies, lo, hi, 2u);
+}
+
+// The exact sequential pass (world.cpp:4456-4481), taken only when a vanished
+// pair has both bodies inert: its Persist event is appended and its key
+// joins the searched key list, as in the reference.  One thread.
+__global__ void kEventEndsExact(const vx_body* __restrict__ bodies, uint32_t n, int nb,
+                                const uint64_t* __restrict__ prevKeys, uint32_t np,
+                                uint64_t* __restrict__ curKeys, uint32_t* __restrict__ runs,
+                                const uint32_t* __restrict__ inertCount, const vx_contact* __restrict__ prevContacts,
+                                uint32_t prevContactCount, vx_contact_event* __restrict__ events,
+                                uint32_t* __restrict__ eventCount) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || *inertCount == 0) return;
+    uint32_t nk = *runs, ne = *runs;
+    for (uint32_t i = 0; i < np; ++i) {
+        const uint64_t key = prevKeys[i];
+        // std::binary_search over the key list as it grows (libstdc++ lower_bound).
+        uint32_t first = 0, len = nk;
+        while (len > 0) {
+            const uint32_t half = len >> 1;
+            if (curKeys[first + half] < key) { first += half + 1; len -= half + 1; } else len = half;
+        }
+        if (first != nk && !(key < curKeys[first])) continue;
+        const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
+        if (lo >= n || hi >= n) continue;
+        const bool inert = (isStatic(bodies[lo]) || asleepFully(bodies[lo])) &&
+                           (isStatic(bodies[hi]) || asleepFully(bodies[hi]));
+        if (inert) {
+            vx_contact_event e = eventOf(bodies, lo, hi, 1u);
+            for (uint32_t k = 0; k < prevContactCount; ++k) {
+                const vx_contact& c = prevContacts[k];
+                const uint32_t clo = c.a < c.b ? c.a : c.b, chi = c.a < c.b ? c.b : c.a;
+                if (clo == lo && chi == hi) { eventRep(e, c, lo); break; }
+            }
+            curKeys[nk++] = key;
+            events[ne++] = e;
+            continue;
+        }
+        events[ne++] = eventOf(bodies, lo, hi, 2u);
+    }
+    // Sort the appended keys into the list (both parts are sorted).
+    for (uint32_t i = *runs; i < nk; ++i) {
+        const uint64_t v = curKeys[i];
+        uint32_t j = i;
+        while (j > 0 && curKeys[j - 1] > v) { curKeys[j] = curKeys[j - 1]; --j; }
+        curKeys[j] = v;
+    }
+    *runs = nk;
+    *eventCount = ne;
+}
+
 // world.cpp:4364-4397, one colour at a time.
 __global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,
                                                            const uint32_t* __restrict__ colorStart, uint32_t color, float h,

@@ -68,6 +68,21 @@ struct Buf {
         HIPCK(hipMalloc((void**)&p, want * sizeof(T)));
         cap = want;
     }
+    // Like reserve, but the first `keep` elements survive a reallocation.
+    void grow(size_t n, size_t keep, hipStream_t s) {
+        if (n <= cap) return;
+        T* old = p;
+        p = nullptr;
+        cap = 0;
+        size_t want = n + n / 4 + 64;
+        HIPCK(hipMalloc((void**)&p, want * sizeof(T)));
+        cap = want;
+        if (old && keep) HIPCK(hipMemcpyAsync(p, old, keep * sizeof(T), hipMemcpyDeviceToDevice, s));
+        if (old) {
+            HIPCK(hipStreamSynchronize(s));
+            (void)hipFree(old);
+        }
+    }
     void release() {
         if (p) (void)hipFree(p);
         p = nullptr;
@@ -116,6 +131,12 @@ struct vxhip_ctx {
     Buf<uint32_t> worldOf, worldOversize, worldStart, worldFill, oversizeByWorld;
     uint32_t worldCount = 1, worldN = 0;
     Buf<uint2> cellRange;  // per-cell run of the sorted keys (small grids)
+    // Contact events of the last step and the pair keys they leave for the next.
+    Buf<uint64_t> evKeys, evKeysTmp, evPrevKeys, evCurKeys;
+    Buf<uint32_t> evVals, evValsTmp, evHead, evRunPos, evEndFlag, evEndPos;
+    Buf<vx_contact_event> events;
+    uint32_t eventCount = 0, evPrevCount = 0;
+    bool eventsEnabled = true;
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta, sMetaK;
     Buf<float4> sSide;
@@ -162,7 +183,7 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cCount = 40
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39, cCount = 40
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -712,6 +733,55 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     HIPCK(hipGetLastError());
 }
 
+// Contact events (world.cpp:4399-4485) from the step's solved contacts; the
+// counts are read with the step's last counter readback.
+void contactEvents(vxhip_ctx* c, uint32_t m, float dt) {
+    hipStream_t s = c->stream;
+    uint32_t* k = c->counters.p;
+    const uint32_t n = c->n, np = c->evPrevCount;
+    const int nb = bodyBits(n);
+    const uint64_t inactive = 1ull << (2 * nb);
+    const size_t keyCap = (size_t)m + np + 1;
+    c->evKeys.reserve(m + 1); c->evKeysTmp.reserve(m + 1); c->evVals.reserve(m + 1); c->evValsTmp.reserve(m + 1);
+    c->evHead.reserve(m + 1); c->evRunPos.reserve(m + 1); c->evEndFlag.reserve(np + 1); c->evEndPos.reserve(np + 1);
+    c->evPrevKeys.grow(keyCap, np, s);  // holds the previous step's keys
+    c->evCurKeys.reserve(keyCap); c->events.reserve(keyCap);
+    if (m) {
+        kEventKeys<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, c->solve.p, m, dt, nb, inactive, c->evKeys.p, c->evVals.p,
+                                                  k + cEvActive);
+        HIPCK(vxhost::radixSort(c->evKeys.p, c->evVals.p, c->evKeysTmp.p, c->evValsTmp.p, m, 2 * nb + 1,
+                                tempFor(c, vxhost::radixTempBytes(m)), s));
+        kEventRunHeads<<<blocks(m), kThreads, 0, s>>>(c->evKeys.p, k + cEvActive, m, c->evHead.p);
+    }
+    HIPCK(vxhost::exclusiveScan(c->evHead.p, c->evRunPos.p, m, k + cEvRuns, tempFor(c, vxhost::scanTempBytes(m)), s));
+    if (m)
+        kEventRuns<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, c->solve.p, c->evKeys.p, c->evVals.p, c->evHead.p,
+                                                  c->evRunPos.p, k + cEvActive, m, nb, c->evPrevKeys.p, np,
+                                                  c->events.p, c->evCurKeys.p);
+    if (np) {
+        kEventEnds<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, n, nb, c->evPrevKeys.p, np, c->evCurKeys.p,
+                                                   k + cEvRuns, c->evEndFlag.p, k + cEvInert);
+        HIPCK(vxhost::exclusiveScan(c->evEndFlag.p, c->evEndPos.p, np, k + cEvEnds,
+                                    tempFor(c, vxhost::scanTempBytes(np)), s));
+        kEventEndWrite<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, nb, c->evPrevKeys.p, np, c->evEndFlag.p,
+                                                       c->evEndPos.p, k + cEvRuns, k + cEvInert, c->events.p);
+        kEventEndsExact<<<1, 1, 0, s>>>(c->bodies.p, n, nb, c->evPrevKeys.p, np, c->evCurKeys.p, k + cEvRuns,
+                                        k + cEvInert, c->prev.p, c->prevCount, c->events.p, k + cEvCount);
+    }
+    HIPCK(hipGetLastError());
+}
+
+// Host side of the events after the step's counter readback: event count and
+// the key list handed to the next step.
+void finishEvents(vxhip_ctx* c) {
+    const uint32_t* h = c->hostCounters;
+    const uint32_t runs = h[cEvRuns];
+    c->eventCount = h[cEvInert] ? h[cEvCount] : runs + (c->evPrevCount ? h[cEvEnds] : 0u);
+    std::swap(c->evPrevKeys.p, c->evCurKeys.p);
+    std::swap(c->evPrevKeys.cap, c->evCurKeys.cap);
+    c->evPrevCount = runs;
+}
+
 // One full device step of the resident world.
 void worldStep(vxhip_ctx* c, float dt) {
     injectFailureForTesting();
@@ -753,6 +823,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sPosition);
     if (m) positionCorrection(c, c->solve.p, K, h, cfg.options);
     markStage(c, sFinish);
+    if (c->eventsEnabled) contactEvents(c, m, dt);
     kClearForces<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n);
     c->prev.reserve(m ? m : 1);
     if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p);
@@ -764,6 +835,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sEnd);
     readCounters(c);
     if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: CCD dataflow did not converge");
+    if (c->eventsEnabled) finishEvents(c);
     if (c->hostCounters[cBatchError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: island batch exceeds its LDS capacity");
     collectProfile(c);
     c->stats.bodies = n;
@@ -856,6 +928,8 @@ int vxhip_world_load(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_s
         c->solveCount = 0;
         c->worldCount = 1;
         c->worldN = 0;
+        c->eventCount = 0;
+        c->evPrevCount = 0;
         HIPCK(hipStreamSynchronize(c->stream));
         c->loaded = true;
     });
@@ -922,6 +996,27 @@ int vxhip_world_download_contacts(vxhip_ctx* c, vx_contact* out, uint32_t cap, u
         if (c->solveCount)
             HIPCK(hipMemcpyAsync(out, c->solve.p, (size_t)c->solveCount * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_world_download_events(vxhip_ctx* c, vx_contact_event* out, uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        if (!c || !count || (!out && cap)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *count = c->eventCount;
+        if (c->eventCount > cap) fail(VXHIP_ERR_CAPACITY, "vxhip: event buffer too small");
+        if (c->eventCount)
+            HIPCK(hipMemcpyAsync(out, c->events.p, (size_t)c->eventCount * sizeof(vx_contact_event),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_world_set_events(vxhip_ctx* c, int enable) {
+    return guarded([&] {
+        if (!c) fail(VXHIP_ERR_INVALID, "vxhip: null context");
+        c->eventsEnabled = enable != 0;
+        c->eventCount = 0;
+        c->evPrevCount = 0;
     });
 }
 

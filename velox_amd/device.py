@@ -56,6 +56,8 @@ SIGNATURES = {
     "vxhip_world_step": (_i32, [_vp, _f32, _u32]),
     "vxhip_world_set_worlds": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_set_island_limit": (_i32, [_vp, _u32]),
+    "vxhip_world_download_events": (_i32, [_vp, _vp, _u32, _vp]),
+    "vxhip_world_set_events": (_i32, [_vp, _i32]),
     "vxhip_world_sync": (_i32, [_vp]),
     "vxhip_world_download_bodies": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_upload_bodies": (_i32, [_vp, _vp, _u32]),
@@ -265,6 +267,21 @@ class Context:
         t = Timings()
         check(library().vxhip_world_timings(self._h, C.byref(t)))
         return t.as_dict()
+
+    def events(self):
+        """Contact events of the last step (vx_contact_event records)."""
+        from .records import EVENT
+        count = C.c_uint32(0)
+        rc = library().vxhip_world_download_events(self._h, None, 0, C.byref(count))
+        if rc not in (OK, ERR_CAPACITY):
+            check(rc)
+        out = np.zeros(count.value, EVENT)
+        check(library().vxhip_world_download_events(self._h, ptr(out) if len(out) else None, len(out), C.byref(count)))
+        return out
+
+    def set_events(self, enable):
+        check(library().vxhip_world_set_events(self._h, 1 if enable else 0))
+        return self
 
     def stats(self):
         s = Stats()

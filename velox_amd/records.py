@@ -91,6 +91,12 @@ def ptr(a):
     return None if a is None or a.size == 0 else C.c_void_p(a.ctypes.data)
 
 
+# vx_contact_event (include/vx_records.h), 44 B
+EVENT = np.dtype([("a", np.uint32), ("b", np.uint32), ("point", np.float32, 3), ("normal", np.float32, 3),
+                  ("impulse", np.float32), ("type", np.uint32), ("sensor", np.uint32)])
+assert EVENT.itemsize == 44
+
+
 class SoupArrays:
     """Owns the numpy arrays behind one vx_soup view."""
 
