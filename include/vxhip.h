@@ -147,11 +147,12 @@ int vxhip_world_set_worlds(vxhip_ctx* ctx, const uint32_t* worldOf, uint32_t wor
 int vxhip_world_set_island_limit(vxhip_ctx* ctx, uint32_t limit);
 
 /* Advances the resident world `steps` times by dt: the device form of
- * World::stepImpl's physics (world.cpp:3975-4397 without events, sleeping,
- * joints or host callbacks): orientation renormalisation, integration, broad
- * phase, narrow phase, warm-start matching, greedy coloring, substepped
+ * World::stepImpl's physics (world.cpp:3975-4521 without joints, soft bodies
+ * or host callbacks): orientation renormalisation, integration, broad phase,
+ * narrow phase, wake pass, warm-start matching, greedy coloring, substepped
  * colored TGS solve, conservative-advancement net, positional correction,
- * force clearing.  Asynchronous with respect to the host. */
+ * contact events, sleeping (when enabled), force clearing.  Asynchronous with
+ * respect to the host. */
 int vxhip_world_step(vxhip_ctx* ctx, float dt, uint32_t steps);
 
 /* Waits for queued device work. */
@@ -179,6 +180,43 @@ int vxhip_world_set_events(vxhip_ctx* ctx, int enable);
 
 /* Counters of the last step (StepStats subset, world.h:175-208). */
 int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
+
+/* Island sleeping (SleepConfig, sleep.h:98-160; field meaning and defaults as
+ * there: 0.05, 0.05, 0.01, 0.5, 0.25, 0.25, 1, 1, 0.02, 10). */
+typedef struct vx_sleep_config {
+    float linearVelocityThreshold;
+    float angularVelocityThreshold;
+    float accelerationThreshold;
+    float timeToSleep;
+    float timeToDrowsy;
+    float drowsySimulationRate;  /* validated, not applied (as in the reference) */
+    float contactStabilityThreshold;
+    int32_t contactStabilityFrames;
+    uint8_t enableGradualSleep;
+    uint8_t enableContactStability;
+    uint8_t reserved[2];
+} vx_sleep_config;
+
+/* SleepStats (sleep.h:167-180) without the wall-clock field. */
+typedef struct vx_sleep_stats {
+    uint32_t totalDynamicBodies, awakeBodies, drowsyBodies, sleepingBodies;
+    uint32_t islandCount, sleepingIslandCount, drowsyIslandCount, awakeIslandCount;
+    uint64_t totalSleepTransitions, totalWakeTransitions, totalDrowsyTransitions;
+} vx_sleep_stats;
+
+/* Turns the device sleep update on with `config` (validated and clamped as
+ * SleepConfig::validate, sleep.h:143-159), or off with NULL (the default).
+ * While on, every vxhip_world_step ends like World::stepImpl
+ * (world.cpp:4491-4521): SleepManager::update (sleep.cpp:50-325) over the
+ * step's contacts, then the warm-start cache with the contacts of inert pairs
+ * carried over; the wake pass (world.cpp:4109-4128) runs whenever a body is
+ * asleep.  Contact pair history for events is kept either way.  Resets the
+ * per-body contact stability counters and the cumulative transition counts.
+ * Replaces World::setSleepConfig (world.h:741) for the device step. */
+int vxhip_world_set_sleep(vxhip_ctx* ctx, const vx_sleep_config* config);
+
+/* Sleep statistics after the last step (World::sleepStats, world.h:744). */
+int vxhip_world_sleep_stats(vxhip_ctx* ctx, vx_sleep_stats* out);
 
 /* Stage timings measured with HIP events on the context's own stream while
  * profiling is enabled (accumulated over steps; StepStats timings,

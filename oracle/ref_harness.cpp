@@ -72,7 +72,7 @@ template <class T> void writeRecords(const std::string& path, const std::vector<
 }
 
 // ------------------------------------------------------------ scene build --
-std::vector<BodyId> build(World& w, const vxscene::Scene& s) {
+std::vector<BodyId> build(World& w, const vxscene::Scene& s, bool sleep = false) {
     w.gravity = {s.gravity[0], s.gravity[1], s.gravity[2]};
     w.substeps = s.substeps;
     std::vector<BodyId> ids;
@@ -108,7 +108,9 @@ std::vector<BodyId> build(World& w, const vxscene::Scene& s) {
         }
         body.friction = b.friction;
         body.restitution = b.restitution;
-        body.enableSleep = 0;  // the device step has no sleep manager
+        // The sleep manager runs inside every reference World::step; scenes
+        // opt in per body (the device step's sleep update is off by default).
+        body.enableSleep = sleep ? b.enableSleep : 0;
         ids.push_back(id);
     }
     return ids;
@@ -324,11 +326,11 @@ std::vector<EventRecord> eventRecords(const World& w) {
 }
 
 void trajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene,
-                const std::vector<int>& steps, bool coneBlock = false) {
+                const std::vector<int>& steps, bool coneBlock = false, bool sleep = false) {
     std::string dir = out + "/" + name;
     mkdir(dir.c_str(), 0755);
     World w(BackendType::Cpu);
-    build(w, scene);
+    build(w, scene, sleep);
     SolverOptions o = w.solverOptions_;
     if (coneBlock) o.frictionModel = FrictionModel::ConeBlockSolver;
     w.setSolverOptions(o);
@@ -339,9 +341,21 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
     bool wroteContacts = false;
     std::string colorsJson = ", \"contacts_at\": [";
     std::string kJson = ", \"colors\": [";
+    std::string sleepJson = ", \"sleep_stats\": [";
     for (int target : steps) {
         while (done < target) { w.step(scene.dt); ++done; }
         writeRecords(dir + "/state_" + std::to_string(target) + ".npy", states(w.bodies_));
+        if (sleep) {  // full records: asleep byte and sleep timer
+            writeRecords(dir + "/bodies_" + std::to_string(target) + ".npy", w.bodies_);
+            const SleepStats& st = w.sleepStats();
+            const size_t v[11] = {st.totalDynamicBodies, st.awakeBodies, st.drowsyBodies, st.sleepingBodies,
+                                  st.islandCount, st.sleepingIslandCount, st.drowsyIslandCount, st.awakeIslandCount,
+                                  (size_t)st.totalSleepTransitions, (size_t)st.totalWakeTransitions,
+                                  (size_t)st.totalDrowsyTransitions};
+            sleepJson += target == steps[0] ? "[" : ", [";
+            for (int q = 0; q < 11; ++q) sleepJson += (q ? ", " : "") + std::to_string(v[q]);
+            sleepJson += "]";
+        }
         writeRecords(dir + "/events_" + std::to_string(target) + ".npy", eventRecords(w));
         colorsJson += (target == steps[0] ? "" : ", ") + std::to_string(w.contacts_.size());
         kJson += (target == steps[0] ? "" : ", ") + std::to_string(static_cast<CudaSemantics*>(w.backend_.get())->colors);
@@ -349,7 +363,7 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
         std::fprintf(stderr, "  %s step %d: %zu contacts\n", name.c_str(), target, w.contacts_.size());
     }
     writeRecords(dir + "/bodies_out.npy", w.bodies_);
-    writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]");
+    writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]" + (sleep ? sleepJson + "]" : std::string()));
 }
 
 // Reference narrow phase on snapshots of a reference trajectory: records,
@@ -550,6 +564,24 @@ int fixture(const std::string& out, const std::string& name) {
     mkdir(out.c_str(), 0755);
     if (name == "lattice1000") {  // the 1,000-box lattice over its full 300 steps
         trajectory(out, name, vxscene::lattice(), {1, 100, 200, 300});
+        return 0;
+    }
+    if (name == "settle") {  // sleeping on: towers fall asleep, a late strike wakes one
+        trajectory(out, name, vxscene::settle(), {1, 20, 31, 40, 41, 60, 120, 236, 237, 238, 239, 240, 260, 300}, false, true);
+        return 0;
+    }
+    if (name == "sleepcount") {  // per-step sleep counts (tuning aid)
+        vxscene::Scene s = vxscene::settle();
+        World w(BackendType::Cpu);
+        build(w, s, true);
+        w.backend_.reset(new CudaSemantics());
+        for (int k = 1; k <= 400; ++k) {
+            w.step(s.dt);
+            const SleepStats& st = w.sleepStats();
+            std::fprintf(stderr, "step %d: awake %u drowsy %u asleep %u contacts %zu events %zu prev %zu\n", k,
+                         (unsigned)st.awakeBodies, (unsigned)st.drowsyBodies, (unsigned)st.sleepingBodies,
+                         w.contacts_.size(), w.events_.size(), w.prevContacts_.size());
+        }
         return 0;
     }
     std::fprintf(stderr, "unknown fixture %s\n", name.c_str());

@@ -27,6 +27,8 @@ def main():
     # The full 300-step 1,000-box lattice (several minutes of strict CPU).
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "lattice1000"],
                           cwd=ROOT)
+    # Sleeping on: towers fall asleep island by island, a late strike wakes one.
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "settle"], cwd=ROOT)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,80 @@
+"""Device sleeping against the reference World with sleeping on (bitwise).
+
+The `settle` fixture is written by the reference's own World::step with its
+SleepManager (oracle/ref_harness.cpp, `fixture settle`): eight box towers and
+eight spheres settle, go Drowsy at step 31 and Asleep at step 40 island by
+island (tower 7 holds a body that never sleeps), and a sphere dropped from
+80 m strikes tower 0 at step 237 and wakes it one body per step through the
+wake pass.  Every listed step compares body states, the asleep byte and sleep
+timer of every body, the contact events (including Persist events of inert
+pairs, whose representative comes from the warm-start cache as std::sort
+leaves it), and SleepStats.  Needs an MI355X."""
+import numpy as np
+import pytest
+
+from vxfix import as_bodies, as_events, as_state, bits_equal, body_state, default_options, load_case
+
+pytestmark = pytest.mark.gpu
+
+STATS = ("totalDynamicBodies", "awakeBodies", "drowsyBodies", "sleepingBodies", "islandCount",
+         "sleepingIslandCount", "drowsyIslandCount", "awakeIslandCount", "totalSleepTransitions",
+         "totalWakeTransitions", "totalDrowsyTransitions")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from velox_amd.device import Context
+    c = Context()
+    yield c
+    c.close()
+
+
+def _bad_rows(got, want):
+    a8 = np.ascontiguousarray(got).view(np.uint8).reshape(len(got), -1)
+    b8 = np.ascontiguousarray(want).view(np.uint8).reshape(len(want), -1)
+    return np.nonzero((a8 != b8).any(axis=1))[0].tolist()
+
+
+def _run_settle(ctx, events_on=True):
+    case = load_case("settle")
+    meta = case["meta"]
+    opts = default_options(friction_model=meta["frictionModel"], velocity_iterations=meta["velocityIterations"],
+                           position_iterations=meta["positionIterations"])
+    ctx.load(as_bodies(case["bodies_in"]), case["soup"], gravity=tuple(meta["gravity"]), substeps=meta["substeps"],
+             options=opts)
+    ctx.set_events(events_on)
+    ctx.set_sleep()
+    done = 0
+    for k, target in enumerate(meta["steps"]):
+        print(f"[gpu] settle step {target} events={events_on}", flush=True)
+        ctx.step(np.float32(meta["dt"]), target - done)
+        done = target
+        yield case, k, target
+
+
+def test_settle_sleep_and_wake_match_reference(ctx):
+    for case, k, target in _run_settle(ctx):
+        meta = case["meta"]
+        bodies = ctx.bodies()
+        want_b = as_bodies(case[f"bodies_{target}"])
+        got, want = body_state(bodies), as_state(case[f"state_{target}"])
+        assert bits_equal(got, want), f"step {target}: states differ at {_bad_rows(got, want)[:8]}"
+        assert np.array_equal(bodies["asleep"], want_b["asleep"]), f"step {target}: asleep bytes"
+        assert bits_equal(bodies["sleepTimer"], want_b["sleepTimer"]), f"step {target}: sleep timers"
+        st = ctx.stats()
+        assert st["contacts"] == meta["contacts_at"][k], f"step {target}: contacts"
+        want_e = as_events(case[f"events_{target}"])
+        got_e = ctx.events()
+        assert len(got_e) == len(want_e), f"step {target}: {len(got_e)} events, want {len(want_e)}"
+        assert bits_equal(got_e, want_e), f"step {target}: events differ at {_bad_rows(got_e, want_e)[:8]}"
+        sl = ctx.sleep_stats()
+        assert [sl[n] for n in STATS] == meta["sleep_stats"][k], f"step {target}: sleep stats {sl}"
+
+
+def test_settle_without_event_download_matches(ctx):
+    """Events switched off for the caller: the pair history that decides the
+    carried contacts is still kept, so the bodies follow the same trajectory."""
+    for case, k, target in _run_settle(ctx, events_on=False):
+        got, want = body_state(ctx.bodies()), as_state(case[f"state_{target}"])
+        assert bits_equal(got, want), f"step {target}: states differ at {_bad_rows(got, want)[:8]}"
+        assert len(ctx.events()) == 0

@@ -57,6 +57,7 @@ struct BodySpec {
     uint32_t hullFirst = 0, hullCount = 0;  // into Scene::hullPoints (xyz triples)
     uint32_t hfIndex = 0;         // into Scene::heightfields
     uint32_t world = 0;           // batched scenes: owning world
+    uint8_t enableSleep = 0;      // Body::enableSleep (body.h:155)
 };
 
 struct HeightfieldSpec {
@@ -110,7 +111,8 @@ inline double hullVolume(const float* p, uint32_t n) {
 
 // Velox's demo scenes seed the same way; one seed per workload.
 constexpr uint32_t kSeedLattice = 1001u, kSeedDebris = 2002u, kSeedCcd = 3003u,
-                   kSeedBatch = 4004u, kSeedDiff = 5005u;
+                   kSeedBatch = 4004u, kSeedDiff = 5005u,
+                   kSeedSleep = 6006u;
 
 // 1,000 boxes (half-extent 0.5, density 1000) in a 10x10x10 lattice at 1.1 m
 // spacing with U(-0.05,0.05) jitter and uniform orientations, on a static
@@ -338,6 +340,56 @@ inline Scene batchWorlds(uint32_t worlds = 4096, uint32_t seed = kSeedBatch) {
             }
         }
     }
+    return s;
+}
+
+// Sleep and wake: eight towers of three unit boxes (1 mm gaps, U(-0.02,0.02)
+// x/z jitter) and eight resting spheres (r = 0.4) on a plane settle and fall
+// asleep island by island; the middle box of tower 7 has sleeping disabled, so
+// its island stays awake.  A sphere (r = 0.5, 50 kg) released at y = 80 above
+// tower 0 strikes it after about 3.9 s and wakes it.  Friction 0.6,
+// restitution 0.
+inline Scene settle(uint32_t seed = kSeedSleep) {
+    Scene s;
+    Rng r(seed);
+    s.bodies.push_back(plane(0.6f, 0.0f));
+    for (int t = 0; t < 8; ++t)
+        for (int j = 0; j < 3; ++j) {
+            BodySpec b;
+            b.kind = Box;
+            b.mass = 100.0f;
+            b.pos[0] = 3.0f * (float)(t % 4) - 4.5f + r.range(-0.02f, 0.02f);
+            b.pos[1] = 0.501f + 1.001f * (float)j;
+            b.pos[2] = 3.0f * (float)(t / 4) - 1.5f + r.range(-0.02f, 0.02f);
+            b.friction = 0.6f;
+            b.restitution = 0.0f;
+            b.enableSleep = (t == 7 && j == 1) ? 0 : 1;
+            s.bodies.push_back(b);
+        }
+    for (int k = 0; k < 8; ++k) {
+        BodySpec b;
+        b.kind = Sphere;
+        b.radius = 0.4f;
+        b.mass = 20.0f;
+        b.pos[0] = 2.0f * (float)k - 7.0f;
+        b.pos[1] = 0.401f;
+        b.pos[2] = 4.5f;
+        b.friction = 0.6f;
+        b.restitution = 0.0f;
+        b.enableSleep = 1;
+        s.bodies.push_back(b);
+    }
+    BodySpec striker;
+    striker.kind = Sphere;
+    striker.radius = 0.5f;
+    striker.mass = 50.0f;
+    striker.pos[0] = s.bodies[1].pos[0];
+    striker.pos[1] = 80.0f;
+    striker.pos[2] = s.bodies[1].pos[2];
+    striker.friction = 0.6f;
+    striker.restitution = 0.0f;
+    striker.enableSleep = 1;
+    s.bodies.push_back(striker);
     return s;
 }
 

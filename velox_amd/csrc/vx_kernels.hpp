@@ -1,5 +1,6 @@
 // Device kernels of the HIP device step (one body, pair or contact per lane).
 #pragma once
+#include "vx_stdsort.hpp"
 #include "vxd_solver.hpp"
 
 namespace vxk {
@@ -398,19 +399,35 @@ __global__ __launch_bounds__(kThreads) void kCompact(const vx_contact* __restric
 }
 
 // -------------------------------------------------------- warm starting --
-// world.cpp:4111-4128; order-dependent, so one lane walks the contacts.  Only
-// launched when a body is asleep.
-__global__ void kWakePass(vx_body* bodies, const vx_contact* cs, uint32_t m) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    for (uint32_t k = 0; k < m; ++k) {
-        vx_body& a = bodies[cs[k].a];
-        vx_body& b = bodies[cs[k].b];
-        if (isSensor(a) || isSensor(b)) continue;
-        bool impact = cs[k].vn0 < -0.1f;
-        bool movingA = lengthSq(v3(a.velocity)) + lengthSq(v3(a.angularVelocity)) > 1e-3f;
-        bool movingB = lengthSq(v3(b.velocity)) + lengthSq(v3(b.angularVelocity)) > 1e-3f;
-        if (asleepFully(a) && !isStatic(b) && !asleepFully(b) && (impact || movingB)) { a.asleep = 0; a.sleepTimer = 0.0f; }
-        if (asleepFully(b) && !isStatic(a) && !asleepFully(a) && (impact || movingA)) { b.asleep = 0; b.sleepTimer = 0.0f; }
+// world.cpp:4109-4128.  Order-dependent (a body woken by one contact can wake
+// the next), so one lane applies the rule in contact order; but only to
+// contacts with a fully asleep body, which the workgroup finds 1024 at a
+// time.  Bodies only ever wake here, so a stale "asleep" read in the parallel
+// filter can only add a contact that the exact rule then leaves alone.
+constexpr int kWakeThreads = 1024;
+__global__ __launch_bounds__(kWakeThreads) void kWakePass(vx_body* bodies, const vx_contact* cs, uint32_t m) {
+    __shared__ uint8_t relevant[kWakeThreads];
+    for (uint32_t base = 0; base < m; base += kWakeThreads) {
+        const uint32_t k = base + threadIdx.x;
+        bool r = false;
+        if (k < m) r = asleepFully(bodies[cs[k].a]) || asleepFully(bodies[cs[k].b]);
+        relevant[threadIdx.x] = r ? 1 : 0;
+        if (__syncthreads_or(r) && threadIdx.x == 0) {
+            const uint32_t end = m - base < (uint32_t)kWakeThreads ? m - base : (uint32_t)kWakeThreads;
+            for (uint32_t j = 0; j < end; ++j) {
+                if (!relevant[j]) continue;
+                const vx_contact& c = cs[base + j];
+                vx_body& a = bodies[c.a];
+                vx_body& b = bodies[c.b];
+                if (isSensor(a) || isSensor(b)) continue;
+                const bool impact = c.vn0 < -0.1f;
+                const bool movingA = lengthSq(v3(a.velocity)) + lengthSq(v3(a.angularVelocity)) > 1e-3f;
+                const bool movingB = lengthSq(v3(b.velocity)) + lengthSq(v3(b.angularVelocity)) > 1e-3f;
+                if (asleepFully(a) && !isStatic(b) && !asleepFully(b) && (impact || movingB)) { a.asleep = 0; a.sleepTimer = 0.0f; }
+                if (asleepFully(b) && !isStatic(a) && !asleepFully(a) && (impact || movingA)) { b.asleep = 0; b.sleepTimer = 0.0f; }
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -708,94 +725,198 @@ __global__ __launch_bounds__(kThreads) void kEventRuns(const vx_body* __restrict
     curKeys[runPos[t]] = key;
 }
 
-// Pairs active in the previous step and not in this one: End events, after
-// the active pairs, in previous-key order.  A pair whose bodies are both
-// inert (static or fully asleep) is flagged instead (the exact pass below).
-__global__ __launch_bounds__(kThreads) void kEventEnds(const vx_body* __restrict__ bodies, uint32_t n, int nb,
-                                                       const uint64_t* __restrict__ prevKeys, uint32_t np,
-                                                       const uint64_t* __restrict__ curKeys,
-                                                       const uint32_t* __restrict__ runs, uint32_t* __restrict__ endFlag,
-                                                       uint32_t* __restrict__ inertCount) {
+// Pairs of the previous step (prevKeys: ascending, unique) missing from this
+// step's active keys A = curKeys[0, nA): End events, or Persist when both
+// bodies are inert (static or fully asleep), after the active pairs in
+// previous-key order (world.cpp:4454-4481).
+//
+// The reference looks each previous key up with std::binary_search over a
+// list that grows as inert keys are appended behind A.  Previous keys ascend,
+// so every appended key is below the key searched, and the search sees the
+// predicate "element < key" as T^p F^q T^b: the p keys of A below the key,
+// the q = nA - p others, then the b keys appended so far.  libstdc++'s
+// lower_bound over that pattern ends at p or at the end (checked for all
+// p, q, b < 40), so a key absent from A is never found, and a key of A is
+// found only if the search over (p, q, b) ends at p.  A pair in A is never
+// inert (the broad phase drops pairs of two inert bodies), so the appended
+// keys are the inert absent ones, and b follows from a scan.
+__device__ inline bool growingSearchFinds(uint32_t p, uint32_t nA, uint32_t b) {
+    uint32_t first = 0, len = nA + b;
+    while (len > 0) {
+        const uint32_t half = len >> 1, mid = first + half;
+        if (mid < p || mid >= nA) { first = mid + 1; len -= half + 1; } else len = half;
+    }
+    return first == p;
+}
+
+enum : uint32_t { kEndInA = 1u, kEndValid = 2u, kEndInert = 4u };
+
+// Per previous key: its lower bound p in A, whether A holds it, whether both
+// bodies exist and are inert; appended[i] = 1 for an inert key absent from A.
+__global__ __launch_bounds__(kThreads) void kEventEndsClassify(const vx_body* __restrict__ bodies, uint32_t n, int nb,
+                                                               const uint64_t* __restrict__ prevKeys, uint32_t np,
+                                                               const uint64_t* __restrict__ curKeys,
+                                                               const uint32_t* __restrict__ runs,
+                                                               uint32_t* __restrict__ endP, uint32_t* __restrict__ endBits,
+                                                               uint32_t* __restrict__ appended) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= np) return;
-    uint32_t flag = 0;
-    {
-        const uint64_t key = prevKeys[i];
-        const uint32_t r = *runs;
-        const uint32_t at = lowerBound64(curKeys, r, key);
-        if (!(at < r && curKeys[at] == key)) {
-            const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
-            if (lo < n && hi < n) {
-                const bool inert = (isStatic(bodies[lo]) || asleepFully(bodies[lo])) &&
-                                   (isStatic(bodies[hi]) || asleepFully(bodies[hi]));
-                if (inert) atomicAdd(inertCount, 1u);
-                flag = 1;
-            }
-        }
+    const uint64_t key = prevKeys[i];
+    const uint32_t nA = *runs;
+    const uint32_t p = lowerBound64(curKeys, nA, key);
+    const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
+    uint32_t bits = p < nA && curKeys[p] == key ? kEndInA : 0u;
+    if (lo < n && hi < n) {
+        bits |= kEndValid;
+        if (inert(bodies[lo]) && inert(bodies[hi])) bits |= kEndInert;
     }
-    endFlag[i] = flag;
+    endP[i] = p;
+    endBits[i] = bits;
+    appended[i] = !(bits & kEndInA) && (bits & kEndInert) ? 1u : 0u;
 }
 
-__global__ __launch_bounds__(kThreads) void kEventEndWrite(const vx_body* __restrict__ bodies, int nb,
-                                                           const uint64_t* __restrict__ prevKeys, uint32_t prevCap,
-                                                           const uint32_t* __restrict__ endFlag,
-                                                           const uint32_t* __restrict__ endPos,
+__device__ inline bool endEmits(uint32_t bits, uint32_t p, uint32_t nA, uint32_t b) {
+    const bool found = (bits & kEndInA) && growingSearchFinds(p, nA, b);
+    return !found && (bits & kEndValid);
+}
+
+// emit[i] = 1 for a key that yields an event; appended keys are compacted
+// (ascending) into appKeys at their rank.
+__global__ __launch_bounds__(kThreads) void kEventEndsEmit(const uint64_t* __restrict__ prevKeys, uint32_t np,
                                                            const uint32_t* __restrict__ runs,
-                                                           const uint32_t* __restrict__ inertCount,
-                                                           vx_contact_event* __restrict__ events) {
+                                                           const uint32_t* __restrict__ endP,
+                                                           const uint32_t* __restrict__ endBits,
+                                                           const uint32_t* __restrict__ appRank,
+                                                           uint32_t* __restrict__ emit, uint64_t* __restrict__ appKeys) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= prevCap || !endFlag[i] || *inertCount) return;
+    if (i >= np) return;
+    const uint32_t bits = endBits[i], b = appRank[i];
+    emit[i] = endEmits(bits, endP[i], *runs, b) ? 1u : 0u;
+    if (!(bits & kEndInA) && (bits & kEndInert)) appKeys[b] = prevKeys[i];
+}
+
+// First contact of pair {lo, hi} in the previous step's cache, which is sorted
+// by (a << 32 | b); a pair's contacts share one orientation (a, b), either
+// way round, and the (lo, hi) run sorts before a (hi, lo) run.  mp when the
+// pair has none.
+__device__ inline uint32_t cachedRun(const vx_contact* prev, uint32_t mp, uint64_t key) {
+    uint32_t l = 0, h = mp;
+    while (l < h) { const uint32_t mid = (l + h) >> 1; if (pairKeyOf(prev[mid]) < key) l = mid + 1; else h = mid; }
+    return l < mp && pairKeyOf(prev[l]) == key ? l : mp;
+}
+
+__device__ inline uint32_t firstCached(const vx_contact* prev, uint32_t mp, uint32_t lo, uint32_t hi) {
+    const uint32_t t = cachedRun(prev, mp, (uint64_t)lo << 32 | hi);
+    return t < mp ? t : cachedRun(prev, mp, (uint64_t)hi << 32 | lo);
+}
+
+// The events: Persist (representative: the pair's first cached contact) for
+// inert pairs, End otherwise.
+__global__ __launch_bounds__(kThreads) void kEventEndsWrite(const vx_body* __restrict__ bodies, int nb,
+                                                            const uint64_t* __restrict__ prevKeys, uint32_t np,
+                                                            const uint32_t* __restrict__ runs,
+                                                            const uint32_t* __restrict__ endP,
+                                                            const uint32_t* __restrict__ endBits,
+                                                            const uint32_t* __restrict__ appRank,
+                                                            const uint32_t* __restrict__ emitPos,
+                                                            const vx_contact* __restrict__ prev, uint32_t mp,
+                                                            vx_contact_event* __restrict__ events) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t bits = endBits[i], nA = *runs;
+    if (!endEmits(bits, endP[i], nA, appRank[i])) return;
     const uint64_t key = prevKeys[i];
     const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
-    events[*runs + endPos[i]] = eventOf(bodies, lo, hi, 2u);
+    const bool persist = !(bits & kEndInA) && (bits & kEndInert);
+    vx_contact_event e = eventOf(bodies, lo, hi, persist ? 1u : 2u);
+    if (persist) {
+        const uint32_t t = firstCached(prev, mp, lo, hi);
+        if (t < mp) eventRep(e, prev[t], lo);
+    }
+    events[nA + emitPos[i]] = e;
 }
 
-// The exact sequential pass (world.cpp:4456-4481), taken only when a vanished
-// pair has both bodies inert: its Persist event is appended and its key
-// joins the searched key list, as in the reference.  One thread.
-__global__ void kEventEndsExact(const vx_body* __restrict__ bodies, uint32_t n, int nb,
-                                const uint64_t* __restrict__ prevKeys, uint32_t np,
-                                uint64_t* __restrict__ curKeys, uint32_t* __restrict__ runs,
-                                const uint32_t* __restrict__ inertCount, const vx_contact* __restrict__ prevContacts,
-                                uint32_t prevContactCount, vx_contact_event* __restrict__ events,
-                                uint32_t* __restrict__ eventCount) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || *inertCount == 0) return;
-    uint32_t nk = *runs, ne = *runs;
-    for (uint32_t i = 0; i < np; ++i) {
-        const uint64_t key = prevKeys[i];
-        // std::binary_search over the key list as it grows (libstdc++ lower_bound).
-        uint32_t first = 0, len = nk;
-        while (len > 0) {
-            const uint32_t half = len >> 1;
-            if (curKeys[first + half] < key) { first += half + 1; len -= half + 1; } else len = half;
-        }
-        if (first != nk && !(key < curKeys[first])) continue;
-        const uint32_t lo = (uint32_t)(key >> nb), hi = (uint32_t)(key & ((1ull << nb) - 1));
-        if (lo >= n || hi >= n) continue;
-        const bool inert = (isStatic(bodies[lo]) || asleepFully(bodies[lo])) &&
-                           (isStatic(bodies[hi]) || asleepFully(bodies[hi]));
-        if (inert) {
-            vx_contact_event e = eventOf(bodies, lo, hi, 1u);
-            for (uint32_t k = 0; k < prevContactCount; ++k) {
-                const vx_contact& c = prevContacts[k];
-                const uint32_t clo = c.a < c.b ? c.a : c.b, chi = c.a < c.b ? c.b : c.a;
-                if (clo == lo && chi == hi) { eventRep(e, c, lo); break; }
-            }
-            curKeys[nk++] = key;
-            events[ne++] = e;
-            continue;
-        }
-        events[ne++] = eventOf(bodies, lo, hi, 2u);
+// The key list handed to the next step: A merged with the appended keys
+// (disjoint, both ascending).
+__global__ __launch_bounds__(kThreads) void kEventKeysMerge(const uint64_t* __restrict__ curKeys,
+                                                            const uint32_t* __restrict__ runs,
+                                                            const uint64_t* __restrict__ appKeys,
+                                                            const uint32_t* __restrict__ appCount, uint32_t cap,
+                                                            uint64_t* __restrict__ merged) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t nA = *runs, nB = *appCount;
+    if (j >= cap || j >= nA + nB) return;
+    if (j < nA) merged[j + lowerBound64(appKeys, nB, curKeys[j])] = curKeys[j];
+    else merged[(j - nA) + lowerBound64(curKeys, nA, appKeys[j - nA])] = appKeys[j - nA];
+}
+
+// ----------------------------------------------------- warm-start cache --
+// Contacts cached for the next step (world.cpp:4492-4521): this step's, plus,
+// for every pair an inert Persist event kept alive, the first of its cached
+// contacts (later ones of that pair count as already cached).
+__global__ __launch_bounds__(kThreads) void kCarryFlags(const vx_contact* __restrict__ prev, uint32_t mp, int nb,
+                                                        const uint64_t* __restrict__ appKeys,
+                                                        const uint32_t* __restrict__ appCount,
+                                                        uint32_t* __restrict__ carry) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= mp) return;
+    const vx_contact& c = prev[t];
+    bool keep = t == 0 || pairKeyOf(prev[t - 1]) != pairKeyOf(c);
+    if (keep) {
+        const uint32_t lo = c.a < c.b ? c.a : c.b, hi = c.a < c.b ? c.b : c.a;
+        const uint64_t key = ((uint64_t)lo << nb) | hi;
+        const uint32_t nB = *appCount, at = lowerBound64(appKeys, nB, key);
+        keep = at < nB && appKeys[at] == key;
     }
-    // Sort the appended keys into the list (both parts are sorted).
-    for (uint32_t i = *runs; i < nk; ++i) {
-        const uint64_t v = curKeys[i];
-        uint32_t j = i;
-        while (j > 0 && curKeys[j - 1] > v) { curKeys[j] = curKeys[j - 1]; --j; }
-        curKeys[j] = v;
+    carry[t] = keep ? 1u : 0u;
+}
+
+// Sort keys of the cache candidates: this step's contacts (canonical order)
+// then the previous cache, whose entries that are not carried sort last.
+__global__ __launch_bounds__(kThreads) void kCacheKeys(const vx_contact* __restrict__ cur, uint32_t m,
+                                                       const vx_contact* __restrict__ prev, uint32_t mp, int nb,
+                                                       const uint32_t* __restrict__ carry, uint64_t none,
+                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= m + mp) return;
+    const bool own = j < m;
+    const vx_contact& c = own ? cur[j] : prev[j - m];
+    keys[j] = own || carry[j - m] ? ((uint64_t)c.a << nb) | c.b : none;
+    vals[j] = j;
+}
+
+// In a step where a body fell asleep, some pair may turn inert with several
+// contacts cached; the order std::sort leaves them in then decides what the
+// next steps report and keep.  The cache candidates are put in the order the
+// reference hands to std::sort (this step's contacts in solve order, then
+// the carried ones) and sorted by the libstdc++ restatement.  One thread.
+__global__ void kCacheStdSort(const vx_contact* __restrict__ solve, const uint32_t* __restrict__ order, uint32_t m,
+                              const vx_contact* __restrict__ prev, uint32_t mp, int nb,
+                              const uint32_t* __restrict__ carry, const uint32_t* __restrict__ fellAsleep,
+                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    if (threadIdx.x != 0 || blockIdx.x != 0 || *fellAsleep == 0) return;
+    uint32_t n = 0;
+    for (uint32_t j = 0; j < m; ++j, ++n) {
+        keys[n] = ((uint64_t)solve[j].a << nb) | solve[j].b;
+        vals[n] = order[j];
     }
-    *runs = nk;
-    *eventCount = ne;
+    for (uint32_t t = 0; t < mp; ++t)
+        if (carry[t]) {
+            keys[n] = ((uint64_t)prev[t].a << nb) | prev[t].b;
+            vals[n++] = m + t;
+        }
+    vxsort::sort({keys, vals}, n);
+}
+
+__global__ __launch_bounds__(kThreads) void kCacheGather(const vx_contact* __restrict__ cur, uint32_t m,
+                                                         const vx_contact* __restrict__ prev, uint32_t mp,
+                                                         const uint32_t* __restrict__ vals,
+                                                         const uint32_t* __restrict__ carried,
+                                                         vx_contact* __restrict__ out) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t >= m + mp || t >= m + *carried) return;
+    const uint32_t v = vals[t];
+    out[t] = v < m ? cur[v] : prev[v - m];
 }
 
 // world.cpp:4364-4397, one colour at a time.

@@ -8,6 +8,7 @@
 #include <vector>
 #include "vx_ccd.hpp"
 #include "vx_solve.hpp"
+#include "vx_sleep.hpp"
 #include "vx_internal.hpp"
 #include "../../include/vxhip.h"
 
@@ -137,6 +138,18 @@ struct vxhip_ctx {
     Buf<vx_contact_event> events;
     uint32_t eventCount = 0, evPrevCount = 0;
     bool eventsEnabled = true;
+    Buf<uint64_t> evAppKeys, evMerged;
+    Buf<uint32_t> evEndP, evEndBits, evApp, evEmit;
+    // Sleeping (off unless configured) and the warm-start cache it needs.
+    bool sleepOn = false;
+    vxk::SleepParams sleepParams{};
+    vx_sleep_config sleepCfg{};
+    Buf<uint32_t> slParent, slFlags, slIslandMin, slRootState;
+    Buf<uint16_t> slStability;
+    vx_sleep_stats sleepStats{};
+    Buf<vx_contact> prevStage, prevNext;
+    Buf<uint32_t> carry, carryRank, cacheVals, cacheValsTmp;
+    Buf<uint64_t> cacheKeys, cacheKeysTmp;
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta, sMetaK;
     Buf<float4> sSide;
@@ -183,7 +196,9 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39, cCount = 40
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
+    cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
+    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cCount = 52
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -317,6 +332,10 @@ void validateScene(const vx_body* b, uint32_t n, const vx_soup* s) {
 }
 
 void setBodies(vxhip_ctx* c, const vx_body* b, uint32_t n) {
+    if (n != c->n || !c->slStability.p) {  // contact stability counters start at zero (sleep.cpp:64)
+        c->slStability.reserve(n ? n : 1);
+        HIPCK(hipMemsetAsync(c->slStability.p, 0, (size_t)(n ? n : 1) * sizeof(uint16_t), c->stream));
+    }
     c->n = n;
     upload(c, c->bodies, b, n);
     c->anyAsleep = false;
@@ -743,9 +762,11 @@ void contactEvents(vxhip_ctx* c, uint32_t m, float dt) {
     const uint64_t inactive = 1ull << (2 * nb);
     const size_t keyCap = (size_t)m + np + 1;
     c->evKeys.reserve(m + 1); c->evKeysTmp.reserve(m + 1); c->evVals.reserve(m + 1); c->evValsTmp.reserve(m + 1);
-    c->evHead.reserve(m + 1); c->evRunPos.reserve(m + 1); c->evEndFlag.reserve(np + 1); c->evEndPos.reserve(np + 1);
+    c->evHead.reserve(m + 1); c->evRunPos.reserve(m + 1);
+    c->evEndP.reserve(np + 1); c->evEndBits.reserve(np + 1); c->evApp.reserve(np + 1); c->evEmit.reserve(np + 1);
     c->evPrevKeys.grow(keyCap, np, s);  // holds the previous step's keys
-    c->evCurKeys.reserve(keyCap); c->events.reserve(keyCap);
+    c->evCurKeys.reserve(keyCap); c->evMerged.reserve(keyCap); c->evAppKeys.reserve(np + 1);
+    c->events.reserve(keyCap);
     if (m) {
         kEventKeys<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, c->solve.p, m, dt, nb, inactive, c->evKeys.p, c->evVals.p,
                                                   k + cEvActive);
@@ -759,14 +780,17 @@ void contactEvents(vxhip_ctx* c, uint32_t m, float dt) {
                                                   c->evRunPos.p, k + cEvActive, m, nb, c->evPrevKeys.p, np,
                                                   c->events.p, c->evCurKeys.p);
     if (np) {
-        kEventEnds<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, n, nb, c->evPrevKeys.p, np, c->evCurKeys.p,
-                                                   k + cEvRuns, c->evEndFlag.p, k + cEvInert);
-        HIPCK(vxhost::exclusiveScan(c->evEndFlag.p, c->evEndPos.p, np, k + cEvEnds,
-                                    tempFor(c, vxhost::scanTempBytes(np)), s));
-        kEventEndWrite<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, nb, c->evPrevKeys.p, np, c->evEndFlag.p,
-                                                       c->evEndPos.p, k + cEvRuns, k + cEvInert, c->events.p);
-        kEventEndsExact<<<1, 1, 0, s>>>(c->bodies.p, n, nb, c->evPrevKeys.p, np, c->evCurKeys.p, k + cEvRuns,
-                                        k + cEvInert, c->prev.p, c->prevCount, c->events.p, k + cEvCount);
+        kEventEndsClassify<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, n, nb, c->evPrevKeys.p, np, c->evCurKeys.p,
+                                                           k + cEvRuns, c->evEndP.p, c->evEndBits.p, c->evApp.p);
+        HIPCK(vxhost::exclusiveScan(c->evApp.p, c->evApp.p, np, k + cEvInert, tempFor(c, vxhost::scanTempBytes(np)), s));
+        kEventEndsEmit<<<blocks(np), kThreads, 0, s>>>(c->evPrevKeys.p, np, k + cEvRuns, c->evEndP.p, c->evEndBits.p,
+                                                       c->evApp.p, c->evEmit.p, c->evAppKeys.p);
+        HIPCK(vxhost::exclusiveScan(c->evEmit.p, c->evEmit.p, np, k + cEvEnds, tempFor(c, vxhost::scanTempBytes(np)), s));
+        kEventEndsWrite<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, nb, c->evPrevKeys.p, np, k + cEvRuns, c->evEndP.p,
+                                                        c->evEndBits.p, c->evApp.p, c->evEmit.p, c->prev.p,
+                                                        c->prevCount, c->events.p);
+        kEventKeysMerge<<<blocks(m + np), kThreads, 0, s>>>(c->evCurKeys.p, k + cEvRuns, c->evAppKeys.p, k + cEvInert,
+                                                            (uint32_t)(m + np), c->evMerged.p);
     }
     HIPCK(hipGetLastError());
 }
@@ -776,10 +800,97 @@ void contactEvents(vxhip_ctx* c, uint32_t m, float dt) {
 void finishEvents(vxhip_ctx* c) {
     const uint32_t* h = c->hostCounters;
     const uint32_t runs = h[cEvRuns];
-    c->eventCount = h[cEvInert] ? h[cEvCount] : runs + (c->evPrevCount ? h[cEvEnds] : 0u);
-    std::swap(c->evPrevKeys.p, c->evCurKeys.p);
-    std::swap(c->evPrevKeys.cap, c->evCurKeys.cap);
-    c->evPrevCount = runs;
+    if (c->evPrevCount) {
+        c->eventCount = runs + h[cEvEnds];
+        std::swap(c->evPrevKeys.p, c->evMerged.p);
+        std::swap(c->evPrevKeys.cap, c->evMerged.cap);
+        c->evPrevCount = runs + h[cEvInert];
+    } else {
+        c->eventCount = runs;
+        std::swap(c->evPrevKeys.p, c->evCurKeys.p);
+        std::swap(c->evPrevKeys.cap, c->evCurKeys.cap);
+        c->evPrevCount = runs;
+    }
+}
+
+// World::updateSleeping (world.cpp:3788-3794, sleep.cpp:50-325) over the
+// step's contacts, after the events.
+void sleepUpdate(vxhip_ctx* c, uint32_t m, float dt) {
+    hipStream_t s = c->stream;
+    c->sleepParams.dt = dt;
+    uint32_t* k = c->counters.p;
+    const uint32_t n = c->n;
+    if (!n) return;
+    c->slParent.reserve(n); c->slFlags.reserve(n); c->slIslandMin.reserve(n); c->slRootState.reserve(n);
+    kSleepInit<<<blocks(n), kThreads, 0, s>>>(n, c->slParent.p, c->slFlags.p, c->slIslandMin.p, c->slRootState.p);
+    if (m)
+        kSleepContacts<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, n, c->solve.p, m, c->sleepParams, c->slParent.p,
+                                                      c->slFlags.p);
+    kSleepBodies<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->sleepParams, c->slParent.p, c->slFlags.p,
+                                                c->slStability.p, c->slIslandMin.p, k + cSlWoke);
+    kSleepTransitions<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->sleepParams, c->slParent.p, c->slIslandMin.p,
+                                                     c->slRootState.p, k + cSlDynamic);
+    kSleepIslands<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->slParent.p, c->slRootState.p, k + cSlIslands);
+    HIPCK(hipGetLastError());
+}
+
+// The warm-start cache for the next step (world.cpp:4492-4521): this step's
+// contacts plus the first cached contact of every pair kept by an inert
+// Persist event, ordered by pair key; in a step where a body fell asleep, in
+// exactly the order std::sort leaves them.
+void buildCache(vxhip_ctx* c, uint32_t m) {
+    hipStream_t s = c->stream;
+    uint32_t* k = c->counters.p;
+    const uint32_t mp = c->prevCount, total = m + mp;
+    const int nb = bodyBits(c->n);
+    c->prevStage.reserve(m ? m : 1);
+    if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prevStage.p);
+    c->carry.reserve(mp + 1); c->carryRank.reserve(mp + 1);
+    c->cacheKeys.reserve(total + 1); c->cacheKeysTmp.reserve(total + 1);
+    c->cacheVals.reserve(total + 1); c->cacheValsTmp.reserve(total + 1);
+    if (mp) {
+        kCarryFlags<<<blocks(mp), kThreads, 0, s>>>(c->prev.p, mp, nb, c->evAppKeys.p, k + cEvInert, c->carry.p);
+        HIPCK(vxhost::exclusiveScan(c->carry.p, c->carryRank.p, mp, k + cCarried, tempFor(c, vxhost::scanTempBytes(mp)), s));
+    } else {
+        kFill32<<<1, 64, 0, s>>>(k + cCarried, 1, 0u);
+    }
+    if (total) {
+        const uint64_t none = 1ull << (2 * nb);
+        kCacheKeys<<<blocks(total), kThreads, 0, s>>>(c->prevStage.p, m, c->prev.p, mp, nb, c->carry.p, none,
+                                                      c->cacheKeys.p, c->cacheVals.p);
+        HIPCK(vxhost::radixSort(c->cacheKeys.p, c->cacheVals.p, c->cacheKeysTmp.p, c->cacheValsTmp.p, total, 2 * nb + 1,
+                                tempFor(c, vxhost::radixTempBytes(total)), s));
+        kCacheStdSort<<<1, 1, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p, mp, nb, c->carry.p, k + cSlDynamic + 4,
+                                      c->cacheKeys.p, c->cacheVals.p);
+    }
+    c->prevNext.reserve(total ? total : 1);
+    if (total)
+        kCacheGather<<<blocks(total), kThreads, 0, s>>>(c->prevStage.p, m, c->prev.p, mp, c->cacheVals.p, k + cCarried,
+                                                        c->prevNext.p);
+    HIPCK(hipGetLastError());
+}
+
+void finishCache(vxhip_ctx* c, uint32_t m) {
+    std::swap(c->prev.p, c->prevNext.p);
+    std::swap(c->prev.cap, c->prevNext.cap);
+    c->prevCount = m + c->hostCounters[cCarried];
+}
+
+void finishSleep(vxhip_ctx* c) {
+    const uint32_t* h = c->hostCounters;
+    vx_sleep_stats& st = c->sleepStats;
+    st.totalDynamicBodies = h[cSlDynamic];
+    st.awakeBodies = h[cSlDynamic + 1];
+    st.drowsyBodies = h[cSlDynamic + 2];
+    st.sleepingBodies = h[cSlDynamic + 3];
+    st.totalSleepTransitions += h[cSlDynamic + 4];
+    st.totalDrowsyTransitions += h[cSlDynamic + 5];
+    st.totalWakeTransitions += h[cSlWoke];
+    st.islandCount = h[cSlIslands];
+    st.sleepingIslandCount = h[cSlIslands + 1];
+    st.drowsyIslandCount = h[cSlIslands + 2];
+    st.awakeIslandCount = h[cSlIslands + 3];
+    c->anyAsleep = st.sleepingBodies > 0;
 }
 
 // One full device step of the resident world.
@@ -802,7 +913,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sNarrow);
     const uint32_t m = narrowPhase(c, np, dt);
     markStage(c, sMatch);
-    if (c->anyAsleep && m) kWakePass<<<1, 64, 0, s>>>(c->bodies.p, c->contacts.p, m);
+    if (c->anyAsleep && m) kWakePass<<<1, kWakeThreads, 0, s>>>(c->bodies.p, c->contacts.p, m);
     if (m && c->prevCount) kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
     markStage(c, sColor);
     const uint32_t K = colorContacts(c, c->contacts.p, m);
@@ -823,19 +934,31 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sPosition);
     if (m) positionCorrection(c, c->solve.p, K, h, cfg.options);
     markStage(c, sFinish);
-    if (c->eventsEnabled) contactEvents(c, m, dt);
+    // With sleeping on, or a body asleep, the events' pair history also
+    // decides which inert pairs keep their cached contacts.
+    const bool carry = c->sleepOn || c->anyAsleep;
+    const bool events = c->eventsEnabled || carry;
+    if (events) contactEvents(c, m, dt);
+    if (c->sleepOn) sleepUpdate(c, m, dt);
     kClearForces<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n);
-    c->prev.reserve(m ? m : 1);
-    if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p);
+    if (carry) {
+        buildCache(c, m);
+    } else {
+        c->prev.reserve(m ? m : 1);
+        if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p);
+    }
     HIPCK(hipGetLastError());
-    c->prevCount = m;
     c->solveCount = m;
     c->numColors = K;
     c->lastIterations = (uint32_t)(2 * cfg.options.velocityIterations * S);
     markStage(c, sEnd);
     readCounters(c);
     if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: CCD dataflow did not converge");
-    if (c->eventsEnabled) finishEvents(c);
+    if (events) finishEvents(c);
+    if (!c->eventsEnabled) c->eventCount = 0;
+    if (carry) finishCache(c, m);
+    else c->prevCount = m;
+    if (c->sleepOn) finishSleep(c);
     if (c->hostCounters[cBatchError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: island batch exceeds its LDS capacity");
     collectProfile(c);
     c->stats.bodies = n;
@@ -1017,6 +1140,52 @@ int vxhip_world_set_events(vxhip_ctx* c, int enable) {
         c->eventsEnabled = enable != 0;
         c->eventCount = 0;
         c->evPrevCount = 0;
+    });
+}
+
+int vxhip_world_set_sleep(vxhip_ctx* c, const vx_sleep_config* config) {
+    return guarded([&] {
+        if (!c) fail(VXHIP_ERR_INVALID, "vxhip: null context");
+        c->sleepStats = vx_sleep_stats{};
+        if (!config) {
+            c->sleepOn = false;
+            return;
+        }
+        vx_sleep_config v = *config;  // SleepConfig::validate (sleep.h:143-159)
+        if (v.linearVelocityThreshold < 0.0f) v.linearVelocityThreshold = 0.0f;
+        if (v.angularVelocityThreshold < 0.0f) v.angularVelocityThreshold = 0.0f;
+        if (v.accelerationThreshold < 0.0f) v.accelerationThreshold = 0.0f;
+        if (v.timeToSleep < 0.0f) v.timeToSleep = 0.0f;
+        if (v.timeToDrowsy < 0.0f) v.timeToDrowsy = 0.0f;
+        if (v.timeToSleep > 0.0f && v.timeToDrowsy >= v.timeToSleep) v.timeToDrowsy = v.timeToSleep * 0.5f;
+        if (v.timeToSleep == 0.0f) v.timeToDrowsy = 0.0f;
+        if (v.drowsySimulationRate <= 0.0f) v.drowsySimulationRate = 0.01f;
+        if (v.drowsySimulationRate > 1.0f) v.drowsySimulationRate = 1.0f;
+        if (v.contactStabilityThreshold < 0.0f) v.contactStabilityThreshold = 0.0f;
+        if (v.contactStabilityFrames < 0) v.contactStabilityFrames = 0;
+        if (v.contactStabilityFrames > 65535) v.contactStabilityFrames = 65535;
+        c->sleepCfg = v;
+        vxk::SleepParams& p = c->sleepParams;
+        p.linTolSq = v.linearVelocityThreshold * v.linearVelocityThreshold;
+        p.angTolSq = v.angularVelocityThreshold * v.angularVelocityThreshold;
+        p.accTolSq = v.accelerationThreshold * v.accelerationThreshold;
+        p.stabilityThreshold = v.contactStabilityThreshold;
+        p.timeToSleep = v.timeToSleep;
+        p.timeToDrowsy = v.enableGradualSleep ? v.timeToDrowsy : v.timeToSleep;
+        p.stabilityFrames = (uint16_t)v.contactStabilityFrames;
+        p.gradual = v.enableGradualSleep ? 1u : 0u;
+        p.stability = v.enableContactStability ? 1u : 0u;
+        c->slStability.reserve(c->n ? c->n : 1);
+        HIPCK(hipMemsetAsync(c->slStability.p, 0, (size_t)(c->n ? c->n : 1) * sizeof(uint16_t), c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        c->sleepOn = true;
+    });
+}
+
+int vxhip_world_sleep_stats(vxhip_ctx* c, vx_sleep_stats* out) {
+    return guarded([&] {
+        if (!c || !out) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *out = c->sleepStats;
     });
 }
 

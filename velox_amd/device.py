@@ -38,6 +38,34 @@ class Stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class SleepConfig(C.Structure):
+    """vx_sleep_config (SleepConfig, sleep.h:98-160) with the reference defaults."""
+    _fields_ = [("linearVelocityThreshold", C.c_float), ("angularVelocityThreshold", C.c_float),
+                ("accelerationThreshold", C.c_float), ("timeToSleep", C.c_float), ("timeToDrowsy", C.c_float),
+                ("drowsySimulationRate", C.c_float), ("contactStabilityThreshold", C.c_float),
+                ("contactStabilityFrames", C.c_int32), ("enableGradualSleep", C.c_uint8),
+                ("enableContactStability", C.c_uint8), ("reserved", C.c_uint8 * 2)]
+
+    def __init__(self, **kw):
+        defaults = dict(linearVelocityThreshold=0.05, angularVelocityThreshold=0.05, accelerationThreshold=0.01,
+                        timeToSleep=0.5, timeToDrowsy=0.25, drowsySimulationRate=0.25,
+                        contactStabilityThreshold=0.02, contactStabilityFrames=10, enableGradualSleep=1,
+                        enableContactStability=1)
+        defaults.update(kw)
+        super().__init__(**defaults)
+
+
+class SleepStats(C.Structure):
+    _fields_ = [(name, C.c_uint32) for name in ("totalDynamicBodies", "awakeBodies", "drowsyBodies", "sleepingBodies",
+                                                "islandCount", "sleepingIslandCount", "drowsyIslandCount",
+                                                "awakeIslandCount")] + \
+               [(name, C.c_uint64) for name in ("totalSleepTransitions", "totalWakeTransitions",
+                                                "totalDrowsyTransitions")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 # Exported symbols and their signatures (include/vxhip.h).
 _vp, _u32, _f32, _i32 = C.c_void_p, C.c_uint32, C.c_float, C.c_int
 SIGNATURES = {
@@ -58,6 +86,8 @@ SIGNATURES = {
     "vxhip_world_set_island_limit": (_i32, [_vp, _u32]),
     "vxhip_world_download_events": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_set_events": (_i32, [_vp, _i32]),
+    "vxhip_world_set_sleep": (_i32, [_vp, _vp]),
+    "vxhip_world_sleep_stats": (_i32, [_vp, _vp]),
     "vxhip_world_sync": (_i32, [_vp]),
     "vxhip_world_download_bodies": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_upload_bodies": (_i32, [_vp, _vp, _u32]),
@@ -282,6 +312,21 @@ class Context:
     def set_events(self, enable):
         check(library().vxhip_world_set_events(self._h, 1 if enable else 0))
         return self
+
+    def set_sleep(self, config=None, **kw):
+        """Device sleep update on (a SleepConfig, or keyword overrides of the
+        defaults) or off (no arguments and enable=False via set_sleep(False))."""
+        if config is False:
+            check(library().vxhip_world_set_sleep(self._h, None))
+            return self
+        cfg = config if isinstance(config, SleepConfig) else SleepConfig(**kw)
+        check(library().vxhip_world_set_sleep(self._h, C.byref(cfg)))
+        return self
+
+    def sleep_stats(self):
+        st = SleepStats()
+        check(library().vxhip_world_sleep_stats(self._h, C.byref(st)))
+        return st.as_dict()
 
     def stats(self):
         s = Stats()
