@@ -920,13 +920,10 @@ __global__ __launch_bounds__(kThreads) void kCacheGather(const vx_contact* __res
 }
 
 // world.cpp:4364-4397, one colour at a time.
-__global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,
-                                                           const uint32_t* __restrict__ colorStart, uint32_t color, float h,
-                                                           vx_solver_options opt) {
-    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
-    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= count) return;
-    const vx_contact c = cs[first + k];
+__device__ __attribute__((always_inline)) inline void positionRow(vx_body* __restrict__ bodies,
+                                                                 const vx_contact* __restrict__ cs, uint32_t k,
+                                                                 float h, const vx_solver_options& opt) {
+    const vx_contact c = cs[k];
     vx_body a = bodies[c.a], b = bodies[c.b];
     if (isSensor(a) || isSensor(b)) return;
     float ims = solverInvMass(a) + solverInvMass(b);
@@ -945,6 +942,30 @@ __global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__
     float push = -resolve * (g + 1e-3f) / ims;
     if (isDynamic(a) && !asleepFully(a)) bodies[c.a].position = raw(v3(a.position) + v3(c.normal) * (push * solverInvMass(a)));
     if (isDynamic(b) && !asleepFully(b)) bodies[c.b].position = raw(v3(b.position) - v3(c.normal) * (push * solverInvMass(b)));
+}
+
+__global__ __launch_bounds__(kThreads) void kPositionColor(vx_body* __restrict__ bodies, const vx_contact* __restrict__ cs,
+                                                           const uint32_t* __restrict__ colorStart, uint32_t color, float h,
+                                                           vx_solver_options opt) {
+    uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    positionRow(bodies, cs, first + k, h, opt);
+}
+
+// A run of narrow colours [c0, c1) in one workgroup, one colour after the
+// other: the bodies a colour writes are read by later colours only after the
+// workgroup barrier, on the same compute unit.
+constexpr int kRunThreads = 1024;
+__global__ __launch_bounds__(kRunThreads) void kPositionColorRun(vx_body* __restrict__ bodies,
+                                                                 const vx_contact* __restrict__ cs,
+                                                                 const uint32_t* __restrict__ colorStart, uint32_t c0,
+                                                                 uint32_t c1, float h, vx_solver_options opt) {
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t first = colorStart[c], last = colorStart[c + 1];
+        for (uint32_t k = first + threadIdx.x; k < last; k += kRunThreads) positionRow(bodies, cs, k, h, opt);
+        __syncthreads();
+    }
 }
 
 }  // namespace vxk

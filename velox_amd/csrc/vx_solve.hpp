@@ -333,6 +333,76 @@ __global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, uint3
     }
 }
 
+// A run of narrow big-island colours [c0, c1) in one workgroup, one colour
+// after the other (a launch per colour costs more than a few rounds of 1024
+// rows); the barrier orders the colours on one compute unit.  512 threads
+// leave each wave the registers to hold a prefetched row without spilling.  Everything a
+// row reads except the two bodies' velocities (its rows, body frames and
+// inertia, its own accumulated impulses) is loaded during the previous
+// colour, so only the velocity loads sit between two barriers.
+constexpr int kSweepRunThreads = 512;
+
+struct RowPrefetch {
+    RowRegs R;
+    float4 fa, ia, fb, ib, i0, i1;
+    uint32_t a, b;
+};
+
+__device__ __attribute__((always_inline)) inline void prefetchRow(SolveArrays sa, uint32_t k, RowPrefetch& P) {
+    const uint4 meta = sa.meta[k];
+    P.a = meta.x;
+    P.b = meta.y;
+    P.R.flags = meta.z;
+    for (int q = 0; q < 8; ++q) P.R.r[q] = sa.rows[8 * (size_t)k + q];
+    P.fa = sa.frame[P.a]; P.ia = sa.invI[P.a];
+    P.fb = sa.frame[P.b]; P.ib = sa.invI[P.b];
+    P.i0 = sa.imp[2 * k]; P.i1 = sa.imp[2 * k + 1];
+}
+
+template <bool kCone>
+__device__ __attribute__((always_inline)) inline void finishRow(SolveArrays sa, uint32_t k, const RowPrefetch& P) {
+    if (P.R.flags & kSkip) return;
+    const uint32_t ia = P.a, ib = P.b;
+    V3 vA = xyz(sa.vel[ia]), wA = xyz(sa.ang[ia]), vB = xyz(sa.vel[ib]), wB = xyz(sa.ang[ib]);
+    float cN = P.i0.x, cT1 = P.i0.y, cT2 = P.i0.z, cR1 = P.i0.w, cR2 = P.i1.x, cS = P.i1.y;
+    solveRegs<kCone>(P.R, P.fa, P.ia, P.fb, P.ib, vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
+    if (P.R.flags & kDynA) { sa.vel[ia] = make_float4(vA.x, vA.y, vA.z, 0.0f); sa.ang[ia] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
+    if (P.R.flags & kDynB) { sa.vel[ib] = make_float4(vB.x, vB.y, vB.z, 0.0f); sa.ang[ib] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
+    sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
+    sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
+}
+
+template <int kMode>
+__global__ __launch_bounds__(kSweepRunThreads) void kSweepColorRun(SolveArrays sa, uint32_t base,
+                                                              const uint32_t* __restrict__ colorStart, uint32_t c0,
+                                                              uint32_t c1) {
+    if (kMode == 2) {
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t first = colorStart[c], last = colorStart[c + 1];
+            for (uint32_t j = first + threadIdx.x; j < last; j += kSweepRunThreads) warmRow(sa, base + j);
+            __syncthreads();
+        }
+        return;
+    }
+    RowPrefetch P;
+    uint32_t k = kNone;
+    {
+        const uint32_t j = colorStart[c0] + threadIdx.x;
+        if (j < colorStart[c0 + 1]) { k = base + j; prefetchRow(sa, k, P); }
+    }
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t first = colorStart[c], last = colorStart[c + 1];
+        if (k != kNone) finishRow<kMode == 1>(sa, k, P);
+        for (uint32_t j = first + threadIdx.x + kSweepRunThreads; j < last; j += kSweepRunThreads) solveRow<kMode == 1>(sa, base + j);
+        k = kNone;
+        if (c + 1 < c1) {
+            const uint32_t j = colorStart[c + 1] + threadIdx.x;
+            if (j < colorStart[c + 2]) { k = base + j; prefetchRow(sa, k, P); }
+        }
+        __syncthreads();
+    }
+}
+
 // --------------------------------------------------------------- islands --
 // Connected components of the contact graph over dynamic bodies
 // (the reference's buildIslands relation, solver.cpp:507-581).  Lock-free

@@ -101,6 +101,10 @@ inline uint32_t blocks(uint64_t n, int t = kThreads) { return (uint32_t)((n + t 
 
 struct vxhip_ctx {
     hipStream_t stream = nullptr;
+    // Second stream: the big-island colour sweeps run beside the island-batch
+    // kernels (disjoint bodies and contacts); forked and joined with events.
+    hipStream_t side = nullptr;
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
     // resident scene
     uint32_t n = 0;
     Buf<vx_body> bodies;
@@ -347,6 +351,9 @@ int bodyBits(uint32_t n) { return vxhost::bitsFor(n ? n - 1 : 0) ? vxhost::bitsF
 // --------------------------------------------------------------- stages --
 // Broad phase: fills c->pairs (sorted packed contact keys), returns count.
 constexpr uint64_t kCellTableMax = 1ull << 22;
+// Colours with at most this many contacts are solved in runs by one
+// workgroup instead of a launch each.
+constexpr uint32_t kNarrowColor = 2048, kNarrowSweepColor = 1024;
 
 uint32_t broadPhase(vxhip_ctx* c, float dt) {
     const uint32_t n = c->n;
@@ -627,6 +634,15 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
 template <int kMode>
 void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
     hipStream_t s = c->stream;
+    // Big islands and island batches hold disjoint bodies and contacts, so
+    // their sweeps may run concurrently (not while profiling: the per-launch
+    // events time one kernel at a time).
+    hipStream_t bs = s;
+    if (c->bigCount && c->batchCount && !c->profile) {
+        bs = c->side;
+        HIPCK(hipEventRecord(c->forkEv, s));
+        HIPCK(hipStreamWaitEvent(bs, c->forkEv, 0));
+    }
     if (c->batchCount) {
         const uint32_t lds = c->ldsBatchCount, global = c->batchCount - c->ldsBatchCount;
         if (lds) {
@@ -645,15 +661,31 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
             solveLaunchEnd(c, VXHIP_KERNEL_ISLAND, (uint64_t)(c->smallCount - c->ldsContactCount) * sweeps);
         }
     }
-    if (c->bigCount)
+    if (c->bigCount) {
+        const std::vector<uint32_t>& cs = c->bigColorStartHost;
         for (int it = 0; it < sweeps; ++it)
-            for (uint32_t col = 0; col < K; ++col) {
-                uint32_t cnt = c->bigColorStartHost[col + 1] - c->bigColorStartHost[col];
-                if (!cnt) continue;
+            for (uint32_t col = 0; col < K;) {
+                const uint32_t cnt = cs[col + 1] - cs[col];
+                if (!cnt) { ++col; continue; }
+                if (cnt > kNarrowSweepColor) {
+                    solveLaunchBegin(c);
+                    kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col);
+                    solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
+                    ++col;
+                    continue;
+                }
+                uint32_t end = col + 1;
+                while (end < K && cs[end + 1] - cs[end] <= kNarrowSweepColor) ++end;
                 solveLaunchBegin(c);
-                kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, s>>>(sa, c->smallCount, c->bigColorStart.p, col);
-                solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
+                kSweepColorRun<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, end);
+                solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[end] - cs[col]);
+                col = end;
             }
+        if (bs != s) {
+            HIPCK(hipEventRecord(c->joinEv, bs));
+            HIPCK(hipStreamWaitEvent(s, c->joinEv, 0));
+        }
+    }
     HIPCK(hipGetLastError());
 }
 
@@ -688,10 +720,20 @@ void substepSolve(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx
 }
 
 void positionCorrection(vxhip_ctx* c, const vx_contact* solve, uint32_t K, float h, const vx_solver_options& opt) {
+    const std::vector<uint32_t>& cs = c->colorStartHost;
     for (int it = 0; it < opt.positionIterations; ++it)
-        for (uint32_t col = 0; col < K; ++col) {
-            uint32_t cnt = c->colorStartHost[col + 1] - c->colorStartHost[col];
-            if (cnt) kPositionColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+        for (uint32_t col = 0; col < K;) {
+            const uint32_t cnt = cs[col + 1] - cs[col];
+            if (!cnt) { ++col; continue; }
+            if (cnt > kNarrowColor) {
+                kPositionColor<<<blocks(cnt), kThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, h, opt);
+                ++col;
+                continue;
+            }
+            uint32_t end = col + 1;
+            while (end < K && cs[end + 1] - cs[end] <= kNarrowColor) ++end;
+            kPositionColorRun<<<1, kRunThreads, 0, c->stream>>>(c->bodies.p, solve, c->colorStart.p, col, end, h, opt);
+            col = end;
         }
     HIPCK(hipGetLastError());
 }
@@ -991,6 +1033,9 @@ int guarded(F&& f) {
 vxhip_ctx* newContext() {
     vxhip_ctx* c = new vxhip_ctx();
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCK(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+    HIPCK(hipEventCreateWithFlags(&c->forkEv, hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&c->joinEv, hipEventDisableTiming));
     c->counters.reserve(cCount);
     HIPCK(hipHostMalloc((void**)&c->hostCounters, cCount * sizeof(uint32_t), hipHostMallocDefault));
     return c;
@@ -1036,6 +1081,12 @@ void vxhip_destroy(vxhip_ctx* c) {
     for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
+    if (c->side) {
+        (void)hipStreamSynchronize(c->side);
+        (void)hipStreamDestroy(c->side);
+    }
+    if (c->forkEv) (void)hipEventDestroy(c->forkEv);
+    if (c->joinEv) (void)hipEventDestroy(c->joinEv);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
