@@ -333,6 +333,15 @@ __global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, uint3
     }
 }
 
+// Workgroup barrier that orders LDS only: global loads in flight (the next
+// rows' prefetch) and global stores (a row's own impulses, read back only by
+// the same thread) are not waited for.
+__device__ __attribute__((always_inline)) inline void ldsBarrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // A run of narrow big-island colours [c0, c1) in one workgroup, one colour
 // after the other (a launch per colour costs more than a few rounds of 1024
 // rows); the barrier orders the colours on one compute unit.  512 threads
@@ -400,6 +409,125 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepColorRun(SolveArrays s
             if (j < colorStart[c + 2]) { k = base + j; prefetchRow(sa, k, P); }
         }
         __syncthreads();
+    }
+}
+
+// The tail of the big-island colours (the last colours, every one narrow)
+// touches few bodies; their velocities then live in LDS for the whole run:
+// loaded once, updated in place colour after colour, written back at the end.
+// Rows index their bodies' LDS slots through rowSlot (2 per row, kNone for a
+// body the row does not write).  More slots than fit: the global-memory run.
+constexpr uint32_t kTailCap = 4096;
+
+// (body, 2 * row + side) entries of the tail rows' written bodies; the other
+// sides sort last under `sentinel`.
+__global__ __launch_bounds__(kThreads) void kTailEntries(SolveArrays sa, uint32_t r0, uint32_t rows, uint64_t sentinel,
+                                                         uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= rows) return;
+    const uint4 m = sa.meta[r0 + i];
+    keys[2 * i] = (m.z & kDynA) ? (uint64_t)m.x : sentinel;
+    keys[2 * i + 1] = (m.z & kDynB) ? (uint64_t)m.y : sentinel;
+    vals[2 * i] = 2 * i;
+    vals[2 * i + 1] = 2 * i + 1;
+}
+
+__global__ __launch_bounds__(kThreads) void kTailHeads(const uint64_t* __restrict__ keys, uint32_t n, uint64_t sentinel,
+                                                       uint32_t* __restrict__ head) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    head[i] = keys[i] != sentinel && (i == 0 || keys[i - 1] != keys[i]) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void kTailSlots(const uint64_t* __restrict__ keys,
+                                                       const uint32_t* __restrict__ vals, uint32_t n, uint64_t sentinel,
+                                                       const uint32_t* __restrict__ head,
+                                                       const uint32_t* __restrict__ headPos,
+                                                       uint32_t* __restrict__ rowSlot, uint32_t* __restrict__ slotBody) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    if (keys[i] == sentinel) { rowSlot[vals[i]] = kNone; return; }
+    const uint32_t slot = headPos[i] + head[i] - 1;
+    rowSlot[vals[i]] = slot;
+    if (head[i]) slotBody[slot] = (uint32_t)keys[i];
+}
+
+template <bool kCone>
+__device__ __attribute__((always_inline)) inline void finishRowLds(SolveArrays sa, uint32_t k, const RowPrefetch& P,
+                                                                  uint32_t sA, uint32_t sB, float4* lv, float4* la) {
+    if (P.R.flags & kSkip) return;
+    const uint32_t ia = P.a, ib = P.b;
+    const float4 va = sA != kNone ? lv[sA] : sa.vel[ia], wa = sA != kNone ? la[sA] : sa.ang[ia];
+    const float4 vb = sB != kNone ? lv[sB] : sa.vel[ib], wb = sB != kNone ? la[sB] : sa.ang[ib];
+    V3 vA = xyz(va), wA = xyz(wa), vB = xyz(vb), wB = xyz(wb);
+    float cN = P.i0.x, cT1 = P.i0.y, cT2 = P.i0.z, cR1 = P.i0.w, cR2 = P.i1.x, cS = P.i1.y;
+    solveRegs<kCone>(P.R, P.fa, P.ia, P.fb, P.ib, vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
+    if (P.R.flags & kDynA) { lv[sA] = make_float4(vA.x, vA.y, vA.z, 0.0f); la[sA] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
+    if (P.R.flags & kDynB) { lv[sB] = make_float4(vB.x, vB.y, vB.z, 0.0f); la[sB] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
+    sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
+    sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
+}
+
+template <int kMode>  // 0 two-axis, 1 cone-block
+__global__ __launch_bounds__(kSweepRunThreads) void kSweepTailLds(SolveArrays sa, uint32_t base,
+                                                                  const uint32_t* __restrict__ colorStart, uint32_t c0,
+                                                                  uint32_t c1, const uint32_t* __restrict__ rowSlot,
+                                                                  const uint32_t* __restrict__ slotBody,
+                                                                  const uint32_t* __restrict__ slotCount) {
+    __shared__ float4 lv[kTailCap], la[kTailCap];
+    const uint32_t ns = *slotCount;
+    const uint32_t r0 = base + colorStart[c0];
+    if (ns > kTailCap) {  // uniform: the global-memory run
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t first = colorStart[c], last = colorStart[c + 1];
+            for (uint32_t j = first + threadIdx.x; j < last; j += kSweepRunThreads) solveRow<kMode == 1>(sa, base + j);
+            __syncthreads();
+        }
+        return;
+    }
+    for (uint32_t q = threadIdx.x; q < ns; q += kSweepRunThreads) {
+        const uint32_t b = slotBody[q];
+        lv[q] = sa.vel[b];
+        la[q] = sa.ang[b];
+    }
+    RowPrefetch P;
+    uint32_t k = kNone, sA = kNone, sB = kNone;
+    {
+        const uint32_t j = colorStart[c0] + threadIdx.x;
+        if (j < colorStart[c0 + 1]) {
+            k = base + j;
+            prefetchRow(sa, k, P);
+            sA = rowSlot[2 * (k - r0)]; sB = rowSlot[2 * (k - r0) + 1];
+        }
+    }
+    __syncthreads();
+    for (uint32_t c = c0; c < c1; ++c) {
+        const uint32_t first = colorStart[c], last = colorStart[c + 1];
+        // The next colour's row is loaded while this one is solved.
+        RowPrefetch Pn;
+        uint32_t kn = kNone, sAn = kNone, sBn = kNone;
+        if (c + 1 < c1) {
+            const uint32_t j = colorStart[c + 1] + threadIdx.x;
+            if (j < colorStart[c + 2]) {
+                kn = base + j;
+                prefetchRow(sa, kn, Pn);
+                sAn = rowSlot[2 * (kn - r0)]; sBn = rowSlot[2 * (kn - r0) + 1];
+            }
+        }
+        if (k != kNone) finishRowLds<kMode == 1>(sa, k, P, sA, sB, lv, la);
+        for (uint32_t j = first + threadIdx.x + kSweepRunThreads; j < last; j += kSweepRunThreads) {
+            RowPrefetch Q;
+            const uint32_t kk = base + j;
+            prefetchRow(sa, kk, Q);
+            finishRowLds<kMode == 1>(sa, kk, Q, rowSlot[2 * (kk - r0)], rowSlot[2 * (kk - r0) + 1], lv, la);
+        }
+        ldsBarrier();
+        P = Pn; k = kn; sA = sAn; sB = sBn;
+    }
+    for (uint32_t q = threadIdx.x; q < ns; q += kSweepRunThreads) {
+        const uint32_t b = slotBody[q];
+        sa.vel[b] = lv[q];
+        sa.ang[b] = la[q];
     }
 }
 
@@ -920,7 +1048,7 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
             PairItem nxt;
             if (more && nj < nend) nxt = loadPairItem(sa, slotPair, nj, sideB);
             if (j < end) pairRow<kMode>(sa, j, cur, sideB, lv);
-            if (runDone) __syncthreads();
+            if (runDone) ldsBarrier();
             if (!more) break;
             if (nj == j && kMode != 2) { nxt.i0 = cur.i0; nxt.i1 = cur.i1; }
             sw = nsw; r = nr; base = nbase; end = nend; j = nj;

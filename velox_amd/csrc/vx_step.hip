@@ -170,6 +170,10 @@ struct vxhip_ctx {
              largestIsland = 0,
              maxBatchPasses = 0;
     uint32_t islandLimit = 4096;
+    // LDS tail run of the big-island colours (colours tailColor..K-1).
+    uint32_t tailColor = 0;
+    Buf<uint64_t> tailKeys, tailKeysTmp;
+    Buf<uint32_t> tailVals, tailValsTmp, tailHead, tailPos, tailRowSlot, tailSlotBody;
     Buf<uint32_t> ccdDone;
     Buf<float4> ccdGap;
     Buf<uint8_t> temp;
@@ -202,7 +206,7 @@ enum Counter : uint32_t {
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
-    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cCount = 52
+    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cCount = 53
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -629,6 +633,33 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
     }
 }
 
+// The LDS tail of the big-island colours: the last colours that each hold at
+// most kNarrowSweepColor contacts, their written bodies given LDS slots.
+void prepareTail(vxhip_ctx* c, uint32_t K, SolveArrays sa) {
+    c->tailColor = K;
+    if (!c->bigCount) return;
+    const std::vector<uint32_t>& cs = c->bigColorStartHost;
+    uint32_t t = K;
+    while (t > 0 && cs[t] - cs[t - 1] <= kNarrowSweepColor) --t;
+    if (t >= K || cs[K] == cs[t]) return;
+    hipStream_t s = c->stream;
+    const uint32_t rows = cs[K] - cs[t], n2 = 2 * rows, r0 = c->smallCount + cs[t];
+    const int nb = bodyBits(c->n);
+    const uint64_t sentinel = 1ull << nb;
+    c->tailKeys.reserve(n2); c->tailKeysTmp.reserve(n2); c->tailVals.reserve(n2); c->tailValsTmp.reserve(n2);
+    c->tailHead.reserve(n2); c->tailPos.reserve(n2); c->tailRowSlot.reserve(n2); c->tailSlotBody.reserve(n2);
+    kTailEntries<<<blocks(rows), kThreads, 0, s>>>(sa, r0, rows, sentinel, c->tailKeys.p, c->tailVals.p);
+    HIPCK(vxhost::radixSort(c->tailKeys.p, c->tailVals.p, c->tailKeysTmp.p, c->tailValsTmp.p, n2, nb + 1,
+                            tempFor(c, vxhost::radixTempBytes(n2)), s));
+    kTailHeads<<<blocks(n2), kThreads, 0, s>>>(c->tailKeys.p, n2, sentinel, c->tailHead.p);
+    HIPCK(vxhost::exclusiveScan(c->tailHead.p, c->tailPos.p, n2, c->counters.p + cTailSlots,
+                                tempFor(c, vxhost::scanTempBytes(n2)), s));
+    kTailSlots<<<blocks(n2), kThreads, 0, s>>>(c->tailKeys.p, c->tailVals.p, n2, sentinel, c->tailHead.p, c->tailPos.p,
+                                               c->tailRowSlot.p, c->tailSlotBody.p);
+    HIPCK(hipGetLastError());
+    c->tailColor = t;
+}
+
 // One pass (warm start, or 2*I sweeps) over every contact: small islands in
 // one launch, big islands colour by colour.
 template <int kMode>
@@ -667,6 +698,14 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
             for (uint32_t col = 0; col < K;) {
                 const uint32_t cnt = cs[col + 1] - cs[col];
                 if (!cnt) { ++col; continue; }
+                if (kMode != 2 && col == c->tailColor) {
+                    solveLaunchBegin(c);
+                    kSweepTailLds<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, K,
+                                                                        c->tailRowSlot.p, c->tailSlotBody.p,
+                                                                        c->counters.p + cTailSlots);
+                    solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[K] - cs[col]);
+                    break;
+                }
                 if (cnt > kNarrowSweepColor) {
                     solveLaunchBegin(c);
                     kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col);
@@ -710,6 +749,7 @@ void substepSolve(vxhip_ctx* c, vx_contact* solve, uint32_t K, float h, const vx
         kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(solve, m, c->sImp.p, c->solvePos.p);
     }
     kPrepareRows<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, solve, m, h, sa);
+    if (first) prepareTail(c, K, sa);
     if (first) solvePass<2>(c, sa, K, 1);
     const int sweeps = 2 * opt.velocityIterations;
     if (opt.frictionModel == VX_FRICTION_CONE_BLOCK) solvePass<1>(c, sa, K, sweeps);
