@@ -212,7 +212,10 @@ typedef struct vx_sleep_stats {
  * carried over; the wake pass (world.cpp:4109-4128) runs whenever a body is
  * asleep.  Contact pair history for events is kept either way.  Resets the
  * per-body contact stability counters and the cumulative transition counts.
- * Replaces World::setSleepConfig (world.h:741) for the device step. */
+ * Replaces World::setSleepConfig (world.h:741) for the device step.
+ * Limited to worlds of at most 256 bodies in this build (VXHIP_ERR_CAPACITY
+ * above, here and in vxhip_world_load while sleeping is on): a 1,001-body
+ * world with sleeping on faulted the device and the cause is not yet found. */
 int vxhip_world_set_sleep(vxhip_ctx* ctx, const vx_sleep_config* config);
 
 /* Sleep statistics after the last step (World::sleepStats, world.h:744). */

@@ -566,12 +566,19 @@ int fixture(const std::string& out, const std::string& name) {
         trajectory(out, name, vxscene::lattice(), {1, 100, 200, 300});
         return 0;
     }
+    if (name == "lattice1000_sleep") {  // the 1,000-box lattice with sleeping on (sleep and wake from step ~175)
+        vxscene::Scene s = vxscene::lattice();
+        for (vxscene::BodySpec& b : s.bodies) b.enableSleep = 1;
+        trajectory(out, name, s, {150, 200, 250, 300}, false, true);
+        return 0;
+    }
     if (name == "settle") {  // sleeping on: towers fall asleep, a late strike wakes one
         trajectory(out, name, vxscene::settle(), {1, 20, 31, 40, 41, 60, 120, 236, 237, 238, 239, 240, 260, 300}, false, true);
         return 0;
     }
-    if (name == "sleepcount") {  // per-step sleep counts (tuning aid)
-        vxscene::Scene s = vxscene::settle();
+    if (name == "sleepcount" || name == "sleepcount_lattice") {  // per-step sleep counts (tuning aid)
+        vxscene::Scene s = name == "sleepcount" ? vxscene::settle() : vxscene::lattice();
+        for (vxscene::BodySpec& b : s.bodies) if (name != "sleepcount") b.enableSleep = 1;
         World w(BackendType::Cpu);
         build(w, s, true);
         w.backend_.reset(new CudaSemantics());

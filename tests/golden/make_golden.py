@@ -29,6 +29,9 @@ def main():
                           cwd=ROOT)
     # Sleeping on: towers fall asleep island by island, a late strike wakes one.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "settle"], cwd=ROOT)
+    # The 1,000-box lattice with sleeping on over 300 steps (several minutes).
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "lattice1000_sleep"],
+                          cwd=ROOT)
 
 
 if __name__ == "__main__":

@@ -40,3 +40,15 @@ def test_record_layouts_match_header():
     assert BODY.itemsize == 320 and CONTACT.itemsize == 128
     assert BODY.fields["shape"][1] == 232 and BODY.fields["compoundCount"][1] == 296
     assert CONTACT.fields["spinningImpulse"][1] == 116
+
+
+def test_sleep_records_match_header_and_reference_defaults():
+    """vx_sleep_config / vx_sleep_stats byte layouts, and the binding's defaults
+    equal SleepConfig's (sleep.h:98-160)."""
+    assert C.sizeof(device.SleepConfig) == 36 and C.sizeof(device.SleepStats) == 56
+    assert device.SleepStats.totalSleepTransitions.offset == 32
+    cfg = device.SleepConfig()
+    got = [cfg.linearVelocityThreshold, cfg.angularVelocityThreshold, cfg.accelerationThreshold, cfg.timeToSleep,
+           cfg.timeToDrowsy, cfg.drowsySimulationRate, cfg.contactStabilityThreshold]
+    assert got == [C.c_float(x).value for x in (0.05, 0.05, 0.01, 0.5, 0.25, 0.25, 0.02)]
+    assert (cfg.contactStabilityFrames, cfg.enableGradualSleep, cfg.enableContactStability) == (10, 1, 1)

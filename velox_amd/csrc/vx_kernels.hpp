@@ -888,24 +888,43 @@ __global__ __launch_bounds__(kThreads) void kCacheKeys(const vx_contact* __restr
 // In a step where a body fell asleep, some pair may turn inert with several
 // contacts cached; the order std::sort leaves them in then decides what the
 // next steps report and keep.  The cache candidates are put in the order the
-// reference hands to std::sort (this step's contacts in solve order, then
-// the carried ones) and sorted by the libstdc++ restatement.  One thread.
-__global__ void kCacheStdSort(const vx_contact* __restrict__ solve, const uint32_t* __restrict__ order, uint32_t m,
-                              const vx_contact* __restrict__ prev, uint32_t mp, int nb,
-                              const uint32_t* __restrict__ carry, const uint32_t* __restrict__ fellAsleep,
-                              uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    if (threadIdx.x != 0 || blockIdx.x != 0 || *fellAsleep == 0) return;
-    uint32_t n = 0;
-    for (uint32_t j = 0; j < m; ++j, ++n) {
-        keys[n] = ((uint64_t)solve[j].a << nb) | solve[j].b;
-        vals[n] = order[j];
+// reference hands to std::sort (this step's contacts in solve order, then the
+// carried ones in cache order) and sorted by the libstdc++ restatement: one
+// thread, in LDS when they fit.
+constexpr uint32_t kStdSortLds = 10240;  // 120 KB of (key, value) pairs
+__global__ __launch_bounds__(kThreads) void kCacheStdSort(const vx_contact* __restrict__ solve,
+                                                          const uint32_t* __restrict__ order, uint32_t m,
+                                                          const vx_contact* __restrict__ prev, uint32_t mp, int nb,
+                                                          const uint32_t* __restrict__ carry,
+                                                          const uint32_t* __restrict__ carryRank,
+                                                          const uint32_t* __restrict__ carried,
+                                                          const uint32_t* __restrict__ fellAsleep,
+                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    __shared__ uint64_t lk[kStdSortLds];
+    __shared__ uint32_t lw[kStdSortLds];
+    if (*fellAsleep == 0) return;
+    const uint32_t n = m + *carried;
+    const bool lds = n <= kStdSortLds;
+    uint64_t* K = lds ? lk : keys;
+    uint32_t* V = lds ? lw : vals;
+    for (uint32_t j = threadIdx.x; j < m; j += kThreads) {
+        K[j] = ((uint64_t)solve[j].a << nb) | solve[j].b;
+        V[j] = order[j];
     }
-    for (uint32_t t = 0; t < mp; ++t)
+    for (uint32_t t = threadIdx.x; t < mp; t += kThreads)
         if (carry[t]) {
-            keys[n] = ((uint64_t)prev[t].a << nb) | prev[t].b;
-            vals[n++] = m + t;
+            const uint32_t r = m + carryRank[t];
+            K[r] = ((uint64_t)prev[t].a << nb) | prev[t].b;
+            V[r] = m + t;
         }
-    vxsort::sort({keys, vals}, n);
+    __syncthreads();
+    if (threadIdx.x == 0) vxsort::sort({K, V}, n);
+    __syncthreads();
+    if (lds)
+        for (uint32_t j = threadIdx.x; j < n; j += kThreads) {
+            keys[j] = K[j];
+            vals[j] = V[j];
+        }
 }
 
 __global__ __launch_bounds__(kThreads) void kCacheGather(const vx_contact* __restrict__ cur, uint32_t m,
