@@ -181,6 +181,24 @@ int vxhip_world_set_events(vxhip_ctx* ctx, int enable);
 /* Counters of the last step (StepStats subset, world.h:175-208). */
 int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
 
+/* Moves the world origin: every stored world coordinate of the resident world
+ * (body positions, plane offsets, mesh vertices, mesh and BVH bounds, the last
+ * step's contacts, the warm-start cache, the last events' points) becomes
+ * value - offset.  Replaces World::shiftOrigin (world.cpp:1978-2036): a
+ * non-finite offset or result fails with VXHIP_ERR_INVALID and changes
+ * nothing. */
+int vxhip_world_shift_origin(vxhip_ctx* ctx, const float offset[3]);
+
+/* The device state the next vxhip_world_step reads, as one host blob: bodies,
+ * warm-start cache, the events' pair history, the last events and contacts.
+ * The device form of World::saveSnapshot / restoreSnapshot
+ * (world.cpp:2038-2127); as there, sleep-manager counters are not part of
+ * it.  Restoring needs a world of the same body count (VXHIP_ERR_INVALID
+ * otherwise); a step after a restore continues bit for bit as after the save. */
+int vxhip_world_state_size(vxhip_ctx* ctx, size_t* bytes);
+int vxhip_world_save_state(vxhip_ctx* ctx, void* out, size_t cap);
+int vxhip_world_restore_state(vxhip_ctx* ctx, const void* in, size_t size);
+
 /* Island sleeping (SleepConfig, sleep.h:98-160; field meaning and defaults as
  * there: 0.05, 0.05, 0.01, 0.5, 0.25, 0.25, 1, 1, 0.02, 10). */
 typedef struct vx_sleep_config {

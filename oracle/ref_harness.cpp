@@ -366,6 +366,36 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
     writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]" + (sleep ? sleepJson + "]" : std::string()));
 }
 
+// Origin shift mid-trajectory (World::shiftOrigin, world.cpp:1978-2036): the
+// reference World steps to `at`, shifts by `offset`, and steps on to `end`.
+// Writes the states and events at `at` (after the shift) and at `end`, and
+// the shifted soup vertices.
+void shiftTrajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene, int at,
+                     Vec3 offset, int end) {
+    std::string dir = out + "/" + name;
+    mkdir(dir.c_str(), 0755);
+    World w(BackendType::Cpu);
+    build(w, scene);
+    w.backend_.reset(new CudaSemantics());
+    writeRecords(dir + "/bodies_in.npy", w.bodies_);
+    writeSoup(dir, w.meshes_);
+    for (int k = 0; k < at; ++k) w.step(scene.dt);
+    w.shiftOrigin(offset);
+    writeRecords(dir + "/bodies_shifted.npy", w.bodies_);
+    writeRecords(dir + "/events_shifted.npy", eventRecords(w));
+    writeRecords(dir + "/prev_shifted.npy", w.prevContacts_);
+    writeNpy(dir + "/vertices_shifted.npy", "<f4", {w.meshes_.vertices.size(), 3}, w.meshes_.vertices.data(),
+             w.meshes_.vertices.size() * 12);
+    for (int k = at; k < end; ++k) w.step(scene.dt);
+    writeRecords(dir + "/state_" + std::to_string(end) + ".npy", states(w.bodies_));
+    writeRecords(dir + "/events_" + std::to_string(end) + ".npy", eventRecords(w));
+    char extra[160];
+    std::snprintf(extra, sizeof extra, ", \"shift_at\": %d, \"offset\": [%.9g, %.9g, %.9g]", at, offset.x, offset.y,
+                  offset.z);
+    writeMeta(dir, w, scene.dt, {end}, extra);
+    std::fprintf(stderr, "  %s: shifted at %d, %zu contacts at %d\n", name.c_str(), at, w.contacts_.size(), end);
+}
+
 // Reference narrow phase on snapshots of a reference trajectory: records,
 // the exact pair set (sorted (lo<<32|hi)) and every pair's contacts from
 // collidePair in canonical order.
@@ -570,6 +600,11 @@ int fixture(const std::string& out, const std::string& name) {
         vxscene::Scene s = vxscene::lattice();
         for (vxscene::BodySpec& b : s.bodies) b.enableSleep = 1;
         trajectory(out, name, s, {150, 200, 250, 300}, false, true);
+        return 0;
+    }
+    if (name == "shift") {  // origin shifts of a box lattice and of a heightfield terrain scene
+        shiftTrajectory(out, "shift_lattice64", vxscene::lattice(4, 4, 4), 30, {1000.25f, -2.5f, 500.75f}, 60);
+        shiftTrajectory(out, "shift_terrain", vxscene::terrain(16, 0.5f, 0.8f, 40), 20, {-64.5f, 3.125f, 17.0f}, 40);
         return 0;
     }
     if (name == "settle") {  // sleeping on: towers fall asleep, a late strike wakes one

@@ -938,6 +938,72 @@ __global__ __launch_bounds__(kThreads) void kCacheGather(const vx_contact* __res
     out[t] = v < m ? cur[v] : prev[v - m];
 }
 
+// --------------------------------------------------------- origin shift --
+// World::shiftOrigin (world.cpp:1978-2036): every stored world coordinate
+// moves by -offset.  Run once with apply = false to flag a result that is not
+// finite (the reference checks everything before it changes anything), then
+// with apply = true.
+__device__ inline bool finite3(V3 v) { return isfinite(v.x) && isfinite(v.y) && isfinite(v.z); }
+
+__device__ inline void shiftVec(vx_vec3& p, V3 off, bool apply, bool& bad) {
+    const V3 r = v3(p) - off;
+    if (!finite3(r)) bad = true;
+    if (apply) p = raw(r);
+}
+
+__global__ __launch_bounds__(kThreads) void kShiftBodies(vx_body* __restrict__ bodies, uint32_t n, V3 off, int apply,
+                                                         uint32_t* __restrict__ bad) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    vx_body& b = bodies[i];
+    bool e = false;
+    shiftVec(b.position, off, apply, e);
+    if (b.shape == VX_SHAPE_PLANE) {
+        const float po = b.planeOffset - dot(v3(b.planeNormal), off);
+        if (!isfinite(po)) e = true;
+        if (apply) b.planeOffset = po;
+    }
+    if (e) atomicOr(bad, 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void kShiftPoints(vx_vec3* __restrict__ pts, uint32_t count, V3 off, int apply,
+                                                         uint32_t* __restrict__ bad) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= count) return;
+    bool e = false;
+    shiftVec(pts[i], off, apply, e);
+    if (e) atomicOr(bad, 1u);
+}
+
+template <class T>  // vx_mesh, vx_bvh_node: aabbMin / aabbMax
+__global__ __launch_bounds__(kThreads) void kShiftBounds(T* __restrict__ items, uint32_t count, V3 off, int apply,
+                                                         uint32_t* __restrict__ bad) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= count) return;
+    bool e = false;
+    shiftVec(items[i].aabbMin, off, apply, e);
+    shiftVec(items[i].aabbMax, off, apply, e);
+    if (e) atomicOr(bad, 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void kShiftContacts(vx_contact* __restrict__ cs, uint32_t count, V3 off,
+                                                           int apply, uint32_t* __restrict__ bad) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= count) return;
+    bool e = false;
+    shiftVec(cs[i].point, off, apply, e);
+    if (e) atomicOr(bad, 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void kShiftEvents(vx_contact_event* __restrict__ ev, uint32_t count, V3 off,
+                                                         int apply, uint32_t* __restrict__ bad) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= count || ev[i].type == 2u) return;  // End events carry no point
+    const V3 r = V3{ev[i].point[0], ev[i].point[1], ev[i].point[2]} - off;
+    if (!finite3(r)) atomicOr(bad, 1u);
+    if (apply) { ev[i].point[0] = r.x; ev[i].point[1] = r.y; ev[i].point[2] = r.z; }
+}
+
 // world.cpp:4364-4397, one colour at a time.
 __device__ __attribute__((always_inline)) inline void positionRow(vx_body* __restrict__ bodies,
                                                                  const vx_contact* __restrict__ cs, uint32_t k,

@@ -1,6 +1,7 @@
 // Host orchestration and C ABI of libvxhip: the device step of Velox on one
 // MI355X (gfx950), device-resident across steps.  See include/vxhip.h.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -110,6 +111,7 @@ struct vxhip_ctx {
     Buf<vx_bvh_node> nodes;
     Buf<vx_compound_child> children;
     Soup soup;
+    uint32_t vertexCount = 0, meshCount = 0, nodeCount = 0;
     size_t soupSig[8] = {};
     vxhip_config cfg{};
     bool anyAsleep = false;
@@ -202,7 +204,7 @@ enum Counter : uint32_t {
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
-    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cCount = 53
+    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cCount = 54
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -315,6 +317,7 @@ void uploadSoup(vxhip_ctx* c, const vx_soup* s) {
     upload(c, c->hullPts, s->hullPoints, s->hullPointCount);
     upload(c, c->hullFaces, s->hullFaceIndices, s->hullFaceIndexCount);
     upload(c, c->children, s->compoundChildren, s->compoundChildCount);
+    c->vertexCount = s->vertexCount; c->meshCount = s->meshCount; c->nodeCount = s->bvhNodeCount;
     c->soup.vertices = c->vertices.p; c->soup.indices = c->indices.p; c->soup.meshes = c->meshes.p;
     c->soup.nodes = c->nodes.p; c->soup.triRefs = c->triRefs.p; c->soup.hullPoints = c->hullPts.p;
     c->soup.hullFaceIndices = c->hullFaces.p; c->soup.children = c->children.p;
@@ -1210,6 +1213,118 @@ int vxhip_world_set_events(vxhip_ctx* c, int enable) {
         c->eventsEnabled = enable != 0;
         c->eventCount = 0;
         c->evPrevCount = 0;
+    });
+}
+
+int vxhip_world_shift_origin(vxhip_ctx* c, const float offset[3]) {
+    return guarded([&] {
+        if (!c || !c->loaded || !offset) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        const V3 off{offset[0], offset[1], offset[2]};
+        if (!std::isfinite(off.x) || !std::isfinite(off.y) || !std::isfinite(off.z))
+            fail(VXHIP_ERR_INVALID, "velox: origin shift must be finite");
+        hipStream_t s = c->stream;
+        uint32_t* bad = c->counters.p + cShiftBad;
+        for (int apply = 0; apply < 2; ++apply) {
+            if (!apply) kFill32<<<1, 64, 0, s>>>(bad, 1, 0u);
+            if (c->n) kShiftBodies<<<blocks(c->n), kThreads, 0, s>>>(c->bodies.p, c->n, off, apply, bad);
+            if (c->vertexCount)
+                kShiftPoints<<<blocks(c->vertexCount), kThreads, 0, s>>>(c->vertices.p, c->vertexCount, off, apply, bad);
+            if (c->meshCount)
+                kShiftBounds<vx_mesh><<<blocks(c->meshCount), kThreads, 0, s>>>(c->meshes.p, c->meshCount, off, apply, bad);
+            if (c->nodeCount)
+                kShiftBounds<vx_bvh_node><<<blocks(c->nodeCount), kThreads, 0, s>>>(c->nodes.p, c->nodeCount, off, apply,
+                                                                                   bad);
+            if (c->solveCount)
+                kShiftContacts<<<blocks(c->solveCount), kThreads, 0, s>>>(c->solve.p, c->solveCount, off, apply, bad);
+            if (c->prevCount)
+                kShiftContacts<<<blocks(c->prevCount), kThreads, 0, s>>>(c->prev.p, c->prevCount, off, apply, bad);
+            if (c->eventCount)
+                kShiftEvents<<<blocks(c->eventCount), kThreads, 0, s>>>(c->events.p, c->eventCount, off, apply, bad);
+            HIPCK(hipGetLastError());
+            if (!apply) {
+                uint32_t flag = 0;
+                HIPCK(hipMemcpyAsync(&flag, bad, sizeof(flag), hipMemcpyDeviceToHost, s));
+                HIPCK(hipStreamSynchronize(s));
+                if (flag) fail(VXHIP_ERR_INVALID, "velox: origin shift overflows world coordinates");
+            }
+        }
+        HIPCK(hipStreamSynchronize(s));
+    });
+}
+
+namespace {
+struct StateHeader {
+    uint32_t magic, version, n, prevCount, evPrevCount, eventCount, solveCount, reserved;
+};
+constexpr uint32_t kStateMagic = 0x54535856u;  // "VXST"
+
+size_t stateBytes(const vxhip_ctx* c) {
+    return sizeof(StateHeader) + (size_t)c->n * sizeof(vx_body) + (size_t)c->prevCount * sizeof(vx_contact) +
+           (size_t)c->evPrevCount * sizeof(uint64_t) + (size_t)c->eventCount * sizeof(vx_contact_event) +
+           (size_t)c->solveCount * sizeof(vx_contact);
+}
+}  // namespace
+
+int vxhip_world_state_size(vxhip_ctx* c, size_t* bytes) {
+    return guarded([&] {
+        if (!c || !c->loaded || !bytes) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *bytes = stateBytes(c);
+    });
+}
+
+int vxhip_world_save_state(vxhip_ctx* c, void* out, size_t cap) {
+    return guarded([&] {
+        if (!c || !c->loaded || !out) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (cap < stateBytes(c)) fail(VXHIP_ERR_CAPACITY, "vxhip: state buffer too small");
+        hipStream_t s = c->stream;
+        uint8_t* p = (uint8_t*)out;
+        const StateHeader h{kStateMagic, 1u, c->n, c->prevCount, c->evPrevCount, c->eventCount, c->solveCount, 0u};
+        std::memcpy(p, &h, sizeof h);
+        p += sizeof h;
+        auto down = [&](const void* src, size_t bytes) {
+            if (bytes) HIPCK(hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToHost, s));
+            p += bytes;
+        };
+        down(c->bodies.p, (size_t)c->n * sizeof(vx_body));
+        down(c->prev.p, (size_t)c->prevCount * sizeof(vx_contact));
+        down(c->evPrevKeys.p, (size_t)c->evPrevCount * sizeof(uint64_t));
+        down(c->events.p, (size_t)c->eventCount * sizeof(vx_contact_event));
+        down(c->solve.p, (size_t)c->solveCount * sizeof(vx_contact));
+        HIPCK(hipStreamSynchronize(s));
+    });
+}
+
+int vxhip_world_restore_state(vxhip_ctx* c, const void* in, size_t size) {
+    return guarded([&] {
+        if (!c || !c->loaded || !in || size < sizeof(StateHeader)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        StateHeader h;
+        std::memcpy(&h, in, sizeof h);
+        if (h.magic != kStateMagic || h.version != 1u) fail(VXHIP_ERR_INVALID, "vxhip: not a saved world state");
+        if (h.n != c->n) fail(VXHIP_ERR_INVALID, "vxhip: saved state belongs to a world of another size");
+        const size_t want = sizeof h + (size_t)h.n * sizeof(vx_body) + (size_t)h.prevCount * sizeof(vx_contact) +
+                            (size_t)h.evPrevCount * sizeof(uint64_t) + (size_t)h.eventCount * sizeof(vx_contact_event) +
+                            (size_t)h.solveCount * sizeof(vx_contact);
+        if (size != want) fail(VXHIP_ERR_INVALID, "vxhip: saved state has the wrong size");
+        const uint8_t* p = (const uint8_t*)in + sizeof h;
+        std::vector<vx_body> bodies(h.n);
+        if (h.n) std::memcpy(bodies.data(), p, (size_t)h.n * sizeof(vx_body));
+        p += (size_t)h.n * sizeof(vx_body);
+        setBodies(c, bodies.data(), h.n);
+        hipStream_t s = c->stream;
+        auto up = [&](auto& buf, size_t count, size_t elem) {
+            buf.reserve(count ? count : 1);
+            if (count) HIPCK(hipMemcpyAsync(buf.p, p, count * elem, hipMemcpyHostToDevice, s));
+            p += count * elem;
+        };
+        up(c->prev, h.prevCount, sizeof(vx_contact));
+        up(c->evPrevKeys, h.evPrevCount, sizeof(uint64_t));
+        up(c->events, h.eventCount, sizeof(vx_contact_event));
+        up(c->solve, h.solveCount, sizeof(vx_contact));
+        c->prevCount = h.prevCount;
+        c->evPrevCount = h.evPrevCount;
+        c->eventCount = h.eventCount;
+        c->solveCount = h.solveCount;
+        HIPCK(hipStreamSynchronize(s));
     });
 }
 

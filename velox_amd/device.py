@@ -87,6 +87,10 @@ SIGNATURES = {
     "vxhip_world_download_events": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_set_events": (_i32, [_vp, _i32]),
     "vxhip_world_set_sleep": (_i32, [_vp, _vp]),
+    "vxhip_world_shift_origin": (_i32, [_vp, _vp]),
+    "vxhip_world_state_size": (_i32, [_vp, _vp]),
+    "vxhip_world_save_state": (_i32, [_vp, _vp, C.c_size_t]),
+    "vxhip_world_restore_state": (_i32, [_vp, _vp, C.c_size_t]),
     "vxhip_world_sleep_stats": (_i32, [_vp, _vp]),
     "vxhip_world_sync": (_i32, [_vp]),
     "vxhip_world_download_bodies": (_i32, [_vp, _vp, _u32]),
@@ -321,6 +325,24 @@ class Context:
             return self
         cfg = config if isinstance(config, SleepConfig) else SleepConfig(**kw)
         check(library().vxhip_world_set_sleep(self._h, C.byref(cfg)))
+        return self
+
+    def shift_origin(self, offset):
+        off = (C.c_float * 3)(*offset)
+        check(library().vxhip_world_shift_origin(self._h, off))
+        return self
+
+    def save_state(self):
+        """Host blob of the device state the next step reads (bytes)."""
+        size = C.c_size_t(0)
+        check(library().vxhip_world_state_size(self._h, C.byref(size)))
+        buf = (C.c_uint8 * size.value)()
+        check(library().vxhip_world_save_state(self._h, buf, size.value))
+        return bytes(buf)
+
+    def restore_state(self, blob):
+        buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
+        check(library().vxhip_world_restore_state(self._h, buf, len(blob)))
         return self
 
     def sleep_stats(self):
