@@ -68,7 +68,8 @@ typedef struct {
     uint32_t bigIslandContacts;/* contacts solved colour by colour (islands above the size limit) */
     uint32_t largestIsland;    /* contacts in the largest island */
     uint32_t maxBatchPasses;   /* most solver passes (colour runs x chunks) one batch needs per sweep */
-    uint32_t reserved[2];
+    uint32_t multiToiEvents;   /* StepStats::multiToiEvents (processHighToi events) */
+    uint32_t reserved;
 } vxhip_stats;
 
 /* ---------------------------------------------------------------- setup -- */
@@ -180,6 +181,23 @@ int vxhip_world_set_events(vxhip_ctx* ctx, int enable);
 
 /* Counters of the last step (StepStats subset, world.h:175-208). */
 int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
+
+/* Multi-TOI scheduler settings (WorldMultiToiSettings, ccd.h:111-122; defaults
+ * 1, 4, 0.1, 1e-3, 256, 1).  MotionQuality::High bodies that touch only static
+ * or other such bodies are replayed by processHighToi (world.cpp:3019-3142)
+ * inside vxhip_world_step.  Invalid settings (validateMultiToiSettings,
+ * world.cpp:60-67) fail with VXHIP_ERR_INVALID.  Replaces
+ * World::setMultiToiSettings (world.cpp:580-584) for the device step. */
+typedef struct vx_multi_toi_settings {
+    uint8_t enabled;                 /* CcdConfig::enabled */
+    uint8_t enableSubstepSplitting;
+    uint8_t reserved[2];
+    uint32_t maxToiEventsPerBody;
+    float toiVelocityFloor;
+    float toiPenetrationBias;
+    uint32_t maxTotalEventsPerStep;
+} vx_multi_toi_settings;
+int vxhip_world_set_multi_toi(vxhip_ctx* ctx, const vx_multi_toi_settings* settings);
 
 /* Moves the world origin: every stored world coordinate of the resident world
  * (body positions, plane offsets, mesh vertices, mesh and BVH bounds, the last

@@ -111,6 +111,7 @@ std::vector<BodyId> build(World& w, const vxscene::Scene& s, bool sleep = false)
         // The sleep manager runs inside every reference World::step; scenes
         // opt in per body (the device step's sleep update is off by default).
         body.enableSleep = sleep ? b.enableSleep : 0;
+        body.ccdTuning.quality = (MotionQuality)b.quality;
         ids.push_back(id);
     }
     return ids;
@@ -326,7 +327,7 @@ std::vector<EventRecord> eventRecords(const World& w) {
 }
 
 void trajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene,
-                const std::vector<int>& steps, bool coneBlock = false, bool sleep = false) {
+                const std::vector<int>& steps, bool coneBlock = false, bool sleep = false, bool toi = false) {
     std::string dir = out + "/" + name;
     mkdir(dir.c_str(), 0755);
     World w(BackendType::Cpu);
@@ -342,9 +343,11 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
     std::string colorsJson = ", \"contacts_at\": [";
     std::string kJson = ", \"colors\": [";
     std::string sleepJson = ", \"sleep_stats\": [";
+    std::string toiJson = ", \"toi_events\": [";
     for (int target : steps) {
         while (done < target) { w.step(scene.dt); ++done; }
         writeRecords(dir + "/state_" + std::to_string(target) + ".npy", states(w.bodies_));
+        toiJson += (target == steps[0] ? "" : ", ") + std::to_string(w.lastStepStats_.multiToiEvents);
         if (sleep) {  // full records: asleep byte and sleep timer
             writeRecords(dir + "/bodies_" + std::to_string(target) + ".npy", w.bodies_);
             const SleepStats& st = w.sleepStats();
@@ -363,7 +366,8 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
         std::fprintf(stderr, "  %s step %d: %zu contacts\n", name.c_str(), target, w.contacts_.size());
     }
     writeRecords(dir + "/bodies_out.npy", w.bodies_);
-    writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]" + (sleep ? sleepJson + "]" : std::string()));
+    writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]" + (sleep ? sleepJson + "]" : std::string()) +
+                                           (toi ? toiJson + "]" : std::string()));
 }
 
 // Origin shift mid-trajectory (World::shiftOrigin, world.cpp:1978-2036): the
@@ -600,6 +604,22 @@ int fixture(const std::string& out, const std::string& name) {
         vxscene::Scene s = vxscene::lattice();
         for (vxscene::BodySpec& b : s.bodies) b.enableSleep = 1;
         trajectory(out, name, s, {150, 200, 250, 300}, false, true);
+        return 0;
+    }
+    if (name == "hightoi") {  // MotionQuality::High bodies through processHighToi
+        trajectory(out, name, vxscene::highToi(), {1, 3, 5, 6, 7, 8, 12, 18, 28, 40, 60}, false, false, true);
+        return 0;
+    }
+    if (name == "toicount") {  // per-step multi-TOI event counts (tuning aid)
+        vxscene::Scene s = vxscene::highToi();
+        World w(BackendType::Cpu);
+        build(w, s);
+        w.backend_.reset(new CudaSemantics());
+        for (int k = 1; k <= 60; ++k) {
+            w.step(s.dt);
+            std::fprintf(stderr, "step %d: toi events %u contacts %zu\n", k, (unsigned)w.lastStepStats_.multiToiEvents,
+                         w.contacts_.size());
+        }
         return 0;
     }
     if (name == "shift") {  // origin shifts of a box lattice and of a heightfield terrain scene

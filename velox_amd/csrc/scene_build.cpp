@@ -349,6 +349,7 @@ void addBody(vxhip_scene& sc, const vxscene::Scene& s, const vxscene::BodySpec& 
     b.friction = sp.friction;
     b.restitution = sp.restitution;
     b.enableSleep = sp.enableSleep;
+    b.ccdTuning.quality = sp.quality;
     sc.bodies.push_back(b);
     sc.worldOf.push_back(sp.world);
 }
@@ -381,6 +382,7 @@ vxscene::Scene vxSceneByName(const std::string& name, uint32_t param, bool& ok) 
     if (name == "batch") return vxscene::batchWorlds(param ? param : 4096);
     if (name == "ccdstress") return vxscene::ccdStress();
     if (name == "settle") return vxscene::settle();
+    if (name == "hightoi") return vxscene::highToi();
     // The exact descriptions behind the golden trajectory fixtures.
     if (name == "fixture-pile") return vxscene::pile(160, 5.0f, 6.0f);
     if (name == "fixture-terrain") return vxscene::terrain(16, 0.5f, 0.8f, 40);

@@ -58,6 +58,7 @@ struct BodySpec {
     uint32_t hfIndex = 0;         // into Scene::heightfields
     uint32_t world = 0;           // batched scenes: owning world
     uint8_t enableSleep = 0;      // Body::enableSleep (body.h:155)
+    uint8_t quality = 1;          // BodyCcdTuning::quality (ccd.h:13): 1 Medium, 2 High
 };
 
 struct HeightfieldSpec {
@@ -112,7 +113,7 @@ inline double hullVolume(const float* p, uint32_t n) {
 // Velox's demo scenes seed the same way; one seed per workload.
 constexpr uint32_t kSeedLattice = 1001u, kSeedDebris = 2002u, kSeedCcd = 3003u,
                    kSeedBatch = 4004u, kSeedDiff = 5005u,
-                   kSeedSleep = 6006u;
+                   kSeedSleep = 6006u, kSeedToi = 7007u;
 
 // 1,000 boxes (half-extent 0.5, density 1000) in a 10x10x10 lattice at 1.1 m
 // spacing with U(-0.05,0.05) jitter and uniform orientations, on a static
@@ -390,6 +391,66 @@ inline Scene settle(uint32_t seed = kSeedSleep) {
     striker.restitution = 0.0f;
     striker.enableSleep = 1;
     s.bodies.push_back(striker);
+    return s;
+}
+
+// High-quality (MotionQuality::High) bodies for the multi-TOI scheduler: 24
+// spheres (r = 0.1 m, 1 kg) fired along +x at 60-240 m/s into a wall of 12
+// static boxes, 4 High boxes fired at each other in two pairs at 40 m/s, and
+// 16 Medium boxes dropped on the plane away from them.  Friction 0.5,
+// restitution 0.3 (Body defaults).
+inline Scene highToi(uint32_t seed = kSeedToi) {
+    Scene s;
+    Rng r(seed);
+    s.bodies.push_back(plane());
+    for (int j = 0; j < 3; ++j)
+        for (int k = 0; k < 4; ++k) {
+            BodySpec w;
+            w.kind = Box;
+            w.mass = 0.0f;
+            w.half[0] = 0.25f; w.half[1] = 1.0f; w.half[2] = 1.0f;
+            w.pos[0] = 12.0f; w.pos[1] = 1.0f + 2.0f * (float)j; w.pos[2] = -3.0f + 2.0f * (float)k;
+            s.bodies.push_back(w);
+        }
+    for (int i = 0; i < 24; ++i) {
+        BodySpec b;
+        b.kind = Sphere;
+        b.radius = 0.1f;
+        b.mass = 1.0f;
+        b.pos[0] = r.range(-2.0f, 2.0f);
+        b.pos[1] = r.range(0.5f, 5.5f);
+        b.pos[2] = r.range(-3.5f, 3.5f);
+        b.vel[0] = r.range(60.0f, 240.0f);
+        b.vel[1] = r.range(-2.0f, 2.0f);
+        b.vel[2] = r.range(-2.0f, 2.0f);
+        b.quality = 2;
+        s.bodies.push_back(b);
+    }
+    for (int p = 0; p < 2; ++p)
+        for (int side = 0; side < 2; ++side) {
+            BodySpec b;
+            b.kind = Box;
+            b.half[0] = b.half[1] = b.half[2] = 0.2f;
+            b.mass = 5.0f;
+            b.pos[0] = side ? -4.0f : -12.0f;
+            b.pos[1] = 2.0f + 3.0f * (float)p;
+            b.pos[2] = 8.0f + 0.05f * (float)side;
+            r.quat(b.quat);
+            b.vel[0] = side ? -40.0f : 40.0f;
+            b.quality = 2;
+            s.bodies.push_back(b);
+        }
+    for (int i = 0; i < 16; ++i) {
+        BodySpec b;
+        b.kind = Box;
+        b.half[0] = b.half[1] = b.half[2] = 0.3f;
+        b.mass = 10.0f;
+        b.pos[0] = -20.0f + 1.5f * (float)(i % 4);
+        b.pos[1] = 0.6f + 1.0f * (float)(i / 4);
+        b.pos[2] = -20.0f;
+        r.quat(b.quat);
+        s.bodies.push_back(b);
+    }
     return s;
 }
 

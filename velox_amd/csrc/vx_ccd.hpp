@@ -96,7 +96,8 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
                                                       const uint32_t* __restrict__ counts, uint32_t np, int nb,
                                                       uint8_t* __restrict__ flag, float4* __restrict__ endGap,
                                                       uint32_t* __restrict__ flagged,
-                                                      uint32_t* __restrict__ firstFlagged) {
+                                                      uint32_t* __restrict__ firstFlagged,
+                                                      const uint8_t* __restrict__ toiDone) {
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
     uint8_t f = 0;
@@ -104,7 +105,8 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
         uint32_t i, j;
         ccdBodies(bodies, pairs[p], nb, i, j);
         const vx_body a = bodies[i], b = bodies[j];
-        if (ccdEligible(a, b)) {
+        // Bodies the multi-TOI scheduler replayed are left alone (world.cpp:4268-4269).
+        if (ccdEligible(a, b) && !(toiDone && (toiDone[i] || toiDone[j]))) {
             Gap end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup,
                             ccdSearch(a, dt));
             if (end.gap < -8e-3f) {
@@ -284,7 +286,8 @@ __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const 
                                                          const float4* __restrict__ endGap,
                                                          const uint32_t* __restrict__ pred, uint32_t* word,
                                                          uint32_t* __restrict__ examined,
-                                                         uint32_t* __restrict__ rescued, uint32_t* __restrict__ error) {
+                                                         uint32_t* __restrict__ rescued, uint32_t* __restrict__ error,
+                                                         const uint8_t* __restrict__ toiDone) {
     const uint32_t waves = gridDim.x * (kThreads / 64);
     const uint32_t wave = blockIdx.x * (kThreads / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint32_t stride = 64 * waves;
@@ -300,9 +303,9 @@ __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const 
             const bool movedB = pb != kNone && ((wb >> (1 + (pb & 1))) & 1u);
             if (movedA || movedB) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             bool r = false;
-            if (flag[p] || movedA || movedB) {
-                uint32_t i, j;
-                ccdBodies(bodies, pairs[p], nb, i, j);
+            uint32_t i = 0, j = 0;
+            if (flag[p] || movedA || movedB) ccdBodies(bodies, pairs[p], nb, i, j);
+            if ((flag[p] || movedA || movedB) && !(toiDone && (toiDone[i] || toiDone[j]))) {
                 atomicAdd(examined, 1u);
                 r = ccdRescue(bodies, prevPos, prevQ, i, j, soup, dt, (movedA || movedB) ? nullptr : &endGap[p]);
                 if (r) atomicAdd(rescued, 1u);

@@ -10,6 +10,7 @@
 #include "vx_ccd.hpp"
 #include "vx_solve.hpp"
 #include "vx_sleep.hpp"
+#include "vx_toi.hpp"
 #include "vx_internal.hpp"
 #include "../../include/vxhip.h"
 
@@ -172,6 +173,15 @@ struct vxhip_ctx {
     uint32_t tailColor = 0;
     Buf<uint64_t> tailKeys, tailKeysTmp;
     Buf<uint32_t> tailVals, tailValsTmp, tailHead, tailPos, tailRowSlot, tailSlotBody;
+    // Multi-TOI scheduler for MotionQuality::High bodies.
+    bool anyHigh = false, toiProcessed = false;
+    vxk::ToiSettings toiSettings{1u, 1u, 4u, 256u, 0.1f, 1e-3f};  // ccd.h:111-122 defaults
+    Buf<vx_body> toiStarts;
+    Buf<uint8_t> toiEligible;
+    Buf<uint32_t> toiE, toiS, toiEvents, toiBits;
+    Buf<float4> toiHit;
+    Buf<vxk::ToiState> toiState;
+    uint32_t toiEventCount = 0;
     Buf<uint32_t> ccdDone;
     Buf<float4> ccdGap;
     Buf<uint8_t> temp;
@@ -346,7 +356,12 @@ void setBodies(vxhip_ctx* c, const vx_body* b, uint32_t n) {
     c->n = n;
     upload(c, c->bodies, b, n);
     c->anyAsleep = false;
-    for (uint32_t i = 0; i < n; ++i) c->anyAsleep |= b[i].asleep == 1;
+    c->anyHigh = false;
+    for (uint32_t i = 0; i < n; ++i) {
+        c->anyAsleep |= b[i].asleep == 1;
+        c->anyHigh |= b[i].motionType == VX_MOTION_DYNAMIC && b[i].ccdTuning.quality == VX_QUALITY_HIGH &&
+                      b[i].ccdTuning.enableContinuous;
+    }
 }
 
 int bodyBits(uint32_t n) { return vxhost::bitsFor(n ? n - 1 : 0) ? vxhost::bitsFor(n ? n - 1 : 0) : 1; }
@@ -766,6 +781,64 @@ void positionCorrection(vxhip_ctx* c, const vx_contact* solve, uint32_t K, float
     HIPCK(hipGetLastError());
 }
 
+__global__ void kToiInit(vxk::ToiState* st, float dt) {
+    if (threadIdx.x || blockIdx.x) return;
+    st->remaining = dt;
+    st->bestBits = 0xffffffffu;
+    st->bestPair = ~0ull;
+    st->prevA = st->prevB = vxk::kNone;
+    st->total = st->eligible = st->statics = 0;
+}
+
+__global__ void kToiReset(vxk::ToiState* st) {
+    if (threadIdx.x || blockIdx.x) return;
+    st->bestBits = 0xffffffffu;
+    st->bestPair = ~0ull;
+}
+
+// processHighToi (world.cpp:3019-3142): the event loop runs on the host, one
+// device search per event (a few per step in practice).
+void highToi(vxhip_ctx* c, uint32_t m, float dt) {
+    c->toiProcessed = false;
+    c->toiEventCount = 0;
+    const vxk::ToiSettings& ts = c->toiSettings;
+    if (!c->anyHigh || !ts.enabled || !ts.splitting) return;
+    hipStream_t s = c->stream;
+    const uint32_t n = c->n;
+    c->toiEligible.reserve(n); c->toiE.reserve(n); c->toiS.reserve(n); c->toiEvents.reserve(n); c->toiState.reserve(1);
+    vxk::ToiState* st = c->toiState.p;
+    kToiEligible<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->toiStarts.p, n, ts, c->toiEligible.p);
+    if (m) kToiContactPass<<<1, kWakeThreads, 0, s>>>(c->bodies.p, c->solve.p, m, c->toiEligible.p);
+    kToiInit<<<1, 1, 0, s>>>(st, dt);
+    fill32(c, c->toiEvents.p, n, 0u);
+    kToiRewind<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->toiStarts.p, n, c->toiEligible.p, c->toiE.p, c->toiS.p, st);
+    HIPCK(hipGetLastError());
+    vxk::ToiState h;
+    HIPCK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCK(hipStreamSynchronize(s));
+    const uint32_t ne = h.eligible, ns = h.statics;
+    if (!ne) return;
+    c->toiProcessed = true;
+    const uint64_t P = (uint64_t)ne * (ne + ns);
+    c->toiBits.reserve(P); c->toiHit.reserve(2 * P);
+    while (h.remaining > 1e-6f && h.total < ts.maxTotal) {
+        kToiReset<<<1, 1, 0, s>>>(st);
+        kToiPairs<<<blocks(P), kThreads, 0, s>>>(c->bodies.p, c->soup, ts, c->toiE.p, c->toiS.p, c->toiEvents.p, ne, ns,
+                                                st, c->toiBits.p, c->toiHit.p);
+        kToiPick<<<blocks(P), kThreads, 0, s>>>(c->toiE.p, c->toiS.p, ne, ns, c->toiBits.p, st);
+        kToiSelect<<<blocks(P), kThreads, 0, s>>>(c->toiE.p, c->toiS.p, ne, ns, c->toiBits.p, c->toiHit.p, st);
+        kToiAdvance<<<blocks(ne), kThreads, 0, s>>>(c->bodies.p, c->toiE.p, ne, st, 0);
+        kToiImpulse<<<1, 1, 0, s>>>(c->bodies.p, ts, c->toiEligible.p, c->toiEvents.p, st);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(&h, st, sizeof h, hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (h.bestBits == 0xffffffffu) break;
+    }
+    kToiAdvance<<<blocks(ne), kThreads, 0, s>>>(c->bodies.p, c->toiE.p, ne, st, 1);
+    HIPCK(hipGetLastError());
+    c->toiEventCount = h.total;
+}
+
 void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     if (np == 0) return;
     hipStream_t s = c->stream;
@@ -775,8 +848,9 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     const int pbits = vxhost::bitsFor(np - 1) ? vxhost::bitsFor(np - 1) : 1;
     c->ccdFlag.reserve(np);
     c->ccdGap.reserve(np);
+    const uint8_t* toiDone = c->toiProcessed ? c->toiEligible.p : nullptr;
     kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb, c->ccdFlag.p,
-                                              c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst);
+                                              c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst, toiDone);
     HIPCK(hipGetLastError());
     readCounters(c);
     if (c->hostCounters[cCcdFlagged] == 0) return;
@@ -818,7 +892,7 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     const uint32_t grid = std::min<uint32_t>(blocks(nr, kThreads / 64), kCcdBlocks);  // one entry per wavefront first
     if (nr) kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->ccdList.p,
                                            nr, nb, c->ccdFlag.p, c->ccdGap.p, c->pred.p, c->ccdDone.p,
-                                           k + cCcdExamined, k + cCcdRescued, k + cColorError);
+                                           k + cCcdExamined, k + cCcdRescued, k + cColorError, toiDone);
     HIPCK(hipGetLastError());
 }
 
@@ -979,6 +1053,10 @@ void worldStep(vxhip_ctx* c, float dt) {
     kResetCounters<<<1, 64, 0, s>>>(k, false);
     kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
     HIPCK(hipGetLastError());
+    if (c->anyHigh && c->toiSettings.enabled && c->toiSettings.splitting && n) {  // world.cpp:4015-4025
+        c->toiStarts.reserve(n);
+        HIPCK(hipMemcpyAsync(c->toiStarts.p, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+    }
     markStage(c, sBroad);
     const uint32_t np = broadPhase(c, dt);
     markStage(c, sNarrow);
@@ -1001,6 +1079,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     HIPCK(hipGetLastError());
     markStage(c, sCcd);
     kResetCounters<<<1, 64, 0, s>>>(k);
+    highToi(c, m, dt);
     ccdNet(c, np, dt);
     markStage(c, sPosition);
     if (m) positionCorrection(c, c->solve.p, K, h, cfg.options);
@@ -1044,6 +1123,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->stats.bigIslandContacts = m ? c->bigCount : 0;
     c->stats.largestIsland = m ? c->largestIsland : 0;
     c->stats.maxBatchPasses = m ? c->maxBatchPasses : 0;
+    c->stats.multiToiEvents = c->toiEventCount;
 }
 
 template <class F>
@@ -1213,6 +1293,18 @@ int vxhip_world_set_events(vxhip_ctx* c, int enable) {
         c->eventsEnabled = enable != 0;
         c->eventCount = 0;
         c->evPrevCount = 0;
+    });
+}
+
+int vxhip_world_set_multi_toi(vxhip_ctx* c, const vx_multi_toi_settings* v) {
+    return guarded([&] {
+        if (!c || !v) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (v->maxToiEventsPerBody == 0 || !std::isfinite(v->toiVelocityFloor) || v->toiVelocityFloor < 0.0f ||
+            !std::isfinite(v->toiPenetrationBias) || v->toiPenetrationBias < 0.0f || v->maxTotalEventsPerStep == 0)
+            fail(VXHIP_ERR_INVALID, "velox: multi-TOI settings are invalid");
+        c->toiSettings = vxk::ToiSettings{v->enabled ? 1u : 0u, v->enableSubstepSplitting ? 1u : 0u,
+                                          v->maxToiEventsPerBody, v->maxTotalEventsPerStep, v->toiVelocityFloor,
+                                          v->toiPenetrationBias};
     });
 }
 

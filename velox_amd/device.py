@@ -32,7 +32,8 @@ class Stats(C.Structure):
                 ("ccdCandidates", C.c_uint32), ("ccdRescued", C.c_uint32), ("velocityIterations", C.c_uint32),
                 ("oversize", C.c_uint32), ("cellSize", C.c_float), ("islands", C.c_uint32),
                 ("islandContacts", C.c_uint32), ("bigIslandContacts", C.c_uint32),
-                ("largestIsland", C.c_uint32), ("maxBatchPasses", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+                ("largestIsland", C.c_uint32), ("maxBatchPasses", C.c_uint32), ("multiToiEvents", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
@@ -51,6 +52,19 @@ class SleepConfig(C.Structure):
                         timeToSleep=0.5, timeToDrowsy=0.25, drowsySimulationRate=0.25,
                         contactStabilityThreshold=0.02, contactStabilityFrames=10, enableGradualSleep=1,
                         enableContactStability=1)
+        defaults.update(kw)
+        super().__init__(**defaults)
+
+
+class MultiToiSettings(C.Structure):
+    """vx_multi_toi_settings (WorldMultiToiSettings, ccd.h:111-122) with the reference defaults."""
+    _fields_ = [("enabled", C.c_uint8), ("enableSubstepSplitting", C.c_uint8), ("reserved", C.c_uint8 * 2),
+                ("maxToiEventsPerBody", C.c_uint32), ("toiVelocityFloor", C.c_float),
+                ("toiPenetrationBias", C.c_float), ("maxTotalEventsPerStep", C.c_uint32)]
+
+    def __init__(self, **kw):
+        defaults = dict(enabled=1, enableSubstepSplitting=1, maxToiEventsPerBody=4, toiVelocityFloor=0.1,
+                        toiPenetrationBias=1e-3, maxTotalEventsPerStep=256)
         defaults.update(kw)
         super().__init__(**defaults)
 
@@ -88,6 +102,7 @@ SIGNATURES = {
     "vxhip_world_set_events": (_i32, [_vp, _i32]),
     "vxhip_world_set_sleep": (_i32, [_vp, _vp]),
     "vxhip_world_shift_origin": (_i32, [_vp, _vp]),
+    "vxhip_world_set_multi_toi": (_i32, [_vp, _vp]),
     "vxhip_world_state_size": (_i32, [_vp, _vp]),
     "vxhip_world_save_state": (_i32, [_vp, _vp, C.c_size_t]),
     "vxhip_world_restore_state": (_i32, [_vp, _vp, C.c_size_t]),
@@ -325,6 +340,11 @@ class Context:
             return self
         cfg = config if isinstance(config, SleepConfig) else SleepConfig(**kw)
         check(library().vxhip_world_set_sleep(self._h, C.byref(cfg)))
+        return self
+
+    def set_multi_toi(self, **kw):
+        cfg = MultiToiSettings(**kw)
+        check(library().vxhip_world_set_multi_toi(self._h, C.byref(cfg)))
         return self
 
     def shift_origin(self, offset):
