@@ -190,6 +190,47 @@ typedef struct {
     uint32_t sensor;
 } vx_contact_event;
 
+/* velox::Joint (include/velox/joint.h:99-186), 420 B, byte for byte. */
+typedef struct vx_joint {
+    uint8_t type;                 /* JointType (joint.h:48-62) */
+    uint8_t pad0[3];
+    uint32_t a, b;                /* dense body indices */
+    vx_vec3 localAnchorA, localAnchorB;
+    vx_vec3 localAxisA, localAxisB;
+    vx_vec3 localRefA, localRefB;
+    float restLength;
+    uint8_t collideConnected, enableSpring, pad1[2];
+    float springFrequencyHz, springDampingRatio;
+    float breakForce, breakTorque;
+    uint8_t enableMotor, pad2[3];
+    float motorSpeed, maxMotorTorque, maxMotorForce;
+    uint8_t enableLimit, pad3[3];
+    float lowerLimit, upperLimit;
+    uint8_t enableSwingLimit, pad4[3];
+    float swingLimit;
+    uint8_t enableTwistLimit, pad5[3];
+    float lowerTwistLimit, upperTwistLimit;
+    uint8_t linearLimitMask, angularLimitMask, linearMotorMask, angularMotorMask;
+    vx_vec3 lowerLinearLimit, upperLinearLimit, lowerAngularLimit, upperAngularLimit;
+    vx_vec3 linearMotorSpeed, angularMotorSpeed, maxLinearMotorForce, maxAngularMotorTorque;
+    float weldFrequencyHz, weldDampingRatio;
+    float suspensionFrequencyHz, suspensionDampingRatio, wheelRadius;
+    uint8_t enableSteering, pad6[3];
+    float steeringAngle, maxSteeringAngle, maxLength;
+    vx_vec3 pulleyAnchorA, pulleyAnchorB;
+    float pulleyRatio, gearRatio;
+    float motorImpulse, limitImpulse, swingImpulse, twistImpulse, springImpulse;
+    vx_vec3 linearMotorImpulse, angularMotorImpulse, linearLimitImpulse, angularLimitImpulse;
+    vx_vec3 reactionLinearImpulse, reactionAngularImpulse;
+    uint8_t broken, pad7[3];
+} vx_joint;
+
+enum {
+    VX_JOINT_BALL = 0, VX_JOINT_DISTANCE = 1, VX_JOINT_HINGE = 2, VX_JOINT_CONE_TWIST = 3, VX_JOINT_FIXED = 4,
+    VX_JOINT_PRISMATIC = 5, VX_JOINT_SIX_DOF = 6, VX_JOINT_MOTOR = 7, VX_JOINT_WELD = 8, VX_JOINT_WHEEL = 9,
+    VX_JOINT_ROPE = 10, VX_JOINT_PULLEY = 11, VX_JOINT_GEAR = 12
+};
+
 /* Raw-pointer view of the flat static geometry (MeshSoupView, backend.h:25-34)
  * plus the element counts a device copy needs. */
 typedef struct {
@@ -208,6 +249,14 @@ typedef struct {
 static_assert(sizeof(vx_ccd_settings) == 32, "CcdQualitySettings layout");
 static_assert(sizeof(vx_ccd_tuning) == 52, "BodyCcdTuning layout");
 static_assert(sizeof(vx_body) == 320, "Body layout");
+static_assert(sizeof(vx_joint) == 420, "Joint layout");
+static_assert(offsetof(vx_joint, restLength) == 84 && offsetof(vx_joint, springFrequencyHz) == 92 &&
+              offsetof(vx_joint, enableMotor) == 108 && offsetof(vx_joint, enableLimit) == 124 &&
+              offsetof(vx_joint, linearLimitMask) == 156 && offsetof(vx_joint, lowerLinearLimit) == 160 &&
+              offsetof(vx_joint, weldFrequencyHz) == 256 && offsetof(vx_joint, enableSteering) == 276 &&
+              offsetof(vx_joint, pulleyAnchorA) == 292 && offsetof(vx_joint, motorImpulse) == 324 &&
+              offsetof(vx_joint, reactionLinearImpulse) == 392 && offsetof(vx_joint, broken) == 416,
+              "Joint field offsets (joint.h:99-186)");
 static_assert(offsetof(vx_body, ccdTuning) == 156, "Body layout");
 static_assert(offsetof(vx_body, shape) == 232, "Body layout");
 static_assert(offsetof(vx_body, compoundCount) == 296, "Body layout");

@@ -182,6 +182,30 @@ int vxhip_world_set_events(vxhip_ctx* ctx, int enable);
 /* Counters of the last step (StepStats subset, world.h:175-208). */
 int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
 
+/* Joints of the resident world (velox::Joint records, dense body indices).
+ * Like the reference's CUDA backend (backend_cuda.cu:691-761), the device
+ * solves them only when there are at least 64: fewer fail with
+ * VXHIP_ERR_UNAVAILABLE and stay with the host's World::solveJoints.  Each
+ * vxhip_world_step then drops contacts of joint-connected pairs
+ * (world.cpp:4045-4066), wakes sleeping bodies jointed to awake ones, solves
+ * the joints in colour order after the contacts of every substep
+ * (joint_solver.h, 8 passes), and removes broken joints as
+ * World::finishBrokenJoints does (world.cpp:3765-3783): from the end, each
+ * swapped with the last.  vxhip_world_load clears the joints. */
+int vxhip_world_set_joints(vxhip_ctx* ctx, const vx_joint* joints, uint32_t count);
+
+/* The joints after the last step (after removals). */
+int vxhip_world_download_joints(vxhip_ctx* ctx, vx_joint* out, uint32_t cap, uint32_t* count);
+
+/* Joints removed by the last step, in removal order (World::jointBreakEvents,
+ * world.cpp:3776-3780): the joint's dense index when removed, its bodies,
+ * reaction force and torque. */
+typedef struct vx_joint_break {
+    uint32_t index, a, b;
+    float force, torque;
+} vx_joint_break;
+int vxhip_world_download_joint_breaks(vxhip_ctx* ctx, vx_joint_break* out, uint32_t cap, uint32_t* count);
+
 /* Multi-TOI scheduler settings (WorldMultiToiSettings, ccd.h:111-122; defaults
  * 1, 4, 0.1, 1e-3, 256, 1).  MotionQuality::High bodies that touch only static
  * or other such bodies are replayed by processHighToi (world.cpp:3019-3142)

@@ -42,6 +42,7 @@
 #undef private
 #undef protected
 #include "narrowphase.h"
+#include "joint_solver.h"
 #include "../velox_amd/csrc/scene_gen.hpp"
 
 using namespace velox;
@@ -246,14 +247,74 @@ public:
         sweeps(bodies, cs, dt, o);
         iters_ = 2 * o.velocityIterations;
     }
+    // backend_cuda.cu:691-761: device substeps, with device joints when there
+    // are at least 64 of a supported type (joint_solver::solve coloured like
+    // prepareJoints, backend_cuda.cu:800-843; reset / colour sweeps /
+    // break marking per substep, backend_cuda.cu:322-348).
     bool advanceSubsteps(std::vector<Body>& bodies, std::vector<Contact>& cs, std::vector<Joint>& joints,
                          const Vec3& g, float h, int S, const SolverOptions& o) override {
-        if (!joints.empty() || o.iterationPolicy == IterationPolicy::Adaptive) return false;
+        if (!joints.empty() && joints.size() < 64) return false;
+        for (const Joint& j : joints)
+            if (!joint_solver::supported(j)) return false;
+        for (const Joint& j : joints) {
+            if (bodies[j.a].asleep == bodies[j.b].asleep) continue;
+            bodies[j.a].asleep = bodies[j.b].asleep = 0;
+            bodies[j.a].sleepTimer = bodies[j.b].sleepTimer = 0.0f;
+        }
+        if (o.iterationPolicy == IterationPolicy::Adaptive) return false;
         solveVelocities(bodies, cs, h, true, o);
+        if (bodies.empty()) return true;
+        // Greedy joint colours in joint order: first free bit of the two
+        // dynamic endpoints' 64-bit masks, else past both bodies' last colour.
+        std::vector<uint64_t> mask(bodies.size(), 0);
+        std::vector<int> next(bodies.size(), 64), colorOf(joints.size());
+        int numColors = 0;
+        for (size_t k = 0; k < joints.size(); ++k) {
+            const Joint& j = joints[k];
+            const bool sa = !bodies[j.a].isDynamic(), sb = !bodies[j.b].isDynamic();
+            const uint64_t used = (sa ? 0 : mask[j.a]) | (sb ? 0 : mask[j.b]);
+            int color;
+            if (~used) {
+                color = 0;
+                while (used & (1ull << color)) ++color;
+            } else {
+                const int na = sa ? 64 : next[j.a], nbv = sb ? 64 : next[j.b];
+                color = na > nbv ? na : nbv;
+                if (!sa) next[j.a] = color + 1;
+                if (!sb) next[j.b] = color + 1;
+            }
+            if (color < 64) {
+                if (!sa) mask[j.a] |= 1ull << color;
+                if (!sb) mask[j.b] |= 1ull << color;
+            }
+            colorOf[k] = color;
+            if (color + 1 > numColors) numColors = color + 1;
+        }
         for (int s = 0; s < S; ++s) {
             if (s > 0) {
                 cpu_->integrate(bodies, g, h);
                 if (!cs.empty()) sweeps(bodies, cs, h, o);
+            }
+            if (!joints.empty()) {
+                for (Joint& j : joints)
+                    if (!(s > 0 && j.broken)) joint_solver::reset(j);
+                for (int it = 0; it < joint_solver::kIterations; ++it)
+                    for (int c = 0; c < numColors; ++c)
+                        for (size_t k = 0; k < joints.size(); ++k) {
+                            if (colorOf[k] != c) continue;
+                            Joint& j = joints[k];
+                            if (j.broken) continue;
+                            Body& a = bodies[j.a];
+                            Body& b = bodies[j.b];
+                            if (isFullyAsleep(a.asleep) && isFullyAsleep(b.asleep)) continue;
+                            joint_solver::solve(j, a, b, h);
+                        }
+                for (Joint& j : joints) {
+                    if (j.broken) continue;
+                    const float force = length(j.reactionLinearImpulse) / h;
+                    const float torque = length(j.reactionAngularImpulse) / h;
+                    j.broken = force > j.breakForce || torque > j.breakTorque;
+                }
             }
             for (Body& b : bodies)
                 if (!(b.isStatic() || b.isLocked() || isFullyAsleep(b.asleep))) b.advanceTransform(h);
@@ -398,6 +459,114 @@ void shiftTrajectory(const std::string& out, const std::string& name, const vxsc
                   offset.z);
     writeMeta(dir, w, scene.dt, {end}, extra);
     std::fprintf(stderr, "  %s: shifted at %d, %zu contacts at %d\n", name.c_str(), at, w.contacts_.size(), end);
+}
+
+// Joint workload: 8 chains of 8 boxes hanging from static anchors by ball
+// joints (64 joints; two of them breakable), plus one joint of most other
+// types between their own bodies (hinge with motor, prismatic with limits
+// and motor, rigid distance, spring, rope, fixed, weld, motor, pulley, gear,
+// wheel, cone-twist without limits).  Written with the reference World's own
+// add*Joint calls; joints_in.npy holds the resulting 420-byte records.
+void jointScene(World& w, std::vector<BodyId>& ids) {
+    w.gravity = {0.0f, -9.81f, 0.0f};
+    w.substeps = 4;
+    ids.push_back(w.addStaticPlane({0, 1, 0}, 0.0f));
+    auto box = [&](Vec3 p, float half, float mass) {
+        BodyId id = w.addBox(p, {half, half, half}, mass);
+        w.body(id).enableSleep = 0;
+        ids.push_back(id);
+        return id;
+    };
+    for (int c = 0; c < 8; ++c) {
+        const float x0 = -20.0f + 5.0f * (float)c, z = (float)(c % 2) * 0.3f;
+        BodyId prev = box({x0, 12.0f, z}, 0.1f, 0.0f);
+        for (int k = 0; k < 8; ++k) {
+            BodyId cur = box({x0 + 0.5f * (float)(k + 1), 12.0f, z}, 0.2f, 1.0f);
+            JointId j = w.addBallJoint(prev, cur, {x0 + 0.5f * (float)k + 0.25f, 12.0f, z});
+            if (c == 7 && k == 3) w.joint(j).breakForce = 30.0f;
+            if (c == 6 && k == 0) w.joint(j).breakForce = 400.0f;
+            prev = cur;
+        }
+    }
+    const float y = 6.0f;
+    {  // hinge with motor
+        BodyId a = box({-20.0f, y, 10.0f}, 0.1f, 0.0f), b = box({-19.5f, y, 10.0f}, 0.2f, 2.0f);
+        JointId j = w.addHingeJoint(a, b, {-20.0f, y, 10.0f}, {0, 0, 1});
+        w.joint(j).enableMotor = true; w.joint(j).motorSpeed = 2.0f; w.joint(j).maxMotorTorque = 50.0f;
+    }
+    {  // prismatic with limits and motor
+        BodyId a = box({-15.0f, y, 10.0f}, 0.1f, 0.0f), b = box({-14.5f, y, 10.0f}, 0.2f, 2.0f);
+        JointId j = w.addPrismaticJoint(a, b, {-14.5f, y, 10.0f}, {1, 0, 0});
+        w.joint(j).enableLimit = true; w.joint(j).lowerLimit = -0.5f; w.joint(j).upperLimit = 0.5f;
+        w.joint(j).enableMotor = true; w.joint(j).motorSpeed = 1.0f; w.joint(j).maxMotorForce = 20.0f;
+    }
+    {  // rigid distance between two dynamic boxes
+        BodyId a = box({-10.0f, y, 10.0f}, 0.2f, 1.0f), b = box({-9.0f, y + 0.5f, 10.0f}, 0.2f, 1.0f);
+        w.addDistanceJoint(a, b, {-10.0f, y, 10.0f}, {-9.0f, y + 0.5f, 10.0f});
+    }
+    {  // spring
+        BodyId a = box({-5.0f, y + 2.0f, 10.0f}, 0.1f, 0.0f), b = box({-5.0f, y, 10.0f}, 0.2f, 1.0f);
+        w.addSpringJoint(a, b, {-5.0f, y + 2.0f, 10.0f}, {-5.0f, y, 10.0f}, 2.0f, 0.3f);
+    }
+    {  // rope
+        BodyId a = box({0.0f, y + 2.0f, 10.0f}, 0.1f, 0.0f), b = box({0.8f, y + 2.0f, 10.0f}, 0.2f, 1.0f);
+        w.addRopeJoint(a, b, {0.0f, y + 2.0f, 10.0f}, {0.8f, y + 2.0f, 10.0f}, 1.5f);
+    }
+    {  // fixed
+        BodyId a = box({5.0f, y, 10.0f}, 0.2f, 1.0f), b = box({5.4f, y, 10.0f}, 0.2f, 1.0f);
+        w.addFixedJoint(a, b, {5.2f, y, 10.0f});
+    }
+    {  // weld (rigid)
+        BodyId a = box({10.0f, y, 10.0f}, 0.2f, 1.0f), b = box({10.4f, y + 0.1f, 10.0f}, 0.2f, 1.0f);
+        w.addWeldJoint(a, b, {10.2f, y, 10.0f});
+    }
+    {  // motor
+        BodyId a = box({15.0f, y, 10.0f}, 0.1f, 0.0f), b = box({15.5f, y, 10.0f}, 0.2f, 1.0f);
+        w.addMotorJoint(a, b, {15.0f, y, 10.0f}, {15.5f, y, 10.0f}, 40.0f, 20.0f);
+    }
+    {  // pulley
+        BodyId a = box({-20.0f, y, 20.0f}, 0.2f, 1.0f), b = box({-18.0f, y, 20.0f}, 0.2f, 2.0f);
+        w.addPulleyJoint(a, b, {-20.0f, y, 20.0f}, {-18.0f, y, 20.0f}, {-20.0f, y + 3.0f, 20.0f},
+                         {-18.0f, y + 3.0f, 20.0f}, 1.0f);
+    }
+    {  // gear
+        BodyId a = box({-10.0f, y, 20.0f}, 0.2f, 1.0f), b = box({-9.0f, y, 20.0f}, 0.2f, 1.0f);
+        w.body(a).angularVelocity = {0, 0, 3.0f};
+        w.addGearJoint(a, b, {0, 0, 1}, {0, 0, 1}, 2.0f);
+    }
+    {  // wheel (no steering)
+        BodyId a = box({0.0f, y, 20.0f}, 0.3f, 5.0f), b = box({0.0f, y - 0.6f, 20.0f}, 0.2f, 1.0f);
+        w.addWheelJoint(a, b, {0.0f, y - 0.6f, 20.0f}, {0, 1, 0});
+    }
+    {  // cone-twist without limits
+        BodyId a = box({10.0f, y, 20.0f}, 0.1f, 0.0f), b = box({10.5f, y, 20.0f}, 0.2f, 1.0f);
+        w.addConeTwistJoint(a, b, {10.0f, y, 20.0f}, {1, 0, 0});
+    }
+}
+
+void jointTrajectory(const std::string& out, const std::string& name, const std::vector<int>& steps) {
+    std::string dir = out + "/" + name;
+    mkdir(dir.c_str(), 0755);
+    World w(BackendType::Cpu);
+    std::vector<BodyId> ids;
+    jointScene(w, ids);
+    w.backend_.reset(new CudaSemantics());
+    const float dt = 1.0f / 60.0f;
+    writeRecords(dir + "/bodies_in.npy", w.bodies_);
+    writeRecords(dir + "/joints_in.npy", w.joints_);
+    writeSoup(dir, w.meshes_);
+    std::string countJson = ", \"joints_at\": [";
+    int done = 0;
+    for (int target : steps) {
+        while (done < target) { w.step(dt); ++done; }
+        writeRecords(dir + "/state_" + std::to_string(target) + ".npy", states(w.bodies_));
+        writeRecords(dir + "/events_" + std::to_string(target) + ".npy", eventRecords(w));
+        writeRecords(dir + "/joints_" + std::to_string(target) + ".npy", w.joints_);
+        countJson += (target == steps[0] ? "" : ", ") + std::to_string(w.joints_.size());
+        std::fprintf(stderr, "  %s step %d: %zu joints, %zu contacts, device substeps %d\n", name.c_str(), target,
+                     w.joints_.size(), w.contacts_.size(), (int)w.lastStepStats_.deviceSubsteps);
+    }
+    writeMeta(dir, w, dt, steps, countJson + "]");
 }
 
 // Reference narrow phase on snapshots of a reference trajectory: records,
@@ -604,6 +773,10 @@ int fixture(const std::string& out, const std::string& name) {
         vxscene::Scene s = vxscene::lattice();
         for (vxscene::BodySpec& b : s.bodies) b.enableSleep = 1;
         trajectory(out, name, s, {150, 200, 250, 300}, false, true);
+        return 0;
+    }
+    if (name == "joints") {  // 76 joints of every arithmetic-only type, two breakable
+        jointTrajectory(out, "joints", {1, 2, 5, 10, 30, 60});
         return 0;
     }
     if (name == "hightoi") {  // MotionQuality::High bodies through processHighToi

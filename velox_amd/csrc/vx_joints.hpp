@@ -1,0 +1,122 @@
+// Joints on the device, as the reference's CUDA backend runs them inside its
+// device substeps (backend_cuda.cu:691-761, 800-891) and as World::stepImpl
+// treats them around it (world.cpp:4045-4066, 3765-3783):
+//
+//   kJointPairsOff    contacts of joint-connected pairs (collideConnected off)
+//                     are dropped, order kept, and their pairs leave the CCD net
+//   kJointWake        a joint between a sleeping and a waking body wakes both,
+//                     in joint order (one lane)
+//   kJointReset       accumulated impulses cleared each substep (broken joints
+//                     kept after the first substep)
+//   kJointColor       the joints of one colour, one lane each (colours never
+//                     share a dynamic body), 8 passes over the colours
+//   kJointMarkBroken  reaction force / torque over the substep against the
+//                     joint's thresholds
+//   kJointFinish      World::finishBrokenJoints: broken joints removed from the
+//                     end, each swapped with the last, with a break event
+#pragma once
+#include "vxd_joint.hpp"
+#include "vx_kernels.hpp"
+
+namespace vxk {
+
+// One joint break (World::jointBreakEvents_, world.cpp:3776-3780): the joint's
+// dense index when it was removed, its bodies, reaction force and torque.
+struct JointBreak { uint32_t index, a, b; float force, torque; };
+
+__global__ __launch_bounds__(kThreads) void kJointContactsOff(const vx_contact* __restrict__ cs, uint32_t m,
+                                                              const uint64_t* __restrict__ excluded, uint32_t ne,
+                                                              uint32_t* __restrict__ keep) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= m) return;
+    const uint32_t lo = cs[k].a < cs[k].b ? cs[k].a : cs[k].b, hi = cs[k].a < cs[k].b ? cs[k].b : cs[k].a;
+    const uint64_t key = (uint64_t)lo << 32 | hi;
+    const uint32_t at = lowerBound64(excluded, ne, key);
+    keep[k] = at < ne && excluded[at] == key ? 0u : 1u;
+}
+
+__global__ __launch_bounds__(kThreads) void kJointContactsCompact(const vx_contact* __restrict__ in, uint32_t m,
+                                                                  const uint32_t* __restrict__ keep,
+                                                                  const uint32_t* __restrict__ pos,
+                                                                  vx_contact* __restrict__ out) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m && keep[k]) out[pos[k]] = in[k];
+}
+
+// The CCD net's pair list is the contacts' pairs; an excluded pair leaves it.
+__global__ __launch_bounds__(kThreads) void kJointPairsOff(const uint64_t* __restrict__ pairs, uint32_t np, int nb,
+                                                           const uint64_t* __restrict__ excluded, uint32_t ne,
+                                                           uint32_t* __restrict__ counts) {
+    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
+    if (p >= np || !counts[p]) return;
+    const uint32_t i = (uint32_t)(pairs[p] >> nb), j = (uint32_t)(pairs[p] & ((1ull << nb) - 1));
+    const uint32_t lo = i < j ? i : j, hi = i < j ? j : i;
+    const uint64_t key = (uint64_t)lo << 32 | hi;
+    const uint32_t at = lowerBound64(excluded, ne, key);
+    if (at < ne && excluded[at] == key) counts[p] = 0;
+}
+
+__global__ void kJointWake(vx_body* bodies, const vx_joint* joints, uint32_t nj) {
+    if (threadIdx.x || blockIdx.x) return;
+    for (uint32_t k = 0; k < nj; ++k) {
+        const uint32_t a = joints[k].a, b = joints[k].b;
+        if (bodies[a].asleep == bodies[b].asleep) continue;
+        bodies[a].asleep = bodies[b].asleep = 0;
+        bodies[a].sleepTimer = bodies[b].sleepTimer = 0.0f;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void kJointReset(vx_joint* joints, uint32_t nj, int keepBroken) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= nj || (keepBroken && joints[k].broken)) return;
+    jointReset(joints[k]);
+}
+
+__global__ __launch_bounds__(kThreads) void kJointColor(vx_body* bodies, vx_joint* joints,
+                                                        const uint32_t* __restrict__ order, uint32_t first,
+                                                        uint32_t count, float h) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    vx_joint& j = joints[order[first + k]];
+    if (j.broken) return;
+    vx_body a = bodies[j.a], b = bodies[j.b];
+    if (asleepFully(a) && asleepFully(b)) return;
+    vx_joint jj = j;
+    solveJoint(jj, a, b, h);
+    j = jj;
+    bodies[j.a].velocity = a.velocity; bodies[j.a].angularVelocity = a.angularVelocity;
+    bodies[j.b].velocity = b.velocity; bodies[j.b].angularVelocity = b.angularVelocity;
+}
+
+__global__ __launch_bounds__(kThreads) void kJointMarkBroken(vx_joint* joints, uint32_t nj, float h) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= nj || joints[k].broken) return;
+    vx_joint& j = joints[k];
+    const float force = length(v3(j.reactionLinearImpulse)) / h;
+    const float torque = length(v3(j.reactionAngularImpulse)) / h;
+    j.broken = force > j.breakForce || torque > j.breakTorque;
+}
+
+__global__ void kJointFinish(vx_joint* joints, uint32_t nj, float h, JointBreak* breaks, uint32_t* remaining,
+                             uint32_t* breakCount) {
+    if (threadIdx.x || blockIdx.x) return;
+    for (uint32_t k = 0; k < nj; ++k) {
+        vx_joint& j = joints[k];
+        const float force = length(v3(j.reactionLinearImpulse)) / h;
+        const float torque = length(v3(j.reactionAngularImpulse)) / h;
+        j.broken = j.broken || force > j.breakForce || torque > j.breakTorque;
+    }
+    uint32_t count = nj, nb = 0;
+    for (uint32_t i = nj; i-- > 0;) {
+        if (!joints[i].broken) continue;
+        const vx_joint& j = joints[i];
+        breaks[nb++] = JointBreak{i, j.a, j.b, length(v3(j.reactionLinearImpulse)) / h,
+                                  length(v3(j.reactionAngularImpulse)) / h};
+        if (i != count - 1) joints[i] = joints[count - 1];
+        --count;
+    }
+    *remaining = count;
+    *breakCount = nb;
+}
+
+}  // namespace vxk

@@ -14,8 +14,8 @@
 //
 // The island partition does not depend on the order in which unions happen,
 // nor do the minimum timers; every per-body quantity is computed with the
-// reference's float expressions in the same order.  Joints are not part of
-// the device step, so islands are formed from contacts alone.
+// reference's float expressions in the same order.  Islands join over
+// contacts and joints (kSleepJoints, sleep.cpp:92-95).
 #pragma once
 #include "vx_solve.hpp"
 
@@ -83,6 +83,17 @@ __global__ __launch_bounds__(kThreads) void kSleepContacts(const vx_body* __rest
     const uint32_t f = kSlHasContact | (rel > sp.stabilityThreshold ? kSlUnstable : 0u);
     if (dynA) atomicOr(&flags[c.a], f);
     if (dynB) atomicOr(&flags[c.b], f);
+}
+
+// Joints join islands too (sleep.cpp:92-95).
+__global__ __launch_bounds__(kThreads) void kSleepJoints(const vx_body* __restrict__ bodies, uint32_t n,
+                                                         const vx_joint* __restrict__ joints, uint32_t nj,
+                                                         uint32_t* parent) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= nj) return;
+    const uint32_t a = joints[k].a, b = joints[k].b;
+    if (a >= n || b >= n) return;
+    if (isDynamic(bodies[a]) && isDynamic(bodies[b])) ufUnite(parent, a, b);
 }
 
 __global__ __launch_bounds__(kThreads) void kSleepBodies(vx_body* __restrict__ bodies, uint32_t n, SleepParams sp,

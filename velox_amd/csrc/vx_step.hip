@@ -11,6 +11,7 @@
 #include "vx_solve.hpp"
 #include "vx_sleep.hpp"
 #include "vx_toi.hpp"
+#include "vx_joints.hpp"
 #include "vx_internal.hpp"
 #include "../../include/vxhip.h"
 
@@ -173,6 +174,18 @@ struct vxhip_ctx {
     uint32_t tailColor = 0;
     Buf<uint64_t> tailKeys, tailKeysTmp;
     Buf<uint32_t> tailVals, tailValsTmp, tailHead, tailPos, tailRowSlot, tailSlotBody;
+    // Joints (solved on the device when there are at least 64, like the
+    // reference's CUDA backend): records in the caller's order, the colour
+    // order of their solve, and the pairs they keep from colliding.
+    Buf<vx_joint> joints;
+    uint32_t jointCount = 0, jointColors = 0, jointExcludedCount = 0, jointBreakCount = 0;
+    std::vector<vx_joint> hostJoints;
+    std::vector<uint8_t> hostDynamic;
+    std::vector<uint32_t> jointColorStartHost;
+    Buf<uint32_t> jointOrder, jointKeep, jointKeepPos;
+    Buf<uint64_t> jointExcluded;
+    Buf<vx_contact> contactsTmp;
+    Buf<vxk::JointBreak> jointBreaks;
     // Multi-TOI scheduler for MotionQuality::High bodies.
     bool anyHigh = false, toiProcessed = false;
     vxk::ToiSettings toiSettings{1u, 1u, 4u, 256u, 0.1f, 1e-3f};  // ccd.h:111-122 defaults
@@ -214,7 +227,7 @@ enum Counter : uint32_t {
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
-    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cCount = 54
+    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55, cCount = 56
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -357,7 +370,9 @@ void setBodies(vxhip_ctx* c, const vx_body* b, uint32_t n) {
     upload(c, c->bodies, b, n);
     c->anyAsleep = false;
     c->anyHigh = false;
+    c->hostDynamic.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
+        c->hostDynamic[i] = b[i].motionType == VX_MOTION_DYNAMIC;
         c->anyAsleep |= b[i].asleep == 1;
         c->anyHigh |= b[i].motionType == VX_MOTION_DYNAMIC && b[i].ccdTuning.quality == VX_QUALITY_HIGH &&
                       b[i].ccdTuning.enableContinuous;
@@ -781,6 +796,101 @@ void positionCorrection(vxhip_ctx* c, const vx_contact* solve, uint32_t K, float
     HIPCK(hipGetLastError());
 }
 
+// prepareJoints (backend_cuda.cu:800-843): greedy colours in joint order
+// (first free bit of the two dynamic endpoints' 64-bit masks, else past both
+// bodies' last colour), a stable colour order, and the sorted keys of the
+// pairs the joints keep from colliding (world.cpp:4045-4066).  Host work on
+// the joint list, as in the reference backend; redone when joints break.
+void prepareJoints(vxhip_ctx* c) {
+    const std::vector<vx_joint>& js = c->hostJoints;
+    const uint32_t nj = (uint32_t)js.size();
+    c->jointCount = nj;
+    c->jointColors = 0;
+    c->jointExcludedCount = 0;
+    if (!nj) return;
+    std::vector<uint64_t> mask(c->n, 0);
+    std::vector<int> next(c->n, 64), colorOf(nj);
+    int numColors = 0;
+    for (uint32_t k = 0; k < nj; ++k) {
+        const bool sa = !c->hostDynamic[js[k].a], sb = !c->hostDynamic[js[k].b];
+        const uint64_t used = (sa ? 0 : mask[js[k].a]) | (sb ? 0 : mask[js[k].b]);
+        int color;
+        if (~used) {
+            color = 0;
+            while (used & (1ull << color)) ++color;
+        } else {
+            const int na = sa ? 64 : next[js[k].a], nb = sb ? 64 : next[js[k].b];
+            color = na > nb ? na : nb;
+            if (!sa) next[js[k].a] = color + 1;
+            if (!sb) next[js[k].b] = color + 1;
+        }
+        if (color < 64) {
+            if (!sa) mask[js[k].a] |= 1ull << color;
+            if (!sb) mask[js[k].b] |= 1ull << color;
+        }
+        colorOf[k] = color;
+        if (color + 1 > numColors) numColors = color + 1;
+    }
+    c->jointColors = (uint32_t)numColors;
+    c->jointColorStartHost.assign(numColors + 1, 0);
+    for (int col : colorOf) ++c->jointColorStartHost[col + 1];
+    for (int k = 0; k < numColors; ++k) c->jointColorStartHost[k + 1] += c->jointColorStartHost[k];
+    std::vector<uint32_t> fill(c->jointColorStartHost.begin(), c->jointColorStartHost.end() - 1), order(nj);
+    for (uint32_t k = 0; k < nj; ++k) order[fill[colorOf[k]]++] = k;
+    std::vector<uint64_t> excluded;
+    for (const vx_joint& j : js)
+        if (!j.collideConnected) {
+            const uint32_t lo = j.a < j.b ? j.a : j.b, hi = j.a < j.b ? j.b : j.a;
+            excluded.push_back((uint64_t)lo << 32 | hi);
+        }
+    std::sort(excluded.begin(), excluded.end());
+    c->jointExcludedCount = (uint32_t)excluded.size();
+    upload(c, c->joints, js.data(), nj);
+    upload(c, c->jointOrder, order.data(), nj);
+    if (!excluded.empty()) upload(c, c->jointExcluded, excluded.data(), (uint32_t)excluded.size());
+    HIPCK(hipStreamSynchronize(c->stream));
+}
+
+// Drops the contacts of joint-connected pairs (order kept) and takes their
+// pairs out of the CCD net; returns the new contact count.
+uint32_t jointContactsOff(vxhip_ctx* c, uint32_t np, uint32_t m) {
+    if (!c->jointExcludedCount) return m;
+    hipStream_t s = c->stream;
+    const int nb = bodyBits(c->n);
+    if (np) kJointPairsOff<<<blocks(np), kThreads, 0, s>>>(c->pairs.p, np, nb, c->jointExcluded.p,
+                                                            c->jointExcludedCount, c->counts.p);
+    if (!m) return m;
+    c->jointKeep.reserve(m); c->jointKeepPos.reserve(m); c->contactsTmp.reserve(m);
+    kJointContactsOff<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->jointExcluded.p, c->jointExcludedCount,
+                                                      c->jointKeep.p);
+    HIPCK(vxhost::exclusiveScan(c->jointKeep.p, c->jointKeepPos.p, m, c->counters.p + cContacts,
+                                tempFor(c, vxhost::scanTempBytes(m)), s));
+    kJointContactsCompact<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->jointKeep.p, c->jointKeepPos.p,
+                                                          c->contactsTmp.p);
+    HIPCK(hipGetLastError());
+    readCounters(c);
+    std::swap(c->contacts.p, c->contactsTmp.p);
+    std::swap(c->contacts.cap, c->contactsTmp.cap);
+    c->contactCount = c->hostCounters[cContacts];
+    return c->contactCount;
+}
+
+// The joints of one substep (backend_cuda.cu:738-745): reset, 8 passes over
+// the colours, break marking.
+void substepJoints(vxhip_ctx* c, float h, bool first) {
+    const uint32_t nj = c->jointCount;
+    if (!nj) return;
+    hipStream_t s = c->stream;
+    kJointReset<<<blocks(nj), kThreads, 0, s>>>(c->joints.p, nj, first ? 0 : 1);
+    for (int it = 0; it < vxd::kJointIterations; ++it)
+        for (uint32_t col = 0; col < c->jointColors; ++col) {
+            const uint32_t first0 = c->jointColorStartHost[col], cnt = c->jointColorStartHost[col + 1] - first0;
+            if (cnt) kJointColor<<<blocks(cnt), kThreads, 0, s>>>(c->bodies.p, c->joints.p, c->jointOrder.p, first0, cnt, h);
+        }
+    kJointMarkBroken<<<blocks(nj), kThreads, 0, s>>>(c->joints.p, nj, h);
+    HIPCK(hipGetLastError());
+}
+
 __global__ void kToiInit(vxk::ToiState* st, float dt) {
     if (threadIdx.x || blockIdx.x) return;
     st->remaining = dt;
@@ -970,6 +1080,8 @@ void sleepUpdate(vxhip_ctx* c, uint32_t m, float dt) {
     if (m)
         kSleepContacts<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, n, c->solve.p, m, c->sleepParams, c->slParent.p,
                                                       c->slFlags.p);
+    if (c->jointCount)
+        kSleepJoints<<<blocks(c->jointCount), kThreads, 0, s>>>(c->bodies.p, n, c->joints.p, c->jointCount, c->slParent.p);
     kSleepBodies<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->sleepParams, c->slParent.p, c->slFlags.p,
                                                 c->slStability.p, c->slIslandMin.p, k + cSlWoke);
     kSleepTransitions<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->sleepParams, c->slParent.p, c->slIslandMin.p,
@@ -1060,7 +1172,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     markStage(c, sBroad);
     const uint32_t np = broadPhase(c, dt);
     markStage(c, sNarrow);
-    const uint32_t m = narrowPhase(c, np, dt);
+    const uint32_t m = jointContactsOff(c, np, narrowPhase(c, np, dt));
     markStage(c, sMatch);
     if (c->anyAsleep && m) kWakePass<<<1, kWakeThreads, 0, s>>>(c->bodies.p, c->contacts.p, m);
     if (m && c->prevCount) kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
@@ -1071,14 +1183,20 @@ void worldStep(vxhip_ctx* c, float dt) {
     if (m) kGather<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, c->order.p, m, c->solve.p);
     c->solveCount = m;
     markStage(c, sSolve);
+    if (c->jointCount) kJointWake<<<1, 1, 0, s>>>(c->bodies.p, c->joints.p, c->jointCount);  // backend_cuda.cu:704-708
     for (int st = 0; st < S; ++st) {
         if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
         if (m) substepSolve(c, c->solve.p, K, h, cfg.options, st == 0, st == S - 1);
+        substepJoints(c, h, st == 0);
         kTransforms<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, h);
     }
     HIPCK(hipGetLastError());
     markStage(c, sCcd);
     kResetCounters<<<1, 64, 0, s>>>(k);
+    if (c->jointCount) {  // World::finishBrokenJoints (world.cpp:3765-3783)
+        c->jointBreaks.reserve(c->jointCount);
+        kJointFinish<<<1, 1, 0, s>>>(c->joints.p, c->jointCount, h, c->jointBreaks.p, k + cJointsLeft, k + cJointBreaks);
+    }
     highToi(c, m, dt);
     ccdNet(c, np, dt);
     markStage(c, sPosition);
@@ -1109,6 +1227,19 @@ void worldStep(vxhip_ctx* c, float dt) {
     if (carry) finishCache(c, m);
     else c->prevCount = m;
     if (c->sleepOn) finishSleep(c);
+    c->jointBreakCount = 0;
+    if (c->jointCount) {
+        c->jointBreakCount = c->hostCounters[cJointBreaks];
+        const uint32_t left = c->hostCounters[cJointsLeft];
+        if (left != c->jointCount) {  // joints broke: the host list follows, colours are redone
+            c->hostJoints.resize(left);
+            if (left)
+                HIPCK(hipMemcpyAsync(c->hostJoints.data(), c->joints.p, (size_t)left * sizeof(vx_joint),
+                                     hipMemcpyDeviceToHost, c->stream));
+            HIPCK(hipStreamSynchronize(c->stream));
+            prepareJoints(c);
+        }
+    }
     if (c->hostCounters[cBatchError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: island batch exceeds its LDS capacity");
     collectProfile(c);
     c->stats.bodies = n;
@@ -1206,6 +1337,8 @@ int vxhip_world_load(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_s
         c->worldN = 0;
         c->eventCount = 0;
         c->evPrevCount = 0;
+        c->hostJoints.clear();
+        c->jointCount = c->jointColors = c->jointExcludedCount = c->jointBreakCount = 0;
         HIPCK(hipStreamSynchronize(c->stream));
         c->loaded = true;
     });
@@ -1293,6 +1426,46 @@ int vxhip_world_set_events(vxhip_ctx* c, int enable) {
         c->eventsEnabled = enable != 0;
         c->eventCount = 0;
         c->evPrevCount = 0;
+    });
+}
+
+int vxhip_world_set_joints(vxhip_ctx* c, const vx_joint* joints, uint32_t count) {
+    return guarded([&] {
+        if (!c || !c->loaded || (!joints && count)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (count && count < 64)
+            fail(VXHIP_ERR_UNAVAILABLE,
+                 "vxhip: fewer than 64 joints are solved by the host world (World::solveJoints), as with the CUDA backend");
+        for (uint32_t k = 0; k < count; ++k)
+            if (joints[k].type > VX_JOINT_GEAR || joints[k].a >= c->n || joints[k].b >= c->n)
+                fail(VXHIP_ERR_INVALID, "vxhip: joint with an unknown type or body");
+        c->hostJoints.assign(joints, joints + count);
+        prepareJoints(c);
+        c->jointBreakCount = 0;
+    });
+}
+
+int vxhip_world_download_joints(vxhip_ctx* c, vx_joint* out, uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        if (!c || !count || (!out && cap)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *count = c->jointCount;
+        if (c->jointCount > cap) fail(VXHIP_ERR_CAPACITY, "vxhip: joint buffer too small");
+        if (c->jointCount)
+            HIPCK(hipMemcpyAsync(out, c->joints.p, (size_t)c->jointCount * sizeof(vx_joint), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_world_download_joint_breaks(vxhip_ctx* c, vx_joint_break* out, uint32_t cap, uint32_t* count) {
+    static_assert(sizeof(vx_joint_break) == sizeof(vxk::JointBreak), "joint break record");
+    return guarded([&] {
+        if (!c || !count || (!out && cap)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        *count = c->jointBreakCount;
+        if (c->jointBreakCount > cap) fail(VXHIP_ERR_CAPACITY, "vxhip: break buffer too small");
+        if (c->jointBreakCount)
+            HIPCK(hipMemcpyAsync(out, c->jointBreaks.p, (size_t)c->jointBreakCount * sizeof(vx_joint_break),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
     });
 }
 

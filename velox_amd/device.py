@@ -103,6 +103,9 @@ SIGNATURES = {
     "vxhip_world_set_sleep": (_i32, [_vp, _vp]),
     "vxhip_world_shift_origin": (_i32, [_vp, _vp]),
     "vxhip_world_set_multi_toi": (_i32, [_vp, _vp]),
+    "vxhip_world_set_joints": (_i32, [_vp, _vp, _u32]),
+    "vxhip_world_download_joints": (_i32, [_vp, _vp, _u32, _vp]),
+    "vxhip_world_download_joint_breaks": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_state_size": (_i32, [_vp, _vp]),
     "vxhip_world_save_state": (_i32, [_vp, _vp, C.c_size_t]),
     "vxhip_world_restore_state": (_i32, [_vp, _vp, C.c_size_t]),
@@ -341,6 +344,35 @@ class Context:
         cfg = config if isinstance(config, SleepConfig) else SleepConfig(**kw)
         check(library().vxhip_world_set_sleep(self._h, C.byref(cfg)))
         return self
+
+    def set_joints(self, joints):
+        """Joint records (numpy array of the JOINT dtype, or raw 420-byte rows)."""
+        from .records import JOINT
+        arr = np.ascontiguousarray(joints).view(np.uint8).reshape(-1, JOINT.itemsize).copy().view(JOINT).reshape(-1)
+        check(library().vxhip_world_set_joints(self._h, ptr(arr) if len(arr) else None, len(arr)))
+        return self
+
+    def joints(self):
+        from .records import JOINT
+        count = C.c_uint32(0)
+        rc = library().vxhip_world_download_joints(self._h, None, 0, C.byref(count))
+        if rc not in (OK, ERR_CAPACITY):
+            check(rc)
+        out = np.zeros(count.value, JOINT)
+        check(library().vxhip_world_download_joints(self._h, ptr(out) if len(out) else None, len(out),
+                                                    C.byref(count)))
+        return out
+
+    def joint_breaks(self):
+        from .records import JOINT_BREAK
+        count = C.c_uint32(0)
+        rc = library().vxhip_world_download_joint_breaks(self._h, None, 0, C.byref(count))
+        if rc not in (OK, ERR_CAPACITY):
+            check(rc)
+        out = np.zeros(count.value, JOINT_BREAK)
+        check(library().vxhip_world_download_joint_breaks(self._h, ptr(out) if len(out) else None, len(out),
+                                                          C.byref(count)))
+        return out
 
     def set_multi_toi(self, **kw):
         cfg = MultiToiSettings(**kw)

@@ -117,3 +117,13 @@ class SoupArrays:
                     ptr(self.meshes), len(self.meshes), ptr(self.nodes), len(self.nodes),
                     ptr(self.trirefs), len(self.trirefs), ptr(self.hullpts), len(self.hullpts),
                     ptr(self.hullfaces), len(self.hullfaces), ptr(self.children), len(self.children))
+
+
+# vx_joint (include/vx_records.h): velox::Joint, 420 B.  Only the fields the
+# host reads by name are spelled out; the rest stay raw bytes.
+JOINT = np.dtype({"names": ["type", "a", "b", "reactionLinearImpulse", "reactionAngularImpulse", "broken"],
+                  "formats": ["u1", "<u4", "<u4", ("<f4", 3), ("<f4", 3), "u1"],
+                  "offsets": [0, 4, 8, 392, 404, 416], "itemsize": 420})
+
+# vx_joint_break (include/vxhip.h), 20 B.
+JOINT_BREAK = np.dtype([("index", "<u4"), ("a", "<u4"), ("b", "<u4"), ("force", "<f4"), ("torque", "<f4")])
