@@ -108,11 +108,17 @@ int vxhip_solve_velocities(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, vx_conta
 
 /* Backend::advanceSubsteps (backend.h:138-142): all substeps of one step on the
  * device (warm start, 2*I colored sweeps per substep, integration between
- * substeps, transforms).  *advanced = 0 when the request is declined
- * (Adaptive iteration policy), mirroring backend_cuda.cu:714-715. */
+ * substeps, joints colour by colour, transforms).  `joints` (nj records, 0
+ * for none) are solved on the device when there are at least 64; their
+ * impulses, reactions and broken flags are written back in place, and
+ * removing broken joints stays the caller's (World::finishBrokenJoints).
+ * *advanced = 0 when the request is declined: 1..63 joints or a joint type
+ * past VX_JOINT_GEAR (backend_cuda.cu:696-701), or the Adaptive iteration
+ * policy (backend_cuda.cu:714-715), which is checked after joints between an
+ * asleep and an awake body have woken both (backend_cuda.cu:703-707). */
 int vxhip_advance_substeps(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m,
-                           const float gravity[3], float h, int substeps, const vx_solver_options* options,
-                           int* advanced);
+                           vx_joint* joints, uint32_t nj, const float gravity[3], float h, int substeps,
+                           const vx_solver_options* options, int* advanced);
 
 /* Backend::fetchImpulses (backend.h:146): accumulated impulses of the last
  * solve, in the order the contacts were left. */

@@ -253,6 +253,14 @@ public:
     // break marking per substep, backend_cuda.cu:322-348).
     bool advanceSubsteps(std::vector<Body>& bodies, std::vector<Contact>& cs, std::vector<Joint>& joints,
                          const Vec3& g, float h, int S, const SolverOptions& o) override {
+        if (capture && capture->armed) {
+            capture->bodiesIn = bodies;
+            capture->contactsIn = cs;
+            capture->jointsIn = joints;
+            capture->gravity = g;
+            capture->h = h;
+            capture->substeps = S;
+        }
         if (!joints.empty() && joints.size() < 64) return false;
         for (const Joint& j : joints)
             if (!joint_solver::supported(j)) return false;
@@ -320,8 +328,25 @@ public:
                 if (!(b.isStatic() || b.isLocked() || isFullyAsleep(b.asleep))) b.advanceTransform(h);
         }
         iters_ = 2 * o.velocityIterations * S;
+        if (capture && capture->armed) {
+            capture->armed = false;
+            capture->bodiesOut = bodies;
+            capture->contactsOut = cs;
+            capture->jointsOut = joints;
+        }
         return true;
     }
+    // One advanceSubsteps call's arguments and results, recorded when armed.
+    struct Capture {
+        bool armed = false;
+        std::vector<Body> bodiesIn, bodiesOut;
+        std::vector<Contact> contactsIn, contactsOut;
+        std::vector<Joint> jointsIn, jointsOut;
+        Vec3 gravity;
+        float h = 0.0f;
+        int substeps = 0;
+    };
+    Capture* capture = nullptr;
     int colors = 0;
 
 private:
@@ -544,12 +569,69 @@ void jointScene(World& w, std::vector<BodyId>& ids) {
     }
 }
 
+// The joint types whose solve calls atan2f / acosf: hinge with limits,
+// cone-twist with swing and twist limits, six-degree-of-freedom with limits
+// and motors, soft weld, wheel with steering (plus the 64 chain joints that
+// put the set on the device path).
+void jointTrigScene(World& w, std::vector<BodyId>& ids) {
+    w.gravity = {0.0f, -9.81f, 0.0f};
+    w.substeps = 4;
+    ids.push_back(w.addStaticPlane({0, 1, 0}, 0.0f));
+    auto box = [&](Vec3 p, float half, float mass) {
+        BodyId id = w.addBox(p, {half, half, half}, mass);
+        w.body(id).enableSleep = 0;
+        ids.push_back(id);
+        return id;
+    };
+    for (int c = 0; c < 8; ++c) {
+        const float x0 = -20.0f + 5.0f * (float)c;
+        BodyId prev = box({x0, 12.0f, 0.0f}, 0.1f, 0.0f);
+        for (int k = 0; k < 8; ++k) {
+            BodyId cur = box({x0 + 0.5f * (float)(k + 1), 12.0f, 0.0f}, 0.2f, 1.0f);
+            w.addBallJoint(prev, cur, {x0 + 0.5f * (float)k + 0.25f, 12.0f, 0.0f});
+            prev = cur;
+        }
+    }
+    const float y = 6.0f;
+    {  // hinge with limits
+        BodyId a = box({-20.0f, y, 10.0f}, 0.1f, 0.0f), b = box({-19.5f, y, 10.0f}, 0.2f, 2.0f);
+        JointId j = w.addHingeJoint(a, b, {-20.0f, y, 10.0f}, {0, 0, 1});
+        w.joint(j).enableLimit = true; w.joint(j).lowerLimit = -0.6f; w.joint(j).upperLimit = 0.4f;
+    }
+    {  // cone-twist with swing and twist limits
+        BodyId a = box({-15.0f, y, 10.0f}, 0.1f, 0.0f), b = box({-14.5f, y, 10.0f}, 0.2f, 2.0f);
+        w.body(b).angularVelocity = {2.0f, 0.5f, 0.0f};
+        JointId j = w.addConeTwistJoint(a, b, {-15.0f, y, 10.0f}, {1, 0, 0});
+        w.joint(j).enableSwingLimit = true; w.joint(j).swingLimit = 0.5f;
+        w.joint(j).enableTwistLimit = true; w.joint(j).lowerTwistLimit = -0.3f; w.joint(j).upperTwistLimit = 0.3f;
+    }
+    {  // six-degree-of-freedom: y free with limits, angular motor about x
+        BodyId a = box({-10.0f, y, 10.0f}, 0.1f, 0.0f), b = box({-9.5f, y, 10.0f}, 0.2f, 2.0f);
+        JointId j = w.addSixDofJoint(a, b, {-9.5f, y, 10.0f});
+        w.joint(j).lowerLinearLimit = {0.0f, -0.3f, 0.0f}; w.joint(j).upperLinearLimit = {0.0f, 0.3f, 0.0f};
+        w.joint(j).angularLimitMask = 6; w.joint(j).angularMotorMask = 1;
+        w.joint(j).angularMotorSpeed = {1.5f, 0.0f, 0.0f}; w.joint(j).maxAngularMotorTorque = {10.0f, 0.0f, 0.0f};
+    }
+    {  // soft weld
+        BodyId a = box({-5.0f, y, 10.0f}, 0.1f, 0.0f), b = box({-4.6f, y, 10.0f}, 0.2f, 1.0f);
+        JointId j = w.addWeldJoint(a, b, {-4.8f, y, 10.0f});
+        w.joint(j).weldFrequencyHz = 30.0f; w.joint(j).weldDampingRatio = 1.0f;
+    }
+    {  // wheel with steering and motor
+        BodyId a = box({0.0f, y, 10.0f}, 0.3f, 0.0f), b = box({0.0f, y - 0.6f, 10.0f}, 0.2f, 1.0f);
+        JointId j = w.addWheelJoint(a, b, {0.0f, y - 0.6f, 10.0f}, {0, 1, 0});
+        w.joint(j).enableSteering = true; w.joint(j).steeringAngle = 0.3f;
+        w.joint(j).enableMotor = true; w.joint(j).motorSpeed = 4.0f; w.joint(j).maxMotorTorque = 10.0f;
+    }
+}
+
 void jointTrajectory(const std::string& out, const std::string& name, const std::vector<int>& steps) {
     std::string dir = out + "/" + name;
     mkdir(dir.c_str(), 0755);
     World w(BackendType::Cpu);
     std::vector<BodyId> ids;
-    jointScene(w, ids);
+    if (name == "joints_trig") jointTrigScene(w, ids);
+    else jointScene(w, ids);
     w.backend_.reset(new CudaSemantics());
     const float dt = 1.0f / 60.0f;
     writeRecords(dir + "/bodies_in.npy", w.bodies_);
@@ -567,6 +649,44 @@ void jointTrajectory(const std::string& out, const std::string& name, const std:
                      w.joints_.size(), w.contacts_.size(), (int)w.lastStepStats_.deviceSubsteps);
     }
     writeMeta(dir, w, dt, steps, countJson + "]");
+}
+
+// The arguments and results of the advanceSubsteps call World::step makes
+// (world.cpp:4200-4202) in step `at` of the joint scene: the per-call device
+// substeps with contacts and joints.
+void substepsCapture(const std::string& out, const std::string& name, int at) {
+    std::string dir = out + "/" + name;
+    mkdir(dir.c_str(), 0755);
+    World w(BackendType::Cpu);
+    std::vector<BodyId> ids;
+    jointScene(w, ids);
+    auto* backend = new CudaSemantics();
+    w.backend_.reset(backend);
+    CudaSemantics::Capture cap;
+    backend->capture = &cap;
+    const float dt = 1.0f / 60.0f;
+    for (int k = 1; k < at; ++k) w.step(dt);
+    cap.armed = true;
+    w.step(dt);
+    backend->capture = nullptr;
+    writeRecords(dir + "/bodies_in.npy", cap.bodiesIn);
+    writeRecords(dir + "/contacts_in.npy", cap.contactsIn);
+    writeRecords(dir + "/joints_in.npy", cap.jointsIn);
+    writeRecords(dir + "/bodies_out.npy", cap.bodiesOut);
+    writeRecords(dir + "/contacts_out.npy", cap.contactsOut);
+    writeRecords(dir + "/joints_out.npy", cap.jointsOut);
+    char meta[512];
+    std::snprintf(meta, sizeof meta,
+                  "{\"h\": %.9g, \"substeps\": %d, \"gravity\": [%.9g, %.9g, %.9g], \"velocityIterations\": %d, "
+                  "\"frictionModel\": %d, \"contacts\": %zu, \"joints\": %zu}\n",
+                  cap.h, cap.substeps, cap.gravity.x, cap.gravity.y, cap.gravity.z,
+                  w.solverOptions_.velocityIterations, (int)w.solverOptions_.frictionModel, cap.contactsIn.size(),
+                  cap.jointsIn.size());
+    FILE* f = std::fopen((dir + "/meta.json").c_str(), "w");
+    std::fputs(meta, f);
+    std::fclose(f);
+    std::fprintf(stderr, "  %s: step %d, %zu contacts, %zu joints, %d substeps\n", name.c_str(), at,
+                 cap.contactsIn.size(), cap.jointsIn.size(), cap.substeps);
 }
 
 // Reference narrow phase on snapshots of a reference trajectory: records,
@@ -777,6 +897,8 @@ int fixture(const std::string& out, const std::string& name) {
     }
     if (name == "joints") {  // 76 joints of every arithmetic-only type, two breakable
         jointTrajectory(out, "joints", {1, 2, 5, 10, 30, 60});
+        jointTrajectory(out, "joints_trig", {1, 5, 10, 15, 20, 25, 30});
+        substepsCapture(out, "substeps_joints", 61);
         return 0;
     }
     if (name == "hightoi") {  // MotionQuality::High bodies through processHighToi

@@ -29,7 +29,8 @@ def main():
                           cwd=ROOT)
     # Sleeping on: towers fall asleep island by island, a late strike wakes one.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "settle"], cwd=ROOT)
-    # 76 joints (chains of ball joints and one of most other types), with breaking.
+    # 76 joints (chains of ball joints and one of most other types), with breaking;
+    # joints_trig: the atan2f/acosf joint types beside 64 chain joints.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "joints"], cwd=ROOT)
     # MotionQuality::High bullets and boxes through the multi-TOI scheduler.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "hightoi"], cwd=ROOT)

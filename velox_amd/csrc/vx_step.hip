@@ -388,6 +388,9 @@ constexpr uint64_t kCellTableMax = 1ull << 22;
 // workgroup instead of a launch each.
 constexpr uint32_t kNarrowColor = 2048, kNarrowSweepColor = 1024;
 constexpr uint32_t kSleepMaxBodies = 256;
+// Joint sets this large go to the device; smaller ones stay with the host
+// world, as backend_cuda.cu:696-701 decides.
+constexpr uint32_t kMinDeviceJoints = 64;
 
 uint32_t broadPhase(vxhip_ctx* c, float dt) {
     const uint32_t n = c->n;
@@ -1432,7 +1435,7 @@ int vxhip_world_set_events(vxhip_ctx* c, int enable) {
 int vxhip_world_set_joints(vxhip_ctx* c, const vx_joint* joints, uint32_t count) {
     return guarded([&] {
         if (!c || !c->loaded || (!joints && count)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
-        if (count && count < 64)
+        if (count && count < kMinDeviceJoints)
             fail(VXHIP_ERR_UNAVAILABLE,
                  "vxhip: fewer than 64 joints are solved by the host world (World::solveJoints), as with the CUDA backend");
         for (uint32_t k = 0; k < count; ++k)
@@ -1819,28 +1822,49 @@ int vxhip_solve_velocities(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact
 }
 
 int vxhip_advance_substeps(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m,
-                           const float gravity[3], float h, int substeps, const vx_solver_options* options,
-                           int* advanced) {
+                           vx_joint* joints, uint32_t nj, const float gravity[3], float h, int substeps,
+                           const vx_solver_options* options, int* advanced) {
     return guarded([&] {
         injectFailureForTesting();
-        if (!c || !options || !advanced || !gravity || (!bodies && n) || (!contacts && m))
+        if (!c || !options || !advanced || !gravity || (!bodies && n) || (!contacts && m) || (!joints && nj))
             fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
         *advanced = 0;
+        // Fewer than 64 joints or an unknown type stay on the host (backend_cuda.cu:696-701).
+        if (nj && nj < kMinDeviceJoints) return;
+        for (uint32_t k = 0; k < nj; ++k) {
+            if (joints[k].type > VX_JOINT_GEAR) return;
+            if (joints[k].a >= n || joints[k].b >= n) fail(VXHIP_ERR_INVALID, "vxhip: joint body index out of range");
+        }
+        // A joint between an asleep and an awake body wakes both, in joint
+        // order, before the policy check (backend_cuda.cu:703-707).
+        for (uint32_t k = 0; k < nj; ++k) {
+            vx_body& a = bodies[joints[k].a];
+            vx_body& b = bodies[joints[k].b];
+            if (a.asleep == b.asleep) continue;
+            a.asleep = b.asleep = 0;
+            a.sleepTimer = b.sleepTimer = 0.0f;
+        }
         if (options->iterationPolicy == VX_ITERATION_ADAPTIVE) return;  // backend_cuda.cu:714-715
         setBodies(c, bodies, n);
         const V3 g{gravity[0], gravity[1], gravity[2]};
         if (m) prepareSolve(c, contacts, m);
         else c->solveCount = 0;
+        // The joints of this call replace any a loaded world holds; the
+        // context's body buffer is the call's too.
+        c->hostJoints.assign(joints, joints + nj);
+        prepareJoints(c);
         const int S = substeps > 0 ? substeps : 1;
         for (int s = 0; s < S; ++s) {
             if (s > 0) kIntegrate<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, g, h);
             if (m) substepSolve(c, c->solve.p, c->numColors, h, *options, s == 0, s == S - 1);
+            substepJoints(c, h, s == 0);
             kTransforms<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, h);
         }
         HIPCK(hipGetLastError());
         c->lastIterations = (uint32_t)(2 * options->velocityIterations * S);
         HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
         if (m) HIPCK(hipMemcpyAsync(contacts, c->solve.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        if (nj) HIPCK(hipMemcpyAsync(joints, c->joints.p, (size_t)nj * sizeof(vx_joint), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         *advanced = 1;
     });

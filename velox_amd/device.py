@@ -90,7 +90,7 @@ SIGNATURES = {
     "vxhip_integrate": (_i32, [_vp, _vp, _u32, _vp, _f32]),
     "vxhip_find_contacts": (_i32, [_vp, _vp, _u32, _vp, _f32, _vp, _u32, _vp]),
     "vxhip_solve_velocities": (_i32, [_vp, _vp, _u32, _vp, _u32, _f32, _i32, _vp]),
-    "vxhip_advance_substeps": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _f32, _i32, _vp, _vp]),
+    "vxhip_advance_substeps": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _u32, _vp, _f32, _i32, _vp, _vp]),
     "vxhip_fetch_impulses": (_i32, [_vp, _vp, _u32]),
     "vxhip_invalidate_caches": (_i32, [_vp]),
     "vxhip_last_velocity_iterations": (_u32, [_vp]),
@@ -458,12 +458,19 @@ class Context:
         check(library().vxhip_solve_velocities(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts),
                                                float(dt), int(warm_start), C.byref(opts)))
 
-    def advance_substeps(self, bodies, contacts, gravity, h, substeps, options=None):
+    def advance_substeps(self, bodies, contacts, gravity, h, substeps, options=None, joints=None):
+        """All substeps of one step in place on `bodies`, `contacts` (left in
+        colour order) and `joints` (a JOINT array, 64 or more to run here);
+        False when declined."""
         opts = options or default_options()
         g = (C.c_float * 3)(*gravity)
         adv = C.c_int(0)
-        check(library().vxhip_advance_substeps(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts), g,
-                                               float(h), int(substeps), C.byref(opts), C.byref(adv)))
+        nj = 0 if joints is None else len(joints)
+        if nj and (not joints.flags.c_contiguous or joints.nbytes != nj * 420):
+            raise ValueError("joints must be a C-contiguous array of 420-byte joint records")
+        check(library().vxhip_advance_substeps(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts),
+                                               ptr(joints) if nj else None, nj, g, float(h), int(substeps),
+                                               C.byref(opts), C.byref(adv)))
         return bool(adv.value)
 
     def fetch_impulses(self, contacts):

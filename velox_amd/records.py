@@ -120,10 +120,12 @@ class SoupArrays:
 
 
 # vx_joint (include/vx_records.h): velox::Joint, 420 B.  Only the fields the
-# host reads by name are spelled out; the rest stay raw bytes.
-JOINT = np.dtype({"names": ["type", "a", "b", "reactionLinearImpulse", "reactionAngularImpulse", "broken"],
-                  "formats": ["u1", "<u4", "<u4", ("<f4", 3), ("<f4", 3), "u1"],
-                  "offsets": [0, 4, 8, 392, 404, 416], "itemsize": 420})
+# host reads by name are spelled out; the rest are named raw-byte fields, so
+# that numpy copies (which skip unnamed bytes) keep the whole record.
+JOINT = np.dtype([("type", "u1"), ("_pad1", "V3"), ("a", "<u4"), ("b", "<u4"), ("_config", "V380"),
+                  ("reactionLinearImpulse", "<f4", 3), ("reactionAngularImpulse", "<f4", 3), ("broken", "u1"),
+                  ("_pad417", "V3")])
+assert JOINT.itemsize == 420 and JOINT.fields["reactionLinearImpulse"][1] == 392 and JOINT.fields["broken"][1] == 416
 
 # vx_joint_break (include/vxhip.h), 20 B.
 JOINT_BREAK = np.dtype([("index", "<u4"), ("a", "<u4"), ("b", "<u4"), ("force", "<f4"), ("torque", "<f4")])
