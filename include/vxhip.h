@@ -69,7 +69,8 @@ typedef struct {
     uint32_t largestIsland;    /* contacts in the largest island */
     uint32_t maxBatchPasses;   /* most solver passes (colour runs x chunks) one batch needs per sweep */
     uint32_t multiToiEvents;   /* StepStats::multiToiEvents (processHighToi events) */
-    uint32_t reserved;
+    uint32_t ccdFlagged;       /* pairs whose end-of-step gap the net flagged */
+    uint32_t ccdRelevant;      /* pairs the ordered net walked (flagged and what can follow them) */
 } vxhip_stats;
 
 /* ---------------------------------------------------------------- setup -- */

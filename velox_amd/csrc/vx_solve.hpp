@@ -322,9 +322,8 @@ __device__ __attribute__((always_inline)) inline void warmRow(SolveArrays sa, ui
 // One colour of the big-island contacts: solve positions
 // base + colorStart[c] .. base + colorStart[c+1].
 template <int kMode>  // 0 = two-axis sweep, 1 = cone-block sweep, 2 = warm start
-__global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, uint32_t base,
-                                                           const uint32_t* __restrict__ colorStart, uint32_t color) {
-    const uint32_t first = colorStart[color], count = colorStart[color + 1] - first;
+__global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, uint32_t base, uint32_t first,
+                                                           uint32_t count) {
     const uint32_t stride = gridDim.x * kThreads;
     for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < count; j += stride) {
         const uint32_t k = base + first + j;
@@ -363,8 +362,11 @@ __device__ __attribute__((always_inline)) inline void prefetchRow(SolveArrays sa
     P.b = meta.y;
     P.R.flags = meta.z;
     for (int q = 0; q < 8; ++q) P.R.r[q] = sa.rows[8 * (size_t)k + q];
-    P.fa = sa.frame[P.a]; P.ia = sa.invI[P.a];
-    P.fb = sa.frame[P.b]; P.ib = sa.invI[P.b];
+    // The row's copy of its bodies' frames (written with the row each
+    // substep): every load here depends on k alone, one memory round trip.
+    const float4* sd = sa.side + 8 * (size_t)k;
+    P.fa = sd[0]; P.ia = sd[1];
+    P.fb = sd[4]; P.ib = sd[5];
     P.i0 = sa.imp[2 * k]; P.i1 = sa.imp[2 * k + 1];
 }
 
@@ -416,8 +418,12 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepColorRun(SolveArrays s
 // touches few bodies; their velocities then live in LDS for the whole run:
 // loaded once, updated in place colour after colour, written back at the end.
 // Rows index their bodies' LDS slots through rowSlot (2 per row, kNone for a
-// body the row does not write).  More slots than fit: the global-memory run.
-constexpr uint32_t kTailCap = 4096;
+// body the row does not write).  The tail is cut to kTailRows rows, so its
+// slots always fit (the global-memory run below is a guard only), and to
+// kTailColors colours, whose bounds are kept in LDS too.
+constexpr uint32_t kTailCap = 4864;     // 2 x 4864 x 16 B of velocities
+constexpr uint32_t kTailRows = kTailCap / 2;
+constexpr uint32_t kTailColors = 2047;  // + 1 bound x 4 B: 160 KB of LDS in all
 
 // (body, 2 * row + side) entries of the tail rows' written bodies; the other
 // sides sort last under `sentinel`.
@@ -475,6 +481,7 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepTailLds(SolveArrays sa
                                                                   const uint32_t* __restrict__ slotBody,
                                                                   const uint32_t* __restrict__ slotCount) {
     __shared__ float4 lv[kTailCap], la[kTailCap];
+    __shared__ uint32_t lcs[kTailColors + 1];
     const uint32_t ns = *slotCount;
     const uint32_t r0 = base + colorStart[c0];
     if (ns > kTailCap) {  // uniform: the global-memory run
@@ -490,6 +497,7 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepTailLds(SolveArrays sa
         lv[q] = sa.vel[b];
         la[q] = sa.ang[b];
     }
+    for (uint32_t q = threadIdx.x; q <= c1 - c0; q += kSweepRunThreads) lcs[q] = colorStart[c0 + q];
     RowPrefetch P;
     uint32_t k = kNone, sA = kNone, sB = kNone;
     {
@@ -502,13 +510,13 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepTailLds(SolveArrays sa
     }
     __syncthreads();
     for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t first = colorStart[c], last = colorStart[c + 1];
+        const uint32_t first = lcs[c - c0], last = lcs[c + 1 - c0];
         // The next colour's row is loaded while this one is solved.
         RowPrefetch Pn;
         uint32_t kn = kNone, sAn = kNone, sBn = kNone;
         if (c + 1 < c1) {
-            const uint32_t j = colorStart[c + 1] + threadIdx.x;
-            if (j < colorStart[c + 2]) {
+            const uint32_t j = lcs[c + 1 - c0] + threadIdx.x;
+            if (j < lcs[c + 2 - c0]) {
                 kn = base + j;
                 prefetchRow(sa, kn, Pn);
                 sAn = rowSlot[2 * (kn - r0)]; sBn = rowSlot[2 * (kn - r0) + 1];

@@ -667,13 +667,19 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
 }
 
 // The LDS tail of the big-island colours: the last colours that each hold at
-// most kNarrowSweepColor contacts, their written bodies given LDS slots.
+// most kNarrowSweepColor contacts, kTailRows rows in all, their written
+// bodies given LDS slots.
 void prepareTail(vxhip_ctx* c, uint32_t K, SolveArrays sa) {
     c->tailColor = K;
     if (!c->bigCount) return;
     const std::vector<uint32_t>& cs = c->bigColorStartHost;
-    uint32_t t = K;
-    while (t > 0 && cs[t] - cs[t - 1] <= kNarrowSweepColor) --t;
+    uint32_t t = K, tailRows = 0;
+    while (t > 0) {
+        const uint32_t cnt = cs[t] - cs[t - 1];
+        if (cnt > kNarrowSweepColor || tailRows + cnt > kTailRows || K - t >= kTailColors) break;
+        tailRows += cnt;
+        --t;
+    }
     if (t >= K || cs[K] == cs[t]) return;
     hipStream_t s = c->stream;
     const uint32_t rows = cs[K] - cs[t], n2 = 2 * rows, r0 = c->smallCount + cs[t];
@@ -733,13 +739,14 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 }
                 if (cnt > kNarrowSweepColor) {
                     solveLaunchBegin(c);
-                    kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col);
+                    kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, bs>>>(sa, c->smallCount, cs[col], cnt);
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
                     ++col;
                     continue;
                 }
                 uint32_t end = col + 1;
-                while (end < K && cs[end + 1] - cs[end] <= kNarrowSweepColor) ++end;
+                while (end < K && (kMode == 2 || end != c->tailColor) && cs[end + 1] - cs[end] <= kNarrowSweepColor)
+                    ++end;
                 solveLaunchBegin(c);
                 kSweepColorRun<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, end);
                 solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[end] - cs[col]);
@@ -1251,6 +1258,8 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->stats.colors = K;
     c->stats.ccdCandidates = c->hostCounters[cCcdExamined];
     c->stats.ccdRescued = c->hostCounters[cCcdRescued];
+    c->stats.ccdFlagged = c->hostCounters[cCcdFlagged];
+    c->stats.ccdRelevant = c->hostCounters[cCcdRelevant];
     c->stats.velocityIterations = c->lastIterations;
     c->stats.islands = m ? c->islandCount : 0;
     c->stats.islandContacts = m ? c->smallCount : 0;

@@ -48,29 +48,23 @@ struct Convex {                 // gjk.h:13-29
         }
         case CHull: {
             V3 d = rotateInv(orientation, dir);
-            uint32_t best = 0;
-            V3 bestP = v3(hullPts[0]);
-            float bestDot = dot(d, bestP);
-            // Eight points per step, loaded together (one memory round trip
-            // instead of eight), compared in index order: the first maximum
-            // wins as in the one-at-a-time scan.
-            uint32_t i = 1;
-            for (; i + 8 <= hullCount; i += 8) {
+            V3 bestP{};
+            float bestDot = 0.0f;
+            // Eight points per batch, loaded together (one memory round trip
+            // instead of eight; the last batch masked to the count), compared
+            // in index order: the first point starts the scan and the first
+            // maximum wins, as in the one-at-a-time scan.
+            for (uint32_t i = 0; i < hullCount; i += 8) {
                 V3 p[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) p[q] = v3(hullPts[i + q]);
+                for (int q = 0; q < 8; ++q) p[q] = i + q < hullCount ? v3(hullPts[i + q]) : V3{};
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
-                    float t = dot(d, p[q]);
-                    if (t > bestDot) { bestDot = t; best = i + q; bestP = p[q]; }
+                    if (i + q >= hullCount) break;
+                    const float t = dot(d, p[q]);
+                    if ((i == 0 && q == 0) || t > bestDot) { bestDot = t; bestP = p[q]; }
                 }
             }
-            for (; i < hullCount; ++i) {
-                V3 p = v3(hullPts[i]);
-                float t = dot(d, p);
-                if (t > bestDot) { bestDot = t; best = i; bestP = p; }
-            }
-            (void)best;
             return rotate(orientation, bestP);
         }
         case CCylinder: {

@@ -33,10 +33,10 @@ class Stats(C.Structure):
                 ("oversize", C.c_uint32), ("cellSize", C.c_float), ("islands", C.c_uint32),
                 ("islandContacts", C.c_uint32), ("bigIslandContacts", C.c_uint32),
                 ("largestIsland", C.c_uint32), ("maxBatchPasses", C.c_uint32), ("multiToiEvents", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("ccdFlagged", C.c_uint32), ("ccdRelevant", C.c_uint32)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class SleepConfig(C.Structure):
