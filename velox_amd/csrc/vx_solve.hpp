@@ -415,15 +415,15 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepColorRun(SolveArrays s
 }
 
 // The tail of the big-island colours (the last colours, every one narrow)
-// touches few bodies; their velocities then live in LDS for the whole run:
+// touches few bodies; their velocities then live in LDS for the whole sweep:
 // loaded once, updated in place colour after colour, written back at the end.
-// Rows index their bodies' LDS slots through rowSlot (2 per row, kNone for a
-// body the row does not write).  The tail is cut to kTailRows rows, so its
-// slots always fit (the global-memory run below is a guard only), and to
-// kTailColors colours, whose bounds are kept in LDS too.
-constexpr uint32_t kTailCap = 4864;     // 2 x 4864 x 16 B of velocities
+// Two lanes per row as in the LDS batch sweeps (kSweepTailPair).  The tail is
+// cut to kTailRows rows, so its slots always fit (the global-memory run in
+// the kernel is a guard only), and to kTailColors colours, whose bounds are
+// kept in LDS too.
+constexpr uint32_t kTailCap = 6144;     // 6 floats x 6144 slots = 144 KB of velocities
 constexpr uint32_t kTailRows = kTailCap / 2;
-constexpr uint32_t kTailColors = 2047;  // + 1 bound x 4 B: 160 KB of LDS in all
+constexpr uint32_t kTailColors = 2047;  // + 1 bound x 4 B: 152 KB of LDS in all
 
 // (body, 2 * row + side) entries of the tail rows' written bodies; the other
 // sides sort last under `sentinel`.
@@ -445,98 +445,19 @@ __global__ __launch_bounds__(kThreads) void kTailHeads(const uint64_t* __restric
     head[i] = keys[i] != sentinel && (i == 0 || keys[i - 1] != keys[i]) ? 1u : 0u;
 }
 
+// Slots are 16-bit, side A in the low half of a row's 32-bit word and side B
+// in the high half (the pair layout of the LDS sweeps); 0xffff for none.
 __global__ __launch_bounds__(kThreads) void kTailSlots(const uint64_t* __restrict__ keys,
                                                        const uint32_t* __restrict__ vals, uint32_t n, uint64_t sentinel,
                                                        const uint32_t* __restrict__ head,
                                                        const uint32_t* __restrict__ headPos,
-                                                       uint32_t* __restrict__ rowSlot, uint32_t* __restrict__ slotBody) {
+                                                       uint16_t* __restrict__ rowSlot, uint32_t* __restrict__ slotBody) {
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     if (i >= n) return;
-    if (keys[i] == sentinel) { rowSlot[vals[i]] = kNone; return; }
+    if (keys[i] == sentinel) { rowSlot[vals[i]] = 0xffffu; return; }
     const uint32_t slot = headPos[i] + head[i] - 1;
-    rowSlot[vals[i]] = slot;
+    rowSlot[vals[i]] = (uint16_t)slot;
     if (head[i]) slotBody[slot] = (uint32_t)keys[i];
-}
-
-template <bool kCone>
-__device__ __attribute__((always_inline)) inline void finishRowLds(SolveArrays sa, uint32_t k, const RowPrefetch& P,
-                                                                  uint32_t sA, uint32_t sB, float4* lv, float4* la) {
-    if (P.R.flags & kSkip) return;
-    const uint32_t ia = P.a, ib = P.b;
-    const float4 va = sA != kNone ? lv[sA] : sa.vel[ia], wa = sA != kNone ? la[sA] : sa.ang[ia];
-    const float4 vb = sB != kNone ? lv[sB] : sa.vel[ib], wb = sB != kNone ? la[sB] : sa.ang[ib];
-    V3 vA = xyz(va), wA = xyz(wa), vB = xyz(vb), wB = xyz(wb);
-    float cN = P.i0.x, cT1 = P.i0.y, cT2 = P.i0.z, cR1 = P.i0.w, cR2 = P.i1.x, cS = P.i1.y;
-    solveRegs<kCone>(P.R, P.fa, P.ia, P.fb, P.ib, vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
-    if (P.R.flags & kDynA) { lv[sA] = make_float4(vA.x, vA.y, vA.z, 0.0f); la[sA] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
-    if (P.R.flags & kDynB) { lv[sB] = make_float4(vB.x, vB.y, vB.z, 0.0f); la[sB] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
-    sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
-    sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
-}
-
-template <int kMode>  // 0 two-axis, 1 cone-block
-__global__ __launch_bounds__(kSweepRunThreads) void kSweepTailLds(SolveArrays sa, uint32_t base,
-                                                                  const uint32_t* __restrict__ colorStart, uint32_t c0,
-                                                                  uint32_t c1, const uint32_t* __restrict__ rowSlot,
-                                                                  const uint32_t* __restrict__ slotBody,
-                                                                  const uint32_t* __restrict__ slotCount) {
-    __shared__ float4 lv[kTailCap], la[kTailCap];
-    __shared__ uint32_t lcs[kTailColors + 1];
-    const uint32_t ns = *slotCount;
-    const uint32_t r0 = base + colorStart[c0];
-    if (ns > kTailCap) {  // uniform: the global-memory run
-        for (uint32_t c = c0; c < c1; ++c) {
-            const uint32_t first = colorStart[c], last = colorStart[c + 1];
-            for (uint32_t j = first + threadIdx.x; j < last; j += kSweepRunThreads) solveRow<kMode == 1>(sa, base + j);
-            __syncthreads();
-        }
-        return;
-    }
-    for (uint32_t q = threadIdx.x; q < ns; q += kSweepRunThreads) {
-        const uint32_t b = slotBody[q];
-        lv[q] = sa.vel[b];
-        la[q] = sa.ang[b];
-    }
-    for (uint32_t q = threadIdx.x; q <= c1 - c0; q += kSweepRunThreads) lcs[q] = colorStart[c0 + q];
-    RowPrefetch P;
-    uint32_t k = kNone, sA = kNone, sB = kNone;
-    {
-        const uint32_t j = colorStart[c0] + threadIdx.x;
-        if (j < colorStart[c0 + 1]) {
-            k = base + j;
-            prefetchRow(sa, k, P);
-            sA = rowSlot[2 * (k - r0)]; sB = rowSlot[2 * (k - r0) + 1];
-        }
-    }
-    __syncthreads();
-    for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t first = lcs[c - c0], last = lcs[c + 1 - c0];
-        // The next colour's row is loaded while this one is solved.
-        RowPrefetch Pn;
-        uint32_t kn = kNone, sAn = kNone, sBn = kNone;
-        if (c + 1 < c1) {
-            const uint32_t j = lcs[c + 1 - c0] + threadIdx.x;
-            if (j < lcs[c + 2 - c0]) {
-                kn = base + j;
-                prefetchRow(sa, kn, Pn);
-                sAn = rowSlot[2 * (kn - r0)]; sBn = rowSlot[2 * (kn - r0) + 1];
-            }
-        }
-        if (k != kNone) finishRowLds<kMode == 1>(sa, k, P, sA, sB, lv, la);
-        for (uint32_t j = first + threadIdx.x + kSweepRunThreads; j < last; j += kSweepRunThreads) {
-            RowPrefetch Q;
-            const uint32_t kk = base + j;
-            prefetchRow(sa, kk, Q);
-            finishRowLds<kMode == 1>(sa, kk, Q, rowSlot[2 * (kk - r0)], rowSlot[2 * (kk - r0) + 1], lv, la);
-        }
-        ldsBarrier();
-        P = Pn; k = kn; sA = sAn; sB = sBn;
-    }
-    for (uint32_t q = threadIdx.x; q < ns; q += kSweepRunThreads) {
-        const uint32_t b = slotBody[q];
-        sa.vel[b] = lv[q];
-        sa.ang[b] = la[q];
-    }
 }
 
 // --------------------------------------------------------------- islands --
@@ -834,11 +755,11 @@ struct PairItem {
     float4 i0, i1;                 // accumulated impulses
 };
 
-__device__ __attribute__((always_inline)) inline PairItem loadPairItem(SolveArrays sa, const uint32_t* slotPair,
-                                                                      uint32_t j, bool sideB) {
+__device__ __attribute__((always_inline)) inline PairItem loadPairItemSlots(SolveArrays sa, uint32_t slots, uint32_t j,
+                                                                           bool sideB) {
     PairItem it;
     it.meta = sa.meta[j];
-    it.slots = slotPair[j];
+    it.slots = slots;
     const float4* row = sa.rows + 8 * (size_t)j;
 #pragma unroll
     for (int q = 0; q < 8; ++q) it.r[q] = row[q];
@@ -849,7 +770,12 @@ __device__ __attribute__((always_inline)) inline PairItem loadPairItem(SolveArra
     return it;
 }
 
-template <int kMode>  // 0 two-axis, 1 cone-block, 2 warm start
+__device__ __attribute__((always_inline)) inline PairItem loadPairItem(SolveArrays sa, const uint32_t* slotPair,
+                                                                      uint32_t j, bool sideB) {
+    return loadPairItemSlots(sa, slotPair[j], j, sideB);
+}
+
+template <int kMode, uint32_t kStride = kLdsBodies>  // 0 two-axis, 1 cone-block, 2 warm start
 __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, uint32_t j, PairItem& it, bool sideB,
                                                               float* __restrict__ lv) {
     const uint32_t flags = it.meta.z;
@@ -862,8 +788,8 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
     const float4 fr = it.frame, ii = it.invI;
     V3 v, w;
     if (dyn) {
-        v = V3{lv[slot], lv[kLdsBodies + slot], lv[2 * kLdsBodies + slot]};
-        w = V3{lv[3 * kLdsBodies + slot], lv[4 * kLdsBodies + slot], lv[5 * kLdsBodies + slot]};
+        v = V3{lv[slot], lv[kStride + slot], lv[2 * kStride + slot]};
+        w = V3{lv[3 * kStride + slot], lv[4 * kStride + slot], lv[5 * kStride + slot]};
     } else {
         v = xyz(it.vel);
         w = xyz(it.ang);
@@ -992,8 +918,8 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
         sa.imp[2 * j + 1] = it.i1;
     }
     if (dyn) {
-        lv[slot] = v.x; lv[kLdsBodies + slot] = v.y; lv[2 * kLdsBodies + slot] = v.z;
-        lv[3 * kLdsBodies + slot] = w.x; lv[4 * kLdsBodies + slot] = w.y; lv[5 * kLdsBodies + slot] = w.z;
+        lv[slot] = v.x; lv[kStride + slot] = v.y; lv[2 * kStride + slot] = v.z;
+        lv[3 * kStride + slot] = w.x; lv[4 * kStride + slot] = w.y; lv[5 * kStride + slot] = w.z;
     }
 }
 
@@ -1068,6 +994,71 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
             sa.ang[body] = make_float4(lv[3 * kLdsBodies + i], lv[4 * kLdsBodies + i], lv[5 * kLdsBodies + i], 0.0f);
         }
         __syncthreads();
+    }
+}
+
+// One sweep of the tail colours c0..c1-1 by one workgroup, two lanes per row,
+// the rows' written bodies in LDS slots (rowSlot: the tail rows' 32-bit slot
+// words, row r0 first).  A colour wider than a pass takes several passes
+// with no barrier between them (its rows share no body); each row's data is
+// loaded one pass ahead.
+template <int kMode>  // 0 two-axis, 1 cone-block
+__global__ __launch_bounds__(kSweepRunThreads) void kSweepTailPair(SolveArrays sa, uint32_t base,
+                                                                   const uint32_t* __restrict__ colorStart, uint32_t c0,
+                                                                   uint32_t c1, const uint32_t* __restrict__ rowSlot,
+                                                                   const uint32_t* __restrict__ slotBody,
+                                                                   const uint32_t* __restrict__ slotCount) {
+    __shared__ float lv[6 * kTailCap];
+    __shared__ uint32_t lcs[kTailColors + 1];
+    const uint32_t ns = *slotCount;
+    if (ns > kTailCap || c1 - c0 > kTailColors) {  // uniform: the global-memory run
+        for (uint32_t c = c0; c < c1; ++c) {
+            const uint32_t first = colorStart[c], last = colorStart[c + 1];
+            for (uint32_t j = first + threadIdx.x; j < last; j += kSweepRunThreads) solveRow<kMode == 1>(sa, base + j);
+            __syncthreads();
+        }
+        return;
+    }
+    const uint32_t t = threadIdx.x;
+    const bool sideB = t & 1;
+    constexpr uint32_t kPass = kSweepRunThreads / 2;
+    for (uint32_t q = t; q < ns; q += kSweepRunThreads) {
+        const uint32_t b = slotBody[q];
+        const float4 v = sa.vel[b], w = sa.ang[b];
+        lv[q] = v.x; lv[kTailCap + q] = v.y; lv[2 * kTailCap + q] = v.z;
+        lv[3 * kTailCap + q] = w.x; lv[4 * kTailCap + q] = w.y; lv[5 * kTailCap + q] = w.z;
+    }
+    for (uint32_t q = t; q <= c1 - c0; q += kSweepRunThreads) lcs[q] = colorStart[c0 + q];
+    const uint32_t f0 = colorStart[c0];  // rowSlot's first row
+    // Pass cursor: colour c, rows [cb, ce) of colorStart's numbering, this
+    // lane's row j.
+    uint32_t c = c0, cb = colorStart[c0], ce = colorStart[c0 + 1];
+    uint32_t j = cb + (t >> 1);
+    PairItem cur;
+    if (j < ce) cur = loadPairItemSlots(sa, rowSlot[j - f0], base + j, sideB);
+    __syncthreads();
+    for (;;) {
+        uint32_t nc = c, nb = cb + kPass, ne = ce;
+        const bool colourDone = nb >= ce;
+        if (colourDone) {
+            nc = c + 1;
+            nb = nc < c1 ? lcs[nc - c0] : 0u;
+            ne = nc < c1 ? lcs[nc + 1 - c0] : 0u;
+        }
+        const bool more = nc < c1;
+        const uint32_t nj = nb + (t >> 1);
+        PairItem nxt;
+        if (more && nj < ne) nxt = loadPairItemSlots(sa, rowSlot[nj - f0], base + nj, sideB);
+        if (j < ce) pairRow<kMode, kTailCap>(sa, base + j, cur, sideB, lv);
+        if (colourDone) ldsBarrier();
+        if (!more) break;
+        c = nc; cb = nb; ce = ne; j = nj;
+        cur = nxt;
+    }
+    for (uint32_t q = t; q < ns; q += kSweepRunThreads) {
+        const uint32_t b = slotBody[q];
+        sa.vel[b] = make_float4(lv[q], lv[kTailCap + q], lv[2 * kTailCap + q], 0.0f);
+        sa.ang[b] = make_float4(lv[3 * kTailCap + q], lv[4 * kTailCap + q], lv[5 * kTailCap + q], 0.0f);
     }
 }
 

@@ -694,7 +694,7 @@ void prepareTail(vxhip_ctx* c, uint32_t K, SolveArrays sa) {
     HIPCK(vxhost::exclusiveScan(c->tailHead.p, c->tailPos.p, n2, c->counters.p + cTailSlots,
                                 tempFor(c, vxhost::scanTempBytes(n2)), s));
     kTailSlots<<<blocks(n2), kThreads, 0, s>>>(c->tailKeys.p, c->tailVals.p, n2, sentinel, c->tailHead.p, c->tailPos.p,
-                                               c->tailRowSlot.p, c->tailSlotBody.p);
+                                               reinterpret_cast<uint16_t*>(c->tailRowSlot.p), c->tailSlotBody.p);
     HIPCK(hipGetLastError());
     c->tailColor = t;
 }
@@ -731,9 +731,9 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 if (!cnt) { ++col; continue; }
                 if (kMode != 2 && col == c->tailColor) {
                     solveLaunchBegin(c);
-                    kSweepTailLds<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, K,
-                                                                        c->tailRowSlot.p, c->tailSlotBody.p,
-                                                                        c->counters.p + cTailSlots);
+                    kSweepTailPair<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, K,
+                                                                         c->tailRowSlot.p, c->tailSlotBody.p,
+                                                                         c->counters.p + cTailSlots);
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[K] - cs[col]);
                     break;
                 }
