@@ -1030,30 +1030,32 @@ __global__ __launch_bounds__(kSweepRunThreads) void kSweepTailPair(SolveArrays s
     }
     for (uint32_t q = t; q <= c1 - c0; q += kSweepRunThreads) lcs[q] = colorStart[c0 + q];
     const uint32_t f0 = colorStart[c0];  // rowSlot's first row
-    // Pass cursor: colour c, rows [cb, ce) of colorStart's numbering, this
-    // lane's row j.
-    uint32_t c = c0, cb = colorStart[c0], ce = colorStart[c0 + 1];
-    uint32_t j = cb + (t >> 1);
-    PairItem cur;
-    if (j < ce) cur = loadPairItemSlots(sa, rowSlot[j - f0], base + j, sideB);
+    // Pass cursors: colour c, rows [cb, ce) of colorStart's numbering; the
+    // pass after a colour's last one starts the next colour.  Rows are
+    // loaded two passes ahead.
+    struct Cursor { uint32_t c, cb, ce; };
+    auto next = [&](const Cursor& x) {
+        if (x.cb + kPass < x.ce) return Cursor{x.c, x.cb + kPass, x.ce};
+        const uint32_t nc = x.c + 1;
+        return nc < c1 ? Cursor{nc, lcs[nc - c0], lcs[nc + 1 - c0]} : Cursor{c1, 0u, 0u};
+    };
+    const uint32_t half = t >> 1;
+    Cursor A{c0, f0, colorStart[c0 + 1]};
+    PairItem itA, itB;
+    if (A.cb + half < A.ce) itA = loadPairItemSlots(sa, rowSlot[A.cb + half - f0], base + A.cb + half, sideB);
     __syncthreads();
+    Cursor B = next(A);
+    if (B.c < c1 && B.cb + half < B.ce) itB = loadPairItemSlots(sa, rowSlot[B.cb + half - f0], base + B.cb + half, sideB);
     for (;;) {
-        uint32_t nc = c, nb = cb + kPass, ne = ce;
-        const bool colourDone = nb >= ce;
-        if (colourDone) {
-            nc = c + 1;
-            nb = nc < c1 ? lcs[nc - c0] : 0u;
-            ne = nc < c1 ? lcs[nc + 1 - c0] : 0u;
-        }
-        const bool more = nc < c1;
-        const uint32_t nj = nb + (t >> 1);
-        PairItem nxt;
-        if (more && nj < ne) nxt = loadPairItemSlots(sa, rowSlot[nj - f0], base + nj, sideB);
-        if (j < ce) pairRow<kMode, kTailCap>(sa, base + j, cur, sideB, lv);
-        if (colourDone) ldsBarrier();
-        if (!more) break;
-        c = nc; cb = nb; ce = ne; j = nj;
-        cur = nxt;
+        const Cursor C = B.c < c1 ? next(B) : Cursor{c1, 0u, 0u};
+        PairItem itC;
+        if (C.c < c1 && C.cb + half < C.ce)
+            itC = loadPairItemSlots(sa, rowSlot[C.cb + half - f0], base + C.cb + half, sideB);
+        if (A.cb + half < A.ce) pairRow<kMode, kTailCap>(sa, base + A.cb + half, itA, sideB, lv);
+        if (A.cb + kPass >= A.ce) ldsBarrier();
+        if (B.c >= c1) break;
+        A = B; itA = itB;
+        B = C; itB = itC;
     }
     for (uint32_t q = t; q < ns; q += kSweepRunThreads) {
         const uint32_t b = slotBody[q];
