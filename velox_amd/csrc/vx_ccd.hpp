@@ -199,10 +199,16 @@ __global__ __launch_bounds__(kThreads) void kCcdFlagsToRel(const uint8_t* __rest
     if (p < np) rel[p] = flag[p] ? 1u : 0u;
 }
 
+constexpr int kCcdRounds = 8;
+
+// `prevAdded`: pairs the previous round added (null in the first round); a
+// round after one that added none has nothing to do.
 __global__ __launch_bounds__(kThreads) void kCcdRelMark(const vx_body* __restrict__ bodies,
                                                         const uint64_t* __restrict__ pairs,
                                                         const uint32_t* __restrict__ rel, uint32_t np, int nb,
-                                                        uint32_t* __restrict__ firstRel) {
+                                                        uint32_t* __restrict__ firstRel,
+                                                        const uint32_t* __restrict__ prevAdded) {
+    if (prevAdded && *prevAdded == 0) return;
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= np || !rel[p]) return;
     uint32_t i, j;
@@ -215,7 +221,9 @@ __global__ __launch_bounds__(kThreads) void kCcdRelGrow(const vx_body* __restric
                                                         const uint64_t* __restrict__ pairs,
                                                         const uint32_t* __restrict__ counts, uint32_t np, int nb,
                                                         const uint32_t* __restrict__ firstRel, uint32_t* __restrict__ rel,
-                                                        uint32_t* __restrict__ changed) {
+                                                        uint32_t* __restrict__ changed,
+                                                        const uint32_t* __restrict__ prevAdded) {
+    if (prevAdded && *prevAdded == 0) return;
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= np || rel[p] || !counts[p]) return;
     uint32_t i, j;
@@ -227,10 +235,12 @@ __global__ __launch_bounds__(kThreads) void kCcdRelGrow(const vx_body* __restric
     }
 }
 
-// Fallback when the closure has not settled after a few rounds: every pair
-// that produced contacts.
+// Fallback when the closure has not settled after kCcdRounds rounds (the
+// last one still added pairs): every pair that produced contacts.
 __global__ __launch_bounds__(kThreads) void kCcdRelAll(const uint32_t* __restrict__ counts, uint32_t np,
-                                                       uint32_t* __restrict__ rel) {
+                                                       uint32_t* __restrict__ rel,
+                                                       const uint32_t* __restrict__ lastAdded) {
+    if (*lastAdded == 0) return;
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p < np) rel[p] = counts[p] ? 1u : 0u;
 }

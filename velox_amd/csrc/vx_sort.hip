@@ -6,7 +6,8 @@
 //    in which every lane ranks itself among equal digits of its wavefront with
 //    eight ballots (a 64-lane match mask) and waves are chained through LDS.
 //    Stability per pass is what makes the multi-pass LSD sort correct.
-//  * Exclusive scan of u32 counts (tile sums, scan of tile sums, add-back).
+//  * Exclusive scan of u32 counts: one workgroup up to 16384 counts, else
+//    tile sums, scan of tile sums, add-back.
 #include "vx_internal.hpp"
 
 namespace vxk {
@@ -145,6 +146,56 @@ __global__ __launch_bounds__(kScanThreads) void kScanAdd(uint32_t* __restrict__ 
     }
 }
 
+// Single-workgroup exclusive scan of up to kScanOneMax counts in one launch
+// (the three-kernel scan above pays two extra launches, which dominate for
+// small arrays such as a sort pass's histogram table).  Each thread scans 16
+// consecutive counts in registers; wavefront and workgroup offsets come from
+// lane shuffles and 16 LDS words.  `out` may alias `in`: every thread reads
+// its own counts before writing them.
+constexpr int kScanOneThreads = 1024;
+constexpr int kScanOneItems = 16;
+constexpr uint32_t kScanOneMax = kScanOneThreads * kScanOneItems;
+
+__global__ __launch_bounds__(kScanOneThreads) void kScanOne(const uint32_t* in, uint32_t n, uint32_t* out,
+                                                            uint32_t* __restrict__ total) {
+    __shared__ uint32_t waveSums[kScanOneThreads / 64];
+    const uint32_t t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const uint32_t base = t * kScanOneItems;
+    uint32_t local[kScanOneItems];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanOneItems; ++k) {
+        local[k] = base + k < n ? in[base + k] : 0u;
+        sum += local[k];
+    }
+    uint32_t x = sum;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) waveSums[wave] = x;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t w = lane < kScanOneThreads / 64 ? waveSums[lane] : 0u;
+#pragma unroll
+        for (int off = 1; off < kScanOneThreads / 64; off <<= 1) {
+            const uint32_t y = __shfl_up(w, off, 64);
+            if (lane >= (uint32_t)off) w += y;
+        }
+        if (lane < kScanOneThreads / 64) waveSums[lane] = w;
+    }
+    __syncthreads();
+    const uint32_t waveOffset = wave ? waveSums[wave - 1] : 0u;
+    uint32_t run = waveOffset + x - sum;
+#pragma unroll
+    for (int k = 0; k < kScanOneItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += local[k];
+    }
+    if (t == kScanOneThreads - 1 && total) *total = waveOffset + x;
+}
+
 }  // namespace vxk
 
 namespace vxhost {
@@ -172,9 +223,13 @@ hipError_t radixSort(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t
     for (int p = 0; p < passes; ++p) {
         const int shift = 8 * p;
         kRadixHistogram<<<tiles, kSortThreads, 0, s>>>(ka, n, shift, hist, tiles);
-        kScanTiles<<<scanTiles, kScanThreads, 0, s>>>(hist, histN, hist, sums);
-        kScanSums<<<1, kScanThreads, 0, s>>>(sums, scanTiles, nullptr);
-        kScanAdd<<<scanTiles, kScanThreads, 0, s>>>(hist, histN, sums);
+        if (histN <= kScanOneMax) {
+            kScanOne<<<1, kScanOneThreads, 0, s>>>(hist, histN, hist, nullptr);
+        } else {
+            kScanTiles<<<scanTiles, kScanThreads, 0, s>>>(hist, histN, hist, sums);
+            kScanSums<<<1, kScanThreads, 0, s>>>(sums, scanTiles, nullptr);
+            kScanAdd<<<scanTiles, kScanThreads, 0, s>>>(hist, histN, sums);
+        }
         kRadixScatter<<<tiles, kSortThreads, 0, s>>>(ka, va, n, shift, hist, tiles, kb, vb);
         uint64_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
@@ -192,6 +247,10 @@ size_t scanTempBytes(uint32_t n) { return ((size_t)(n + kScanTile - 1) / kScanTi
 hipError_t exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, void* temp, hipStream_t s) {
     if (n == 0) {
         (void)hipMemsetAsync(total, 0, sizeof(uint32_t), s);
+        return hipGetLastError();
+    }
+    if (n <= kScanOneMax) {
+        kScanOne<<<1, kScanOneThreads, 0, s>>>(in, n, out, total);
         return hipGetLastError();
     }
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;

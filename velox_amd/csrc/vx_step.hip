@@ -225,9 +225,10 @@ namespace {
 enum Counter : uint32_t {
     cExtent = 0, cBounds = 1 /* 6 ints */, cOversize = 7, cPairs = 8, cContacts = 9, cEntries = 10,
     cColors = 11, cColorError = 12, cCcdFlagged = 13, cCcdFirst = 14, cCcdExamined = 15, cCcdRescued = 16,
-    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdChanged = 32, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
+    cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
-    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55, cCount = 56
+    cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55,
+    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cCount = 64
 };
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
@@ -979,17 +980,19 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     c->ccdRel.reserve(np); c->ccdRelPos.reserve(np); c->ccdList.reserve(np); c->ccdFirstRel.reserve(n);
     kCcdFlagsToRel<<<blocks(np), kThreads, 0, s>>>(c->ccdFlag.p, np, c->ccdRel.p);
     fill32(c, c->ccdFirstRel.p, n, kNone);
-    bool settled = false;
-    for (int round = 0; round < 8 && !settled; ++round) {
-        kCcdRelMark<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdRel.p, np, nb, c->ccdFirstRel.p);
-        fill32(c, k + cCcdChanged, 1, 0);
+    // Eight closure rounds queued at once: a round after one that added
+    // nothing returns at once, and the all-pairs fallback runs only when the
+    // eighth still added pairs (no host round trip per round).
+    fill32(c, k + cCcdRound, kCcdRounds, 0);
+    for (int round = 0; round < kCcdRounds; ++round) {
+        const uint32_t* prev = round ? k + cCcdRound + round - 1 : nullptr;
+        kCcdRelMark<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdRel.p, np, nb, c->ccdFirstRel.p,
+                                                    prev);
         kCcdRelGrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, c->ccdFirstRel.p,
-                                                    c->ccdRel.p, k + cCcdChanged);
-        HIPCK(hipGetLastError());
-        readCounters(c);
-        settled = c->hostCounters[cCcdChanged] == 0;
+                                                    c->ccdRel.p, k + cCcdRound + round, prev);
     }
-    if (!settled) kCcdRelAll<<<blocks(np), kThreads, 0, s>>>(c->counts.p, np, c->ccdRel.p);
+    kCcdRelAll<<<blocks(np), kThreads, 0, s>>>(c->counts.p, np, c->ccdRel.p, k + cCcdRound + kCcdRounds - 1);
+    HIPCK(hipGetLastError());
     HIPCK(vxhost::exclusiveScan(c->ccdRel.p, c->ccdRelPos.p, np, k + cCcdRelevant, tempFor(c, vxhost::scanTempBytes(np)), s));
     kCcdRelList<<<blocks(np), kThreads, 0, s>>>(c->ccdRel.p, c->ccdRelPos.p, np, c->ccdList.p);
     HIPCK(hipGetLastError());
