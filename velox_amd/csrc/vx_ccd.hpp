@@ -94,12 +94,14 @@ __device__ inline float ccdSearch(const vx_body& a, float dt) {
 __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict__ bodies, Soup soup, float dt,
                                                       const uint64_t* __restrict__ pairs,
                                                       const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                      const uint32_t* __restrict__ order,
                                                       uint8_t* __restrict__ flag, float4* __restrict__ endGap,
                                                       uint32_t* __restrict__ flagged,
                                                       uint32_t* __restrict__ firstFlagged,
                                                       const uint8_t* __restrict__ toiDone) {
-    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
-    if (p >= np) return;
+    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= np) return;
+    const uint32_t p = order[q];  // pairs grouped by shape class
     uint8_t f = 0;
     if (counts[p]) {
         uint32_t i, j;

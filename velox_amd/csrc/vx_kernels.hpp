@@ -367,11 +367,31 @@ __device__ inline bool canonicalLess(const vx_contact& x, const vx_contact& y) {
 
 // One pair per lane: collidePair, then the pair's contacts in point order
 // (stable, so equal points keep emission order) into the pair's 12 slots.
-__global__ __launch_bounds__(kThreads) void kNarrow(const vx_body* __restrict__ bodies, Soup soup, float dt,
-                                                    const uint64_t* __restrict__ pairs, uint32_t np, int nb,
-                                                    vx_contact* __restrict__ slots, uint32_t* __restrict__ counts) {
+// Shape class of every candidate pair (the lower body's shape, then the
+// other's) and the identity order.  Sorted by class, this order puts pairs of
+// the same shapes side by side in a wavefront of kNarrow and kCcdCheck, which
+// then run one shape routine instead of all of them; every result is still
+// written at the pair's own index.
+__global__ __launch_bounds__(kThreads) void kPairClass(const vx_body* __restrict__ bodies,
+                                                       const uint64_t* __restrict__ pairs, uint32_t np, int nb,
+                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= np) return;
+    const uint64_t key = pairs[p];
+    const uint32_t ca = (uint32_t)(key >> nb), cb = (uint32_t)(key & ((1ull << nb) - 1));
+    const uint32_t i = ca < cb ? ca : cb, j = ca < cb ? cb : ca;
+    keys[p] = (uint64_t)(bodies[i].shape & 15u) << 4 | (bodies[j].shape & 15u);
+    vals[p] = p;
+}
+constexpr int kPairClassBits = 8;
+
+__global__ __launch_bounds__(kThreads) void kNarrow(const vx_body* __restrict__ bodies, Soup soup, float dt,
+                                                    const uint64_t* __restrict__ pairs, uint32_t np, int nb,
+                                                    const uint32_t* __restrict__ order,
+                                                    vx_contact* __restrict__ slots, uint32_t* __restrict__ counts) {
+    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= np) return;
+    const uint32_t p = order[q];
     uint64_t key = pairs[p];
     uint32_t ca = (uint32_t)(key >> nb), cb = (uint32_t)(key & ((1ull << nb) - 1));
     uint32_t i = ca < cb ? ca : cb, j = ca < cb ? cb : ca;

@@ -124,6 +124,8 @@ struct vxhip_ctx {
     Buf<uint64_t> keys, keysTmp;
     Buf<uint32_t> idx, idxTmp, oversize;
     Buf<uint64_t> pairs;
+    Buf<uint64_t> pairClass, pairClassTmp;  // candidate pairs by shape class (kPairClass)
+    Buf<uint32_t> pairOrder, pairOrderTmp;
     Buf<uint32_t> pairVals, counts, offsets;
     Buf<vx_contact> slots, contacts, solve, prev;
     Buf<uint32_t> colorVals, colorVals2, pred, color, colorCount, order, colorStart;
@@ -495,7 +497,12 @@ uint32_t narrowPhase(vxhip_ctx* c, uint32_t np, float dt) {
     c->counts.reserve(np + 1);
     c->offsets.reserve(np + 1);
     if (np) {
-        kNarrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, np, nb, c->slots.p, c->counts.p);
+        c->pairClass.reserve(np); c->pairClassTmp.reserve(np); c->pairOrder.reserve(np); c->pairOrderTmp.reserve(np);
+        kPairClass<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, np, nb, c->pairClass.p, c->pairOrder.p);
+        HIPCK(vxhost::radixSort(c->pairClass.p, c->pairOrder.p, c->pairClassTmp.p, c->pairOrderTmp.p, np,
+                                kPairClassBits, tempFor(c, vxhost::radixTempBytes(np)), s));
+        kNarrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, np, nb, c->pairOrder.p,
+                                                c->slots.p, c->counts.p);
         HIPCK(hipGetLastError());
     }
     HIPCK(vxhost::exclusiveScan(c->counts.p, c->offsets.p, np, c->counters.p + cContacts,
@@ -970,7 +977,8 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     c->ccdFlag.reserve(np);
     c->ccdGap.reserve(np);
     const uint8_t* toiDone = c->toiProcessed ? c->toiEligible.p : nullptr;
-    kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb, c->ccdFlag.p,
+    kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb,
+                                              c->pairOrder.p, c->ccdFlag.p,
                                               c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst, toiDone);
     HIPCK(hipGetLastError());
     readCounters(c);
