@@ -13,9 +13,13 @@ namespace vxk {
 
 struct Gap { float gap; V3 normal; };
 
-__device__ inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup& soup, float search) {
+// `coop`: the whole wavefront evaluates this same query (hull supports are
+// then shared across the lanes; see hullSupportIndexWave).
+__device__ inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup& soup, float search,
+                                bool coop = false) {
     if (tb.shape == VX_SHAPE_PLANE) {
         Convex c = makeConvex(ta, soup);
+        c.coop = coop;
         V3 pn = v3(tb.planeNormal);
         V3 deep = c.support(-pn);
         return {dot(pn, deep) - tb.planeOffset - c.radius, pn};
@@ -25,13 +29,16 @@ __device__ inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup
         meshGapProbe1(ta, tb, soup, search, best.gap, best.normal);
         return best;
     }
-    GjkResult r = gjkDistance(makeConvex(ta, soup), makeConvex(tb, soup));
+    Convex ca = makeConvex(ta, soup), cb = makeConvex(tb, soup);
+    ca.coop = coop;
+    cb.coop = coop;
+    GjkResult r = gjkDistance(ca, cb);
     return {r.distance, r.normal};
 }
 
 // world.cpp:2916-2958 (compound children expanded without recursion).
 __device__ inline Gap gapAt(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0, V3 pb, Q4 qb, const Soup& soup,
-                            float search) {
+                            float search, bool coop = false) {
     vx_body ta = a0; ta.position = raw(pa); ta.orientation = raw(qa);
     vx_body tb = b0; tb.position = raw(pb); tb.orientation = raw(qb);
     if (ta.shape == VX_SHAPE_COMPOUND) {
@@ -43,11 +50,11 @@ __device__ inline Gap gapAt(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0, 
                 g = {1e30f, {0, 1, 0}};
                 for (uint32_t k = 0; k < tb.compoundCount; ++k) {
                     vx_body cb = childBody(tb, soup.children[tb.compoundFirst + k]);
-                    Gap h = gapSimple(ch, cb, soup, search);
+                    Gap h = gapSimple(ch, cb, soup, search, coop);
                     if (h.gap < g.gap) g = h;
                 }
             } else {
-                g = gapSimple(ch, tb, soup, search);
+                g = gapSimple(ch, tb, soup, search, coop);
             }
             if (g.gap < best.gap) best = g;
         }
@@ -57,12 +64,12 @@ __device__ inline Gap gapAt(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0, 
         Gap best{1e30f, {0, 1, 0}};
         for (uint32_t i = 0; i < tb.compoundCount; ++i) {
             vx_body ch = childBody(tb, soup.children[tb.compoundFirst + i]);
-            Gap g = gapSimple(ta, ch, soup, search);
+            Gap g = gapSimple(ta, ch, soup, search, coop);
             if (g.gap < best.gap) best = g;
         }
         return best;
     }
-    return gapSimple(ta, tb, soup, search);
+    return gapSimple(ta, tb, soup, search, coop);
 }
 
 __device__ inline Q4 orientationAt(const vx_body& body, Q4 start, V3 L, float elapsed) {
@@ -127,14 +134,17 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
 // The reference's per-pair rescue (world.cpp:4276-4355) on one lane.
 // `known` carries the end gap the parallel check measured when neither body
 // has moved since (same state, same result).
+// With `coop` the whole wavefront runs the same rescue (every lane computes
+// the same values) and only the `writer` lane stores the bodies.
 __device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const vx_quat* prevQ, uint32_t i, uint32_t j,
-                                 const Soup& soup, float dt, const float4* known) {
+                                 const Soup& soup, float dt, const float4* known, bool coop = false,
+                                 bool writer = true) {
     vx_body a = bodies[i], b = bodies[j];
     if (!ccdEligible(a, b)) return false;
     const float search = ccdSearch(a, dt);
     Gap end;
     if (known) end = {known->w, {known->x, known->y, known->z}};
-    else end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup, search);
+    else end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup, search, coop);
     if (end.gap >= -8e-3f) return false;
     V3 pa0 = v3(prevPos[i]), da = v3(a.position) - pa0;
     Q4 qa0 = q4(prevQ[i]);
@@ -150,7 +160,7 @@ __device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const 
     for (int it = 0; it < 24 && t < 1.0f; ++it) {
         Q4 qa = orientationAt(a, qa0, La, dt * t);
         Q4 qb = isStatic(b) ? qb0 : orientationAt(b, qb0, Lb, dt * t);
-        Gap g = gapAt(a, pa0 + da * t, qa, b, pb0 + db * t, qb, soup, search);
+        Gap g = gapAt(a, pa0 + da * t, qa, b, pb0 + db * t, qb, soup, search, coop);
         n = g.normal;
         if (g.gap < 1e-3f) break;
         t = fmn(1.0f, t + g.gap / bound);
@@ -177,6 +187,7 @@ __device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const 
     float remaining = dt * (1.0f - t);
     advanceTransform(a, remaining);
     if (isDynamic(b)) advanceTransform(b, remaining);
+    if (!writer) return true;
     bodies[i].position = a.position; bodies[i].orientation = a.orientation;
     bodies[i].velocity = a.velocity; bodies[i].angularVelocity = a.angularVelocity;
     if (isDynamic(b)) {
@@ -289,6 +300,7 @@ __global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restric
 // co-resident, so the lowest unfinished pair is always running and ready; a
 // spin budget bounds waits.
 constexpr int kCcdBlocks = 128;
+constexpr int kCoopRescues = 2;  // at most this many ready rescues share the wavefront
 
 __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const vx_vec3* __restrict__ prevPos,
                                                          const vx_quat* __restrict__ prevQ, Soup soup, float dt,
@@ -305,21 +317,56 @@ __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const 
     const uint32_t stride = 64 * waves;
     uint32_t t = wave + waves * lane;
     uint32_t spins = 0;
-    while (t < nr) {
-        const uint32_t p = list[t];
-        const uint32_t pa = pred[2 * p], pb = pred[2 * p + 1];
-        const uint32_t wa = pa == kNone ? 1u : __hip_atomic_load(&word[pa >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const uint32_t wb = pb == kNone ? 1u : __hip_atomic_load(&word[pb >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((wa & 1u) && (wb & 1u)) {
-            const bool movedA = pa != kNone && ((wa >> (1 + (pa & 1))) & 1u);
-            const bool movedB = pb != kNone && ((wb >> (1 + (pb & 1))) & 1u);
-            if (movedA || movedB) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            bool r = false;
-            uint32_t i = 0, j = 0;
-            if (flag[p] || movedA || movedB) ccdBodies(bodies, pairs[p], nb, i, j);
-            if ((flag[p] || movedA || movedB) && !(toiDone && (toiDone[i] || toiDone[j]))) {
+    bool active = t < nr;
+    // The wavefront loops while any lane has entries left, so it is whole at
+    // the rescues below.
+    while (__any(active)) {
+        bool ready = false, exam = false, movedA = false, movedB = false;
+        uint32_t p = 0, i = 0, j = 0;
+        if (active) {
+            p = list[t];
+            const uint32_t pa = pred[2 * p], pb = pred[2 * p + 1];
+            const uint32_t wa = pa == kNone ? 1u : __hip_atomic_load(&word[pa >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t wb = pb == kNone ? 1u : __hip_atomic_load(&word[pb >> 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ready = (wa & 1u) && (wb & 1u);
+            if (ready) {
+                movedA = pa != kNone && ((wa >> (1 + (pa & 1))) & 1u);
+                movedB = pb != kNone && ((wb >> (1 + (pb & 1))) & 1u);
+                if (movedA || movedB) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                if (flag[p] || movedA || movedB) {
+                    ccdBodies(bodies, pairs[p], nb, i, j);
+                    exam = !(toiDone && (toiDone[i] || toiDone[j]));
+                }
+            } else if (++spins > (1u << 24)) {
+                atomicOr(error, 1u);
+                active = false;
+            }
+        }
+        // Pairs ready together share no dynamic body (a pair waits for every
+        // earlier relevant pair on its bodies), so their rescues commute.  One
+        // or two of them (the usual case: consecutive entries sit on different
+        // wavefronts) run one after the other on the whole wavefront, which
+        // shares each hull support across the lanes; more run one per lane.
+        bool r = false;
+        uint64_t need = __ballot(exam);
+        if (__popcll(need) <= kCoopRescues) {
+            while (need) {
+                const int owner = __ffsll((unsigned long long)need) - 1;
+                const uint32_t po = (uint32_t)__builtin_amdgcn_readlane((int)p, owner);
+                const uint32_t io = (uint32_t)__builtin_amdgcn_readlane((int)i, owner);
+                const uint32_t jo = (uint32_t)__builtin_amdgcn_readlane((int)j, owner);
+                const bool movedO = __builtin_amdgcn_readlane((movedA || movedB) ? 1 : 0, owner) != 0;
+                const bool ro = ccdRescue(bodies, prevPos, prevQ, io, jo, soup, dt, movedO ? nullptr : &endGap[po], true,
+                                          lane == (uint32_t)owner);
+                if (lane == (uint32_t)owner) r = ro;
+                need &= need - 1;
+            }
+        } else if (exam) {
+            r = ccdRescue(bodies, prevPos, prevQ, i, j, soup, dt, (movedA || movedB) ? nullptr : &endGap[p]);
+        }
+        if (ready) {
+            if (exam) {
                 atomicAdd(examined, 1u);
-                r = ccdRescue(bodies, prevPos, prevQ, i, j, soup, dt, (movedA || movedB) ? nullptr : &endGap[p]);
                 if (r) atomicAdd(rescued, 1u);
             }
             const uint32_t out = 1u | ((movedA || r) ? 2u : 0u) | ((movedB || r) ? 4u : 0u);
@@ -327,9 +374,7 @@ __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const 
             else __hip_atomic_store(&word[p], out, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             t += stride;
             spins = 0;
-        } else if (++spins > (1u << 24)) {
-            atomicOr(error, 1u);
-            break;
+            active = t < nr;
         }
     }
 }

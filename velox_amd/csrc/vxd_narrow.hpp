@@ -11,6 +11,44 @@ namespace vxd {
 enum ConvexKind : uint8_t { CPoint, CSegment, CBox, CTriangle, CHull, CCylinder, CCone,
                             CRoundedBox, CEllipsoid };
 
+// Wavefront-cooperative hull support (Convex::coop): every lane of a full
+// wavefront holds the same query.  Lane l scans points l, l + 64, ...; the
+// lanes' best (dot, index) candidates are merged by six DPP steps (row shifts
+// 1, 2, 4, 8, then row broadcasts 15 and 31) into lane 63.  The merge keeps the
+// larger dot and, on equal dots, the lower index; dots that are NaN never
+// win, and a NaN first dot keeps point 0.  That is exactly the point the
+// one-lane scan (first point first, replaced on a strictly larger dot) picks.
+constexpr uint32_t kNoPoint = 0xffffffffu;
+
+template <int kCtrl, int kRowMask>
+VXD void waveBestStep(float& t, uint32_t& i) {
+    const float nt = __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(t), kCtrl, kRowMask, 0xf, false));
+    const uint32_t ni = (uint32_t)__builtin_amdgcn_update_dpp((int)kNoPoint, (int)i, kCtrl, kRowMask, 0xf, false);
+    if (ni != kNoPoint && (i == kNoPoint || nt > t || (nt == t && ni < i))) { t = nt; i = ni; }
+}
+
+VXD uint32_t hullSupportIndexWave(const vx_vec3* pts, uint32_t count, V3 d) {
+    if (!(dot(d, v3(pts[0])) == dot(d, v3(pts[0])))) return 0;
+    const uint32_t lane = __lane_id();
+    float t = 0.0f;
+    uint32_t i = kNoPoint;
+    for (uint32_t base = 0; base < count; base += 64) {
+        const uint32_t k = base + lane;
+        if (k < count) {
+            const float tk = dot(d, v3(pts[k]));
+            if (tk == tk && (i == kNoPoint || tk > t)) { t = tk; i = k; }
+        }
+    }
+    waveBestStep<0x111, 0xf>(t, i);  // row_shr:1
+    waveBestStep<0x112, 0xf>(t, i);  // row_shr:2
+    waveBestStep<0x114, 0xf>(t, i);  // row_shr:4
+    waveBestStep<0x118, 0xf>(t, i);  // row_shr:8
+    waveBestStep<0x142, 0xa>(t, i);  // row_bcast:15
+    waveBestStep<0x143, 0xc>(t, i);  // row_bcast:31
+    const uint32_t best = (uint32_t)__builtin_amdgcn_readlane((int)i, 63);
+    return best < count ? best : 0u;  // lane 0 holds a valid candidate, so lane 63 does too
+}
+
 struct Convex {                 // gjk.h:13-29
     ConvexKind kind = CPoint;
     V3 position;
@@ -25,6 +63,7 @@ struct Convex {                 // gjk.h:13-29
     float geomRadius = 0.0f;
     float boundingRadius = 0.0f;
     uint8_t quality = VX_GEOM_NORMAL;
+    bool coop = false;  // hull support by the whole wavefront (all lanes, same query)
 
     // gjk.h:30-148
     VXD V3 supportOffset(V3 dir) const {
@@ -48,6 +87,7 @@ struct Convex {                 // gjk.h:13-29
         }
         case CHull: {
             V3 d = rotateInv(orientation, dir);
+            if (coop) return rotate(orientation, v3(hullPts[hullSupportIndexWave(hullPts, hullCount, d)]));
             V3 bestP{};
             float bestDot = 0.0f;
             // Eight points per batch, loaded together (one memory round trip
