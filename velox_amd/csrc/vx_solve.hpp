@@ -775,7 +775,9 @@ __device__ __attribute__((always_inline)) inline PairItem loadPairItem(SolveArra
     return loadPairItemSlots(sa, slotPair[j], j, sideB);
 }
 
-template <int kMode, uint32_t kStride = kLdsBodies>  // 0 two-axis, 1 cone-block, 2 warm start
+// kGlobal: the lane's body velocities are read from and written back to the
+// global arrays (slot and lv unused) instead of LDS slots.
+template <int kMode, uint32_t kStride = kLdsBodies, bool kGlobal = false>  // 0 two-axis, 1 cone-block, 2 warm start
 __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, uint32_t j, PairItem& it, bool sideB,
                                                               float* __restrict__ lv) {
     const uint32_t flags = it.meta.z;
@@ -783,11 +785,15 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
     const bool dyn = flags & (sideB ? kDynB : kDynA);
     const bool rot = flags & (sideB ? kRotB : kRotA);
     const uint32_t slot = sideB ? it.slots >> 16 : it.slots & 0xffffu;
+    const uint32_t body = sideB ? it.meta.y : it.meta.x;
     const float4 r0 = it.r[0], r1 = it.r[1], r2 = it.r[2], r3 = it.r[3], r4 = it.r[4], r5 = it.r[5], r6 = it.r[6],
                  r7 = it.r[7];
     const float4 fr = it.frame, ii = it.invI;
     V3 v, w;
-    if (dyn) {
+    if (dyn && kGlobal) {
+        v = xyz(sa.vel[body]);
+        w = xyz(sa.ang[body]);
+    } else if (dyn) {
         v = V3{lv[slot], lv[kStride + slot], lv[2 * kStride + slot]};
         w = V3{lv[3 * kStride + slot], lv[4 * kStride + slot], lv[5 * kStride + slot]};
     } else {
@@ -917,10 +923,28 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
         sa.imp[2 * j] = it.i0;
         sa.imp[2 * j + 1] = it.i1;
     }
-    if (dyn) {
+    if (dyn && kGlobal) {
+        sa.vel[body] = make_float4(v.x, v.y, v.z, 0.0f);
+        sa.ang[body] = make_float4(w.x, w.y, w.z, 0.0f);
+    } else if (dyn) {
         lv[slot] = v.x; lv[kStride + slot] = v.y; lv[2 * kStride + slot] = v.z;
         lv[3 * kStride + slot] = w.x; lv[4 * kStride + slot] = w.y; lv[5 * kStride + slot] = w.z;
     }
+}
+
+// One wide colour over the whole device, two lanes per row (the colour's
+// rows share no body, so every row is independent), velocities in the global
+// arrays.
+template <int kMode>  // 0 two-axis, 1 cone-block, 2 warm start
+__global__ __launch_bounds__(kThreads) void kSweepColorPair(SolveArrays sa, uint32_t base, uint32_t first,
+                                                            uint32_t count) {
+    const uint32_t g = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t row = g >> 1;
+    if (row >= count) return;  // both lanes of a row leave together
+    const bool sideB = g & 1;
+    const uint32_t k = base + first + row;
+    PairItem it = loadPairItemSlots(sa, 0u, k, sideB);
+    pairRow<kMode, kLdsBodies, true>(sa, k, it, sideB, nullptr);
 }
 
 // LDS batches, one workgroup each: the batch's dynamic bodies are loaded

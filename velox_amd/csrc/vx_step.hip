@@ -747,7 +747,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 }
                 if (cnt > kNarrowSweepColor) {
                     solveLaunchBegin(c);
-                    kSweepColorIdx<kMode><<<blocks(cnt), kThreads, 0, bs>>>(sa, c->smallCount, cs[col], cnt);
+                    kSweepColorPair<kMode><<<blocks(2 * cnt), kThreads, 0, bs>>>(sa, c->smallCount, cs[col], cnt);
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
                     ++col;
                     continue;
