@@ -319,19 +319,6 @@ __device__ __attribute__((always_inline)) inline void warmRow(SolveArrays sa, ui
     }
 }
 
-// One colour of the big-island contacts: solve positions
-// base + colorStart[c] .. base + colorStart[c+1].
-template <int kMode>  // 0 = two-axis sweep, 1 = cone-block sweep, 2 = warm start
-__global__ __launch_bounds__(kThreads) void kSweepColorIdx(SolveArrays sa, uint32_t base, uint32_t first,
-                                                           uint32_t count) {
-    const uint32_t stride = gridDim.x * kThreads;
-    for (uint32_t j = blockIdx.x * kThreads + threadIdx.x; j < count; j += stride) {
-        const uint32_t k = base + first + j;
-        if (kMode == 2) warmRow(sa, k);
-        else solveRow<kMode == 1>(sa, k);
-    }
-}
-
 // Workgroup barrier that orders LDS only: global loads in flight (the next
 // rows' prefetch) and global stores (a row's own impulses, read back only by
 // the same thread) are not waited for.
@@ -707,29 +694,6 @@ __global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t
     *ldsContacts = runStart[batchFirstRun[*ldsBatches]];
 }
 
-// Batches of small islands, one workgroup each: every sweep of every colour
-// present in the batch, barrier between colours.  The batch's islands share
-// no dynamic body with any other contact, and contacts of one colour share
-// none with each other, so this is the colour order exactly.  Batches
-// [first, last) of the batch list.
-template <int kMode, int kBlock>
-__global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const uint32_t* __restrict__ runStart,
-                                                        const uint32_t* __restrict__ islandFirstRun,
-                                                        const uint32_t* __restrict__ islandRange, int sweeps) {
-    const uint32_t first = islandRange[0], last = islandRange[1];
-    for (uint32_t isl = first + blockIdx.x; isl < last; isl += gridDim.x) {
-        const uint32_t r0 = islandFirstRun[isl], r1 = islandFirstRun[isl + 1];
-        for (int s = 0; s < sweeps; ++s)
-            for (uint32_t r = r0; r < r1; ++r) {
-                const uint32_t j0 = runStart[r], j1 = runStart[r + 1];
-                for (uint32_t k = j0 + threadIdx.x; k < j1; k += kBlock) {
-                    if (kMode == 2) warmRow(sa, k);
-                    else solveRow<kMode == 1>(sa, k);
-                }
-                __syncthreads();
-            }
-    }
-}
 
 // ------------------------------------------------------ LDS batch sweeps --
 // Two lanes per contact: the even lane carries body A, the odd lane body B.
@@ -929,6 +893,33 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
     } else if (dyn) {
         lv[slot] = v.x; lv[kStride + slot] = v.y; lv[2 * kStride + slot] = v.z;
         lv[3 * kStride + slot] = w.x; lv[4 * kStride + slot] = w.y; lv[5 * kStride + slot] = w.z;
+    }
+}
+
+// Medium islands, one workgroup each on the global velocity arrays: every
+// sweep of every colour, two lanes per row, barrier between colours (the
+// island shares no dynamic body with any other contact, and rows of one
+// colour share none with each other, so this is the colour order exactly).
+// Islands [first, last) of the island list.
+template <int kMode, int kBlock>
+__global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const uint32_t* __restrict__ runStart,
+                                                        const uint32_t* __restrict__ islandFirstRun,
+                                                        const uint32_t* __restrict__ islandRange, int sweeps) {
+    const uint32_t first = islandRange[0], last = islandRange[1];
+    const bool sideB = threadIdx.x & 1;
+    const uint32_t half = threadIdx.x >> 1;
+    constexpr uint32_t kPass = kBlock / 2;
+    for (uint32_t isl = first + blockIdx.x; isl < last; isl += gridDim.x) {
+        const uint32_t r0 = islandFirstRun[isl], r1 = islandFirstRun[isl + 1];
+        for (int s = 0; s < sweeps; ++s)
+            for (uint32_t r = r0; r < r1; ++r) {
+                const uint32_t j0 = runStart[r], j1 = runStart[r + 1];
+                for (uint32_t k = j0 + half; k < j1; k += kPass) {  // both lanes of a row together
+                    PairItem it = loadPairItemSlots(sa, 0u, k, sideB);
+                    pairRow<kMode, kLdsBodies, true>(sa, k, it, sideB, nullptr);
+                }
+                __syncthreads();
+            }
     }
 }
 
