@@ -285,39 +285,6 @@ __device__ __attribute__((always_inline)) inline void solveRow(SolveArrays sa, u
 
 namespace vxk {
 
-// Warm start of one contact on the compact arrays (narrowphase.h:80-124);
-// anchors come from the substep rows (same arithmetic as recomputing them).
-__device__ __attribute__((always_inline)) inline void warmRow(SolveArrays sa, uint32_t k) {
-    const uint4 meta = sa.meta[k];
-    const uint32_t flags = meta.z;
-    const float4 i0 = sa.imp[2 * k], i1 = sa.imp[2 * k + 1];
-    const float cN = i0.x, cT1 = i0.y, cT2 = i0.z, cR1 = i0.w, cR2 = i1.x, cS = i1.y;
-    if ((flags & kSensorRow) || (cN <= 0.0f && cT1 == 0.0f && cT2 == 0.0f && cR1 == 0.0f && cR2 == 0.0f && cS == 0.0f))
-        return;
-    const uint32_t ia = meta.x, ib = meta.y;
-    const float4* row = sa.rows + 8 * (size_t)k;
-    const V3 ra = xyz(row[0]), rb = xyz(row[1]), n = xyz(row[2]), t1 = xyz(row[3]), t2 = xyz(row[4]);
-    const bool dynA = flags & kDynA, dynB = flags & kDynB, rotA = flags & kRotA, rotB = flags & kRotB;
-    const float4 fa = sa.frame[ia], fb = sa.frame[ib], iia = sa.invI[ia], iib = sa.invI[ib];
-    V3 imp = n * cN + t1 * cT1 + t2 * cT2;
-    V3 ang = t1 * cR1 + t2 * cR2 + n * cS;
-    if (dynA) {
-        V3 v = xyz(sa.vel[ia]) + imp * iia.w;
-        V3 w = xyz(sa.ang[ia]);
-        w += frameMul(fa, iia, cross(ra, imp), rotA);
-        w += frameMul(fa, iia, ang, rotA);
-        sa.vel[ia] = make_float4(v.x, v.y, v.z, 0.0f);
-        sa.ang[ia] = make_float4(w.x, w.y, w.z, 0.0f);
-    }
-    if (dynB) {
-        V3 v = xyz(sa.vel[ib]) - imp * iib.w;
-        V3 w = xyz(sa.ang[ib]);
-        w -= frameMul(fb, iib, cross(rb, imp), rotB);
-        w -= frameMul(fb, iib, ang, rotB);
-        sa.vel[ib] = make_float4(v.x, v.y, v.z, 0.0f);
-        sa.ang[ib] = make_float4(w.x, w.y, w.z, 0.0f);
-    }
-}
 
 // Workgroup barrier that orders LDS only: global loads in flight (the next
 // rows' prefetch) and global stores (a row's own impulses, read back only by
@@ -328,78 +295,12 @@ __device__ __attribute__((always_inline)) inline void ldsBarrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-// A run of narrow big-island colours [c0, c1) in one workgroup, one colour
+// Runs of narrow big-island colours are solved in one workgroup, one colour
 // after the other (a launch per colour costs more than a few rounds of 1024
 // rows); the barrier orders the colours on one compute unit.  512 threads
-// leave each wave the registers to hold a prefetched row without spilling.  Everything a
-// row reads except the two bodies' velocities (its rows, body frames and
-// inertia, its own accumulated impulses) is loaded during the previous
-// colour, so only the velocity loads sit between two barriers.
+// leave each wave the registers to hold rows loaded two passes ahead without
+// spilling (kSweepColorRun, kSweepTailPair).
 constexpr int kSweepRunThreads = 512;
-
-struct RowPrefetch {
-    RowRegs R;
-    float4 fa, ia, fb, ib, i0, i1;
-    uint32_t a, b;
-};
-
-__device__ __attribute__((always_inline)) inline void prefetchRow(SolveArrays sa, uint32_t k, RowPrefetch& P) {
-    const uint4 meta = sa.meta[k];
-    P.a = meta.x;
-    P.b = meta.y;
-    P.R.flags = meta.z;
-    for (int q = 0; q < 8; ++q) P.R.r[q] = sa.rows[8 * (size_t)k + q];
-    // The row's copy of its bodies' frames (written with the row each
-    // substep): every load here depends on k alone, one memory round trip.
-    const float4* sd = sa.side + 8 * (size_t)k;
-    P.fa = sd[0]; P.ia = sd[1];
-    P.fb = sd[4]; P.ib = sd[5];
-    P.i0 = sa.imp[2 * k]; P.i1 = sa.imp[2 * k + 1];
-}
-
-template <bool kCone>
-__device__ __attribute__((always_inline)) inline void finishRow(SolveArrays sa, uint32_t k, const RowPrefetch& P) {
-    if (P.R.flags & kSkip) return;
-    const uint32_t ia = P.a, ib = P.b;
-    V3 vA = xyz(sa.vel[ia]), wA = xyz(sa.ang[ia]), vB = xyz(sa.vel[ib]), wB = xyz(sa.ang[ib]);
-    float cN = P.i0.x, cT1 = P.i0.y, cT2 = P.i0.z, cR1 = P.i0.w, cR2 = P.i1.x, cS = P.i1.y;
-    solveRegs<kCone>(P.R, P.fa, P.ia, P.fb, P.ib, vA, wA, vB, wB, cN, cT1, cT2, cR1, cR2, cS);
-    if (P.R.flags & kDynA) { sa.vel[ia] = make_float4(vA.x, vA.y, vA.z, 0.0f); sa.ang[ia] = make_float4(wA.x, wA.y, wA.z, 0.0f); }
-    if (P.R.flags & kDynB) { sa.vel[ib] = make_float4(vB.x, vB.y, vB.z, 0.0f); sa.ang[ib] = make_float4(wB.x, wB.y, wB.z, 0.0f); }
-    sa.imp[2 * k] = make_float4(cN, cT1, cT2, cR1);
-    sa.imp[2 * k + 1] = make_float4(cR2, cS, 0.0f, 0.0f);
-}
-
-template <int kMode>
-__global__ __launch_bounds__(kSweepRunThreads) void kSweepColorRun(SolveArrays sa, uint32_t base,
-                                                              const uint32_t* __restrict__ colorStart, uint32_t c0,
-                                                              uint32_t c1) {
-    if (kMode == 2) {
-        for (uint32_t c = c0; c < c1; ++c) {
-            const uint32_t first = colorStart[c], last = colorStart[c + 1];
-            for (uint32_t j = first + threadIdx.x; j < last; j += kSweepRunThreads) warmRow(sa, base + j);
-            __syncthreads();
-        }
-        return;
-    }
-    RowPrefetch P;
-    uint32_t k = kNone;
-    {
-        const uint32_t j = colorStart[c0] + threadIdx.x;
-        if (j < colorStart[c0 + 1]) { k = base + j; prefetchRow(sa, k, P); }
-    }
-    for (uint32_t c = c0; c < c1; ++c) {
-        const uint32_t first = colorStart[c], last = colorStart[c + 1];
-        if (k != kNone) finishRow<kMode == 1>(sa, k, P);
-        for (uint32_t j = first + threadIdx.x + kSweepRunThreads; j < last; j += kSweepRunThreads) solveRow<kMode == 1>(sa, base + j);
-        k = kNone;
-        if (c + 1 < c1) {
-            const uint32_t j = colorStart[c + 1] + threadIdx.x;
-            if (j < colorStart[c + 2]) { k = base + j; prefetchRow(sa, k, P); }
-        }
-        __syncthreads();
-    }
-}
 
 // The tail of the big-island colours (the last colours, every one narrow)
 // touches few bodies; their velocities then live in LDS for the whole sweep:
@@ -920,6 +821,41 @@ __global__ __launch_bounds__(kBlock) void kIslandSweeps(SolveArrays sa, const ui
                 }
                 __syncthreads();
             }
+    }
+}
+
+// A run of narrow colours c0..c1-1 by one workgroup, two lanes per row on
+// the global velocity arrays, barrier between colours; rows are loaded two
+// passes ahead (the tail kernel's cursor, with the colour bounds read from
+// global memory).
+template <int kMode>  // 0 two-axis, 1 cone-block, 2 warm start
+__global__ __launch_bounds__(kSweepRunThreads) void kSweepColorRun(SolveArrays sa, uint32_t base,
+                                                                   const uint32_t* __restrict__ colorStart, uint32_t c0,
+                                                                   uint32_t c1) {
+    const uint32_t t = threadIdx.x;
+    const bool sideB = t & 1;
+    const uint32_t half = t >> 1;
+    constexpr uint32_t kPass = kSweepRunThreads / 2;
+    struct Cursor { uint32_t c, cb, ce; };
+    auto next = [&](const Cursor& x) {
+        if (x.cb + kPass < x.ce) return Cursor{x.c, x.cb + kPass, x.ce};
+        const uint32_t nc = x.c + 1;
+        return nc < c1 ? Cursor{nc, colorStart[nc], colorStart[nc + 1]} : Cursor{c1, 0u, 0u};
+    };
+    Cursor A{c0, colorStart[c0], colorStart[c0 + 1]};
+    PairItem itA, itB;
+    if (A.cb + half < A.ce) itA = loadPairItemSlots(sa, 0u, base + A.cb + half, sideB);
+    Cursor B = next(A);
+    if (B.c < c1 && B.cb + half < B.ce) itB = loadPairItemSlots(sa, 0u, base + B.cb + half, sideB);
+    for (;;) {
+        const Cursor C = B.c < c1 ? next(B) : Cursor{c1, 0u, 0u};
+        PairItem itC;
+        if (C.c < c1 && C.cb + half < C.ce) itC = loadPairItemSlots(sa, 0u, base + C.cb + half, sideB);
+        if (A.cb + half < A.ce) pairRow<kMode, kLdsBodies, true>(sa, base + A.cb + half, itA, sideB, nullptr);
+        if (A.cb + kPass >= A.ce) __syncthreads();
+        if (B.c >= c1) break;
+        A = B; itA = itB;
+        B = C; itB = itC;
     }
 }
 
