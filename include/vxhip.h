@@ -230,6 +230,30 @@ typedef struct vx_multi_toi_settings {
 } vx_multi_toi_settings;
 int vxhip_world_set_multi_toi(vxhip_ctx* ctx, const vx_multi_toi_settings* settings);
 
+/* One time-of-impact hit (MultiToiHit, world.h:121-127); `body` is the dense
+ * index of the body hit. */
+typedef struct vx_toi_hit {
+    float toi;
+    uint32_t body;
+    float point[3];
+    float normal[3];   /* from the hit body toward the queried body */
+    float fraction;    /* toi / dt */
+} vx_toi_hit;
+
+/* World::queryMultiToi (world.h:405, world.cpp:3146-3185) on the resident
+ * world: findToi of body `body` against every other body that passes the
+ * collision filter, with the multi-TOI settings' velocity floor and
+ * penetration bias.  *count receives the number of hits; `out` receives the
+ * first min(cap, *count) of them sorted by (toi, body index).  The caller
+ * applies the reference's cap (min of maxToiEventsPerBody and
+ * maxTotalEventsPerStep) and, for a World whose handles are not in index
+ * order, re-sorts ties by handle before capping.  A World with a collision
+ * filter callback keeps the host query (the callback is host code).  A
+ * non-finite or non-positive dt fails with VXHIP_ERR_INVALID, as the
+ * reference throws; batched contexts fail with VXHIP_ERR_UNAVAILABLE. */
+int vxhip_world_query_multi_toi(vxhip_ctx* ctx, uint32_t body, float dt, vx_toi_hit* out, uint32_t cap,
+                                uint32_t* count);
+
 /* Moves the world origin: every stored world coordinate of the resident world
  * (body positions, plane offsets, mesh vertices, mesh and BVH bounds, the last
  * step's contacts, the warm-start cache, the last events' points) becomes

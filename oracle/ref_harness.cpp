@@ -689,6 +689,48 @@ void substepsCapture(const std::string& out, const std::string& name, int at) {
                  cap.contactsIn.size(), cap.jointsIn.size(), cap.substeps);
 }
 
+// World::queryMultiToi (world.cpp:3146-3185) on snapshots of the high-quality
+// scene: every dynamic body queried at the scene's dt.  hits.npy rows are
+// (query, toi, body index, point, normal, fraction) for the capped, sorted
+// hits the reference returns; handles are dense indices here (no removals).
+void toiQueryFixture(const std::string& out, const std::string& name, const std::vector<int>& snaps) {
+    const vxscene::Scene scene = vxscene::highToi();
+    World w(BackendType::Cpu);
+    build(w, scene);
+    w.backend_.reset(new CudaSemantics());
+    int done = 0;
+    for (int target : snaps) {
+        while (done < target) { w.step(scene.dt); ++done; }
+        const std::string dir = out + "/" + name + "_" + std::to_string(target);
+        mkdir(dir.c_str(), 0755);
+        writeRecords(dir + "/bodies_in.npy", w.bodies_);
+        writeSoup(dir, w.meshes_);
+        struct Row { uint32_t query; float toi; uint32_t body; float point[3], normal[3], fraction; };
+        std::vector<Row> rows;
+        std::vector<uint32_t> queries;
+        for (BodyIndex i = 0; i < w.bodies_.size(); ++i) {
+            if (!w.bodies_[i].isDynamic()) continue;
+            const BodyId id = w.bodyHandle(i);
+            if (id.slot() != i) throw std::runtime_error("harness: handles are not dense indices");
+            queries.push_back(i);
+            for (const MultiToiHit& h : w.queryMultiToi(id, scene.dt)) {
+                if (h.body.slot() >= w.bodies_.size()) throw std::runtime_error("harness: hit handle");
+                rows.push_back({i, h.toi, h.body.slot(), {h.point.x, h.point.y, h.point.z},
+                                {h.normal.x, h.normal.y, h.normal.z}, h.fraction});
+            }
+        }
+        writeRecords(dir + "/hits.npy", rows);
+        writeNpy(dir + "/queries.npy", "<u4", {queries.size()}, queries.data(), queries.size() * 4);
+        char meta[256];
+        std::snprintf(meta, sizeof meta, "{\"dt\": %.9g, \"step\": %d, \"queries\": %zu, \"hits\": %zu}\n",
+                      scene.dt, target, queries.size(), rows.size());
+        FILE* f = std::fopen((dir + "/meta.json").c_str(), "w");
+        std::fputs(meta, f);
+        std::fclose(f);
+        std::fprintf(stderr, "  %s: %zu queries, %zu hits\n", dir.c_str(), queries.size(), rows.size());
+    }
+}
+
 // Reference narrow phase on snapshots of a reference trajectory: records,
 // the exact pair set (sorted (lo<<32|hi)) and every pair's contacts from
 // collidePair in canonical order.
@@ -903,6 +945,10 @@ int fixture(const std::string& out, const std::string& name) {
     }
     if (name == "hightoi") {  // MotionQuality::High bodies through processHighToi
         trajectory(out, name, vxscene::highToi(), {1, 3, 5, 6, 7, 8, 12, 18, 28, 40, 60}, false, false, true);
+        return 0;
+    }
+    if (name == "toiquery") {  // World::queryMultiToi for every dynamic body of two snapshots
+        toiQueryFixture(out, "toiquery", {5, 12, 20});
         return 0;
     }
     if (name == "toicount") {  // per-step multi-TOI event counts (tuning aid)

@@ -34,6 +34,8 @@ def main():
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "joints"], cwd=ROOT)
     # MotionQuality::High bullets and boxes through the multi-TOI scheduler.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "hightoi"], cwd=ROOT)
+    # World::queryMultiToi for every dynamic body of the same scene at steps 5, 12 and 20.
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "toiquery"], cwd=ROOT)
     # Origin shifts in the middle of a lattice and of a heightfield terrain trajectory.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "shift"], cwd=ROOT)
     # The 1,000-box lattice with sleeping on over 300 steps (several minutes).

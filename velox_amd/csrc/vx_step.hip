@@ -198,6 +198,8 @@ struct vxhip_ctx {
     Buf<vxk::ToiState> toiState;
     uint32_t toiEventCount = 0;
     Buf<uint32_t> ccdDone;
+    Buf<uint32_t> toiQueryBits;  // vxhip_world_query_multi_toi: per other body
+    Buf<float4> toiQueryHit;
     Buf<float4> ccdGap;
     Buf<uint8_t> temp;
     Buf<uint32_t> counters;  // device counters, see Counter
@@ -230,8 +232,9 @@ enum Counter : uint32_t {
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
     cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55,
-    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cCount = 64
+    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cToiQueryHits = 64, cCount = 65
 };
+static_assert(cCount <= 128, "kResetCounters runs one 128-lane workgroup");
 
 void* tempFor(vxhip_ctx* c, size_t bytes) {
     c->temp.reserve(bytes);
@@ -401,7 +404,7 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
     c->lo.reserve(n); c->hi.reserve(n);
     c->keys.reserve(n); c->keysTmp.reserve(n); c->idx.reserve(n); c->idxTmp.reserve(n); c->oversize.reserve(n);
     uint32_t* k = c->counters.p;
-    kResetCounters<<<1, 64, 0, s>>>(k);
+    kResetCounters<<<1, 128, 0, s>>>(k);
     kAabb<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, dt, c->lo.p, c->hi.p, k + cExtent);
     kCellBounds<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, k + cExtent, (int*)(k + cBounds),
                                                k + cOversize2);
@@ -458,7 +461,7 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
     if (c->pairCap == 0) c->pairCap = 32 * n + 1024;
     for (;;) {
         c->pairs.reserve(c->pairCap);
-        kResetCounters<<<1, 64, 0, s>>>(k);  // only cPairs matters here; others re-derived
+        kResetCounters<<<1, 128, 0, s>>>(k);  // only cPairs matters here; others re-derived
         kGridPairs<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->keys.p, c->idx.p,
                                                   table ? c->cellRange.p : nullptr, n, g, nb, c->pairs.p, k + cPairs,
                                                   c->pairCap);
@@ -534,7 +537,7 @@ uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     c->pred.reserve(2 * (size_t)m);
     c->maskAfter.reserve(2 * (size_t)m);
     c->nextAfter.reserve(2 * (size_t)m);
-    kResetCounters<<<1, 64, 0, s>>>(k);
+    kResetCounters<<<1, 128, 0, s>>>(k);
     fill32(c, c->pred.p, 2 * m, kNone);
     fill32(c, c->color.p, m, kNone);
     fill32(c, c->colorCount.p, m + 65, 0);
@@ -601,7 +604,7 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
     c->smallKeys.reserve(m); c->smallKeysTmp.reserve(m); c->smallIdx.reserve(m); c->smallIdxTmp.reserve(m);
     c->runHead.reserve(m); c->islandHead.reserve(m); c->runPos.reserve(m + 1); c->islandPos.reserve(m + 1);
     c->runStart.reserve(m + 1); c->islandFirstRun.reserve(m + 1); c->slotPair.reserve(m); c->batchBodies.reserve(m);
-    kResetCounters<<<1, 64, 0, s>>>(k);
+    kResetCounters<<<1, 128, 0, s>>>(k);
     kIota<<<blocks(n), kThreads, 0, s>>>(c->uf.p, n);
     fill32(c, c->islandSize.p, n, 0);
     fill32(c, c->bodyCount.p, n, 0);
@@ -1183,7 +1186,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->prevPos.reserve(n); c->prevQ.reserve(n);
     uint32_t* k = c->counters.p;
     markStage(c, sBegin);
-    kResetCounters<<<1, 64, 0, s>>>(k, false);
+    kResetCounters<<<1, 128, 0, s>>>(k, false);
     kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
     HIPCK(hipGetLastError());
     if (c->anyHigh && c->toiSettings.enabled && c->toiSettings.splitting && n) {  // world.cpp:4015-4025
@@ -1213,7 +1216,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     }
     HIPCK(hipGetLastError());
     markStage(c, sCcd);
-    kResetCounters<<<1, 64, 0, s>>>(k);
+    kResetCounters<<<1, 128, 0, s>>>(k);
     if (c->jointCount) {  // World::finishBrokenJoints (world.cpp:3765-3783)
         c->jointBreaks.reserve(c->jointCount);
         kJointFinish<<<1, 1, 0, s>>>(c->joints.p, c->jointCount, h, c->jointBreaks.p, k + cJointsLeft, k + cJointBreaks);
@@ -1501,6 +1504,43 @@ int vxhip_world_set_multi_toi(vxhip_ctx* c, const vx_multi_toi_settings* v) {
         c->toiSettings = vxk::ToiSettings{v->enabled ? 1u : 0u, v->enableSubstepSplitting ? 1u : 0u,
                                           v->maxToiEventsPerBody, v->maxTotalEventsPerStep, v->toiVelocityFloor,
                                           v->toiPenetrationBias};
+    });
+}
+
+int vxhip_world_query_multi_toi(vxhip_ctx* c, uint32_t body, float dt, vx_toi_hit* out, uint32_t cap,
+                                uint32_t* count) {
+    return guarded([&] {
+        if (!c || !c->loaded || !count || (!out && cap)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (!std::isfinite(dt) || dt <= 0.0f)
+            fail(VXHIP_ERR_INVALID, "velox: multi-TOI timestep must be finite and positive");
+        if (body >= c->n) fail(VXHIP_ERR_INVALID, "vxhip: body index out of range");
+        if (c->worldCount > 1) fail(VXHIP_ERR_UNAVAILABLE, "vxhip: queryMultiToi runs on a single-world context");
+        const uint32_t n = c->n;
+        hipStream_t s = c->stream;
+        c->toiQueryBits.reserve(n);
+        c->toiQueryHit.reserve(2 * (size_t)n);
+        uint32_t* hits = c->counters.p + cToiQueryHits;
+        kFill32<<<1, 64, 0, s>>>(hits, 1, 0u);
+        kToiQuery<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, body, dt, c->toiSettings,
+                                                c->toiQueryBits.p, c->toiQueryHit.p, hits);
+        HIPCK(hipGetLastError());
+        std::vector<uint32_t> bits(n);
+        std::vector<float4> hit(2 * (size_t)n);
+        HIPCK(hipMemcpyAsync(bits.data(), c->toiQueryBits.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(hit.data(), c->toiQueryHit.p, 2 * (size_t)n * sizeof(float4), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        // Sorted by (toi, body) as the reference sorts its hits (world.cpp:3174-3178);
+        // non-negative times of impact order like their bit patterns.
+        std::vector<uint64_t> order;
+        for (uint32_t o = 0; o < n; ++o)
+            if (bits[o] != vxk::kNone) order.push_back((uint64_t)bits[o] << 32 | o);
+        std::sort(order.begin(), order.end());
+        *count = (uint32_t)order.size();
+        for (uint32_t k = 0; k < cap && k < order.size(); ++k) {
+            const uint32_t o = (uint32_t)order[k];
+            const float4 a = hit[2 * (size_t)o], b = hit[2 * (size_t)o + 1];
+            out[k] = vx_toi_hit{b.w, o, {b.x, b.y, b.z}, {a.x, a.y, a.z}, a.w};
+        }
     });
 }
 

@@ -136,6 +136,40 @@ __device__ inline bool findToi(const vx_body& a0, const vx_body& b0, float dt, f
     return true;
 }
 
+// World::queryMultiToi (world.cpp:3146-3185) for one source body against the
+// resident state: one lane per other body.  An ineligible source (not
+// dynamic, locked, continuous off, Low quality, or slower than the larger of
+// its own CCD floor and the settings' floor) has no hits.  A hit leaves its
+// time of impact bits in toiBits (kNone otherwise) and (normal, fraction),
+// (point, toi) in hit[2 o], hit[2 o + 1].
+__global__ __launch_bounds__(kThreads) void kToiQuery(const vx_body* __restrict__ bodies, uint32_t n, Soup soup,
+                                                      uint32_t src, float dt, ToiSettings ts,
+                                                      uint32_t* __restrict__ toiBits, float4* __restrict__ hit,
+                                                      uint32_t* __restrict__ count) {
+    const uint32_t o = blockIdx.x * kThreads + threadIdx.x;
+    if (o >= n) return;
+    const vx_body& s = bodies[src];
+    const float floorSpeed = s.ccdTuning.minVelocityForCCD < ts.velocityFloor ? ts.velocityFloor
+                                                                               : s.ccdTuning.minVelocityForCCD;
+    const bool eligible = isDynamic(s) && !isLocked(s) && s.ccdTuning.enableContinuous &&
+                          s.ccdTuning.quality != VX_QUALITY_LOW && !(maxPointSpeed(s) < floorSpeed);
+    uint32_t bits = kNone;
+    if (eligible && o != src) {
+        const vx_body& b = bodies[o];
+        if (canCollide(s, b) && !(isStatic(b) && isStatic(s))) {
+            float toi;
+            V3 nrm, pt;
+            if (findToi(s, b, dt, ts.bias, soup, toi, nrm, pt)) {
+                bits = __float_as_uint(toi);
+                hit[2 * o] = make_float4(nrm.x, nrm.y, nrm.z, toi / dt);
+                hit[2 * o + 1] = make_float4(pt.x, pt.y, pt.z, toi);
+                atomicAdd(count, 1u);
+            }
+        }
+    }
+    toiBits[o] = bits;
+}
+
 __device__ inline void toiPair(uint32_t p, uint32_t ne, uint32_t width, const uint32_t* elist, const uint32_t* slist,
                                uint32_t& a, uint32_t& b, bool& dynamicPair) {
     const uint32_t ai = p / width, bj = p % width;

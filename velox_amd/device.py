@@ -103,6 +103,7 @@ SIGNATURES = {
     "vxhip_world_set_sleep": (_i32, [_vp, _vp]),
     "vxhip_world_shift_origin": (_i32, [_vp, _vp]),
     "vxhip_world_set_multi_toi": (_i32, [_vp, _vp]),
+    "vxhip_world_query_multi_toi": (_i32, [_vp, _u32, _f32, _vp, _u32, _vp]),
     "vxhip_world_set_joints": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_download_joints": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_download_joint_breaks": (_i32, [_vp, _vp, _u32, _vp]),
@@ -378,6 +379,18 @@ class Context:
         cfg = MultiToiSettings(**kw)
         check(library().vxhip_world_set_multi_toi(self._h, C.byref(cfg)))
         return self
+
+    def query_multi_toi(self, body, dt, cap=None):
+        """World::queryMultiToi on the resident world: (hits sorted by (toi,
+        body index), total hit count); `cap` limits the hits returned."""
+        from .records import TOI_HIT
+        n = C.c_uint32(0)
+        check(library().vxhip_world_query_multi_toi(self._h, int(body), float(dt), None, 0, C.byref(n)))
+        want = n.value if cap is None else min(int(cap), n.value)
+        out = np.zeros(want, TOI_HIT)
+        if want:
+            check(library().vxhip_world_query_multi_toi(self._h, int(body), float(dt), ptr(out), want, C.byref(n)))
+        return out, n.value
 
     def shift_origin(self, offset):
         off = (C.c_float * 3)(*offset)

@@ -129,3 +129,7 @@ assert JOINT.itemsize == 420 and JOINT.fields["reactionLinearImpulse"][1] == 392
 
 # vx_joint_break (include/vxhip.h), 20 B.
 JOINT_BREAK = np.dtype([("index", "<u4"), ("a", "<u4"), ("b", "<u4"), ("force", "<f4"), ("torque", "<f4")])
+
+# vx_toi_hit (include/vxhip.h), 36 B: MultiToiHit with the dense body index.
+TOI_HIT = np.dtype([("toi", "<f4"), ("body", "<u4"), ("point", "<f4", 3), ("normal", "<f4", 3), ("fraction", "<f4")])
+assert TOI_HIT.itemsize == 36
