@@ -100,3 +100,20 @@ def test_batched_full_size_invariants(ctx):
     # dynamic body is still above its world's plane.
     dyn = b["shape"] != 1
     assert (b["position"][dyn, 1] > -0.5).all()
+
+
+def test_batched_restore_repeats_the_same_steps(ctx):
+    """A saved state of the batched world (every world's bodies and carried
+    contacts) replays the same steps after a restore: the batched snapshot
+    the differentiable checkpoints need."""
+    sc = _worlds()
+    print(f"batched restore, {len(sc.bodies)} bodies", flush=True)
+    ctx.load_scene(sc)
+    ctx.step(sc.dt, 10)
+    blob = ctx.save_state()
+    ctx.step(sc.dt, 15)
+    first = body_state(ctx.bodies())
+    ctx.restore_state(blob)
+    assert ctx.save_state() == blob
+    ctx.step(sc.dt, 15)
+    assert bits_equal(first, body_state(ctx.bodies()))
