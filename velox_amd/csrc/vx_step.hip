@@ -232,7 +232,7 @@ enum Counter : uint32_t {
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
     cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55,
-    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cToiQueryHits = 64, cCount = 65
+    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cCount = 64
 };
 static_assert(cCount <= 128, "kResetCounters runs one 128-lane workgroup");
 
@@ -1519,10 +1519,8 @@ int vxhip_world_query_multi_toi(vxhip_ctx* c, uint32_t body, float dt, vx_toi_hi
         hipStream_t s = c->stream;
         c->toiQueryBits.reserve(n);
         c->toiQueryHit.reserve(2 * (size_t)n);
-        uint32_t* hits = c->counters.p + cToiQueryHits;
-        kFill32<<<1, 64, 0, s>>>(hits, 1, 0u);
         kToiQuery<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, body, dt, c->toiSettings,
-                                                c->toiQueryBits.p, c->toiQueryHit.p, hits);
+                                                c->toiQueryBits.p, c->toiQueryHit.p);
         HIPCK(hipGetLastError());
         std::vector<uint32_t> bits(n);
         std::vector<float4> hit(2 * (size_t)n);

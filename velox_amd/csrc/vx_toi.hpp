@@ -144,8 +144,7 @@ __device__ inline bool findToi(const vx_body& a0, const vx_body& b0, float dt, f
 // (point, toi) in hit[2 o], hit[2 o + 1].
 __global__ __launch_bounds__(kThreads) void kToiQuery(const vx_body* __restrict__ bodies, uint32_t n, Soup soup,
                                                       uint32_t src, float dt, ToiSettings ts,
-                                                      uint32_t* __restrict__ toiBits, float4* __restrict__ hit,
-                                                      uint32_t* __restrict__ count) {
+                                                      uint32_t* __restrict__ toiBits, float4* __restrict__ hit) {
     const uint32_t o = blockIdx.x * kThreads + threadIdx.x;
     if (o >= n) return;
     const vx_body& s = bodies[src];
@@ -163,7 +162,6 @@ __global__ __launch_bounds__(kThreads) void kToiQuery(const vx_body* __restrict_
                 bits = __float_as_uint(toi);
                 hit[2 * o] = make_float4(nrm.x, nrm.y, nrm.z, toi / dt);
                 hit[2 * o + 1] = make_float4(pt.x, pt.y, pt.z, toi);
-                atomicAdd(count, 1u);
             }
         }
     }

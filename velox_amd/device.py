@@ -385,12 +385,10 @@ class Context:
         body index), total hit count); `cap` limits the hits returned."""
         from .records import TOI_HIT
         n = C.c_uint32(0)
-        check(library().vxhip_world_query_multi_toi(self._h, int(body), float(dt), None, 0, C.byref(n)))
+        out = np.zeros(self.n, TOI_HIT)  # at most one hit per other body
+        check(library().vxhip_world_query_multi_toi(self._h, int(body), float(dt), ptr(out), len(out), C.byref(n)))
         want = n.value if cap is None else min(int(cap), n.value)
-        out = np.zeros(want, TOI_HIT)
-        if want:
-            check(library().vxhip_world_query_multi_toi(self._h, int(body), float(dt), ptr(out), want, C.byref(n)))
-        return out, n.value
+        return out[:want].copy(), n.value
 
     def shift_origin(self, offset):
         off = (C.c_float * 3)(*offset)
