@@ -321,7 +321,7 @@ enum {
     VXHIP_KERNEL_BATCH = 0,      /* kBatchSweeps, velocity sweeps (island batches in LDS) */
     VXHIP_KERNEL_BATCH_WARM = 1, /* kBatchSweeps, warm start */
     VXHIP_KERNEL_ISLAND = 2,     /* kIslandSweeps (one larger island per workgroup) */
-    VXHIP_KERNEL_COLOR = 3,      /* kSweepColorIdx (big islands, one launch per colour) */
+    VXHIP_KERNEL_COLOR = 3,      /* big-island colours: kSweepColorPair, kSweepColorRun, kSweepTailPair */
     VXHIP_KERNEL_COUNT = 4
 };
 typedef struct {

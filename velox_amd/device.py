@@ -165,7 +165,7 @@ class Timings(C.Structure):
                 ("kernelLaunches", C.c_uint32 * 4), ("kernelVisits", C.c_uint64 * 4)]
 
     # Solver kernel families, in the order of the per-kernel arrays.
-    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorIdx<0>")
+    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorPair/Run/TailPair")
 
     def as_dict(self):
         d = {}
