@@ -413,7 +413,8 @@ std::vector<EventRecord> eventRecords(const World& w) {
 }
 
 void trajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene,
-                const std::vector<int>& steps, bool coneBlock = false, bool sleep = false, bool toi = false) {
+                const std::vector<int>& steps, bool coneBlock = false, bool sleep = false, bool toi = false,
+                bool lean = false) {
     std::string dir = out + "/" + name;
     mkdir(dir.c_str(), 0755);
     World w(BackendType::Cpu);
@@ -445,13 +446,14 @@ void trajectory(const std::string& out, const std::string& name, const vxscene::
             for (int q = 0; q < 11; ++q) sleepJson += (q ? ", " : "") + std::to_string(v[q]);
             sleepJson += "]";
         }
-        writeRecords(dir + "/events_" + std::to_string(target) + ".npy", eventRecords(w));
+        if (!lean || target == steps.back())
+            writeRecords(dir + "/events_" + std::to_string(target) + ".npy", eventRecords(w));
         colorsJson += (target == steps[0] ? "" : ", ") + std::to_string(w.contacts_.size());
         kJson += (target == steps[0] ? "" : ", ") + std::to_string(static_cast<CudaSemantics*>(w.backend_.get())->colors);
-        if (!wroteContacts) { writeRecords(dir + "/contacts_" + std::to_string(target) + ".npy", w.contacts_); wroteContacts = true; }
+        if (!wroteContacts && !lean) { writeRecords(dir + "/contacts_" + std::to_string(target) + ".npy", w.contacts_); wroteContacts = true; }
         std::fprintf(stderr, "  %s step %d: %zu contacts\n", name.c_str(), target, w.contacts_.size());
     }
-    writeRecords(dir + "/bodies_out.npy", w.bodies_);
+    if (!lean) writeRecords(dir + "/bodies_out.npy", w.bodies_);
     writeMeta(dir, w, scene.dt, steps, colorsJson + "]" + kJson + "]" + (sleep ? sleepJson + "]" : std::string()) +
                                            (toi ? toiJson + "]" : std::string()));
 }
@@ -906,7 +908,8 @@ int bench(const std::string& name, int steps, int workers, int warmup) {
 // for locating the first step where a replay departs from the reference).
 int trace(const std::string& out, const std::string& which, int steps) {
     vxscene::Scene s = which == "terrain" ? vxscene::terrain(16, 0.5f, 0.8f, 40)
-                     : which == "pile" ? vxscene::pile(160, 5.0f, 6.0f) : vxscene::lattice(4, 4, 4);
+                     : which == "pile" ? vxscene::pile(160, 5.0f, 6.0f)
+                     : which == "slabstack" ? vxscene::slabStack() : vxscene::lattice(4, 4, 4);
     mkdir(out.c_str(), 0755);
     World w(BackendType::Cpu);
     build(w, s);
@@ -941,6 +944,14 @@ int fixture(const std::string& out, const std::string& name) {
         jointTrajectory(out, "joints", {1, 2, 5, 10, 30, 60});
         jointTrajectory(out, "joints_trig", {1, 5, 10, 15, 20, 25, 30});
         substepsCapture(out, "substeps_joints", 61);
+        return 0;
+    }
+    if (name == "slabstack") {  // a 35,000-contact island, ~700 colours (states only: the records are large)
+        trajectory(out, name, vxscene::slabStack(), {1, 2, 3}, false, false, false, true);
+        return 0;
+    }
+    if (name == "nslab") {  // per-stage snapshots of the slab stack (debugging aid)
+        narrowSnapshots(out, name, vxscene::slabStack(), {2});
         return 0;
     }
     if (name == "hightoi") {  // MotionQuality::High bodies through processHighToi

@@ -411,7 +411,7 @@ float vxo_gap_at(const vx_body* b, uint32_t i, uint32_t j, const vx_soup* s, flo
 }
 
 // One full device-step replay.  `prev`/`prevCount` hold the previous step's
-// contacts in canonical order (they are replaced on return); `outContacts`
+// contacts in the reference's cache order (they are replaced on return); `outContacts`
 // receives this step's contacts in color order (as the device leaves them).
 int vxo_world_step(vx_body* bodies, uint32_t n, const vx_soup* s, const vxo_config* cfg, float dt,
                    vx_contact* prev, uint32_t* prevCount, uint32_t prevCap,
@@ -447,11 +447,16 @@ int vxo_world_step(vx_body* bodies, uint32_t n, const vx_soup* s, const vxo_conf
     uint32_t rescued = ccdNet(bodies, pp.data(), pq.data(), solve.data(), m, soup, dt);
     positionCorrection(bodies, solve.data(), m, h, opt);
     for (uint32_t i = 0; i < n; ++i) { bodies[i].force = {0, 0, 0}; bodies[i].torque = {0, 0, 0}; }
-    // Next step's warm-start source: this step's contacts in canonical order.
-    std::vector<vx_contact> canon(m);
-    for (uint32_t k = 0; k < m; ++k) canon[order[k]] = solve[k];
+    // Next step's warm-start source (world.cpp:4496-4521): this step's
+    // contacts in solve (colour) order, ordered by std::sort on the pair key.
+    // std::sort is not stable, and the order it leaves a pair's contacts in
+    // decides which cached contact a tie in the next step's matching takes.
+    std::vector<vx_contact> cache(solve.begin(), solve.end());
+    std::sort(cache.begin(), cache.end(), [](const vx_contact& x, const vx_contact& y) {
+        return ((uint64_t)x.a << 32 | x.b) < ((uint64_t)y.a << 32 | y.b);
+    });
     uint32_t keep = m < prevCap ? m : prevCap;
-    for (uint32_t k = 0; k < keep; ++k) prev[k] = canon[k];
+    for (uint32_t k = 0; k < keep; ++k) prev[k] = cache[k];
     *prevCount = keep;
     for (uint32_t k = 0; k < m && k < outCap; ++k) outContacts[k] = solve[k];
     if (stats) { stats->pairs = (uint32_t)keys.size(); stats->contacts = m; stats->colors = K; stats->ccdRescued = rescued; }
@@ -461,7 +466,7 @@ int vxo_world_step(vx_body* bodies, uint32_t n, const vx_soup* s, const vxo_conf
 // Contact events of one step (world.cpp:4399-4485) from the step's contacts
 // as the solver left them (colour order, accumulated impulses).  prevKeys /
 // *prevKeyCount: the previous step's sorted unique pair keys, replaced on
-// return.  prevContacts: the previous step's contacts (canonical order), the
+// return.  prevContacts: the previous step's contacts (cache order), the
 // source of the representative of a pair whose bodies both went inert.
 // Returns the event count (events beyond cap are counted, not written).
 uint32_t vxo_contact_events(const vx_body* bodies, uint32_t n, const vx_contact* cs, uint32_t m, float dt,

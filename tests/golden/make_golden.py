@@ -38,6 +38,10 @@ def main():
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "toiquery"], cwd=ROOT)
     # Origin shifts in the middle of a lattice and of a heightfield terrain trajectory.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "shift"], cwd=ROOT)
+    # The crowded regime of a large pile from step 1: a 35,000-contact island,
+    # ~700-900 colours (states, counts and the last step's events only).
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "slabstack"],
+                          cwd=ROOT)
     # The 1,000-box lattice with sleeping on over 300 steps (several minutes).
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "lattice1000_sleep"],
                           cwd=ROOT)

@@ -237,6 +237,78 @@ inline Scene pile(uint32_t count = 240, float X = 6.0f, float Y = 8.0f, uint32_t
     return s;
 }
 
+// Axis-aligned stack of boxes (half-extent `h`, faces 0.5 mm apart, so every
+// neighbour pair is inside the 1 mm speculative slop at rest) with its
+// lowest layer 0.5 mm above the plane, centred at (cx, cz).
+inline void boxStack(Scene& s, int nx, int ny, int nz, float h, float cx, float cz, float mass) {
+    const float gap = 5e-4f, pitch = 2.0f * h + gap;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                BodySpec b;
+                b.kind = Box;
+                b.half[0] = b.half[1] = b.half[2] = h;
+                b.mass = mass;
+                b.pos[0] = cx + pitch * ((float)i - 0.5f * (float)(nx - 1));
+                b.pos[1] = h + gap + pitch * (float)j;
+                b.pos[2] = cz + pitch * ((float)k - 0.5f * (float)(nz - 1));
+                b.friction = 0.6f;
+                b.restitution = 0.0f;
+                s.bodies.push_back(b);
+            }
+}
+
+// The crowded regime of a large settled pile, reached at step 1: one island
+// of about 35,000 contacts (a 24 x 5 x 24 stack of 0.5 m boxes), a 6 m slab
+// resting on its top layer with a 10 x 10 grid of spheres on the slab (the
+// slab touches ~680 contacts, so the greedy colouring runs to ~700 colours,
+// far past the 64 colours of one mask word), a separate 8 x 4 x 8 stack of
+// about 2,800 contacts, and 64 lone spheres on the plane.  Friction 0.6,
+// restitution 0.
+inline Scene slabStack() {
+    Scene s;
+    s.bodies.push_back(plane(0.6f, 0.0f));
+    const float h = 0.25f, gap = 5e-4f, pitch = 2.0f * h + gap;
+    boxStack(s, 24, 5, 24, h, 0.0f, 0.0f, 125.0f);
+    const float top = gap + 5.0f * pitch - gap;  // upper face of the fifth layer
+    BodySpec slab;
+    slab.kind = Box;
+    slab.half[0] = 3.0f; slab.half[1] = 0.15f; slab.half[2] = 3.0f;
+    slab.mass = 500.0f;
+    slab.pos[1] = top + gap + slab.half[1];
+    slab.friction = 0.6f;
+    slab.restitution = 0.0f;
+    s.bodies.push_back(slab);
+    const float slabTop = slab.pos[1] + slab.half[1];
+    for (int i = 0; i < 10; ++i)
+        for (int k = 0; k < 10; ++k) {
+            BodySpec b;
+            b.kind = Sphere;
+            b.radius = 0.2f;
+            b.mass = 33.5f;
+            b.pos[0] = 0.55f * ((float)i - 4.5f);
+            b.pos[1] = slabTop + gap + b.radius;
+            b.pos[2] = 0.55f * ((float)k - 4.5f);
+            b.friction = 0.6f;
+            b.restitution = 0.0f;
+            s.bodies.push_back(b);
+        }
+    boxStack(s, 8, 4, 8, h, 20.0f, 0.0f, 125.0f);
+    for (int i = 0; i < 64; ++i) {
+        BodySpec b;
+        b.kind = Sphere;
+        b.radius = 0.3f;
+        b.mass = 113.0f;
+        b.pos[0] = -20.0f + 1.0f * (float)(i % 8);
+        b.pos[1] = b.radius + gap;
+        b.pos[2] = 1.0f * (float)(i / 8) - 3.5f;
+        b.friction = 0.6f;
+        b.restitution = 0.0f;
+        s.bodies.push_back(b);
+    }
+    return s;
+}
+
 // Static heightfield terrain (value-noise octaves) with mixed bodies above it.
 inline Scene terrain(uint32_t side = 24, float cell = 0.5f, float amp = 1.0f, uint32_t count = 96,
                      uint32_t seed = 909u) {
