@@ -71,7 +71,20 @@ typedef struct {
     uint32_t multiToiEvents;   /* StepStats::multiToiEvents (processHighToi events) */
     uint32_t ccdFlagged;       /* pairs whose end-of-step gap the net flagged */
     uint32_t ccdRelevant;      /* pairs the ordered net walked (flagged and what can follow them) */
+    uint32_t warmTies;         /* contacts whose warm-start match a tie among a pair's cached contacts
+                                  decided (the cache was then put in the reference's std::sort order) */
+    uint32_t launches[6];      /* launches of the last step by kernel, indexed by VXHIP_LAUNCH_* */
 } vxhip_stats;
+
+/* Kernel launch counts in vxhip_stats.launches (one step). */
+enum {
+    VXHIP_LAUNCH_BATCH = 0,         /* kBatchSweeps: island batches in LDS (warm start and sweeps) */
+    VXHIP_LAUNCH_ISLAND = 1,        /* kIslandSweeps: one island of up to the island limit per workgroup */
+    VXHIP_LAUNCH_COLOR_PAIR = 2,    /* kSweepColorPair: one big-island colour over the device */
+    VXHIP_LAUNCH_COLOR_RUN = 3,     /* kSweepColorRun: a run of narrow colours in one workgroup */
+    VXHIP_LAUNCH_TAIL = 4,          /* kSweepTailPair: the colour tail with LDS-resident velocities */
+    VXHIP_LAUNCH_GREEDY_DEVICE = 5  /* kGreedyColorDevice: the colouring dataflow across the device */
+};
 
 /* ---------------------------------------------------------------- setup -- */
 
@@ -351,6 +364,15 @@ int vxhip_probe_color(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const v
 /* Warm-start matching of `cur` against `prev` (sorted by pair key), in place
  * (world.cpp:4133-4184). */
 int vxhip_probe_warm_start(vxhip_ctx* ctx, vx_contact* cur, uint32_t m, const vx_contact* prev, uint32_t mp);
+
+/* std::sort (libstdc++) of (key, value) pairs by key, reproduced exactly on
+ * the device, in place: the order equal keys are left in is std::sort's.
+ * With segStart (segments + 1 non-decreasing offsets, 0 to n), each segment
+ * is sorted as its own std::sort call would sort it.  The device step orders
+ * its warm-start cache this way, as World::step does with std::sort
+ * (world.cpp:4518-4521). */
+int vxhip_probe_std_sort(vxhip_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart,
+                         uint32_t segments);
 
 /* ------------------------------------------------------ synthetic scenes -- */
 /* Host utility: builds body records and flat geometry for the synthetic

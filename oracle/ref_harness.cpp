@@ -946,12 +946,12 @@ int fixture(const std::string& out, const std::string& name) {
         substepsCapture(out, "substeps_joints", 61);
         return 0;
     }
-    if (name == "slabstack") {  // a 35,000-contact island, ~700 colours (states only: the records are large)
+    if (name == "slabstack") {  // a 55,000+-contact island, ~700 colours (states only: the records are large)
         trajectory(out, name, vxscene::slabStack(), {1, 2, 3}, false, false, false, true);
         return 0;
     }
-    if (name == "nslab") {  // per-stage snapshots of the slab stack (debugging aid)
-        narrowSnapshots(out, name, vxscene::slabStack(), {2});
+    if (name == "nslab") {  // per-stage snapshots of a slab on one layer of boxes (~680 colours)
+        narrowSnapshots(out, name, vxscene::slabStack(true), {2});
         return 0;
     }
     if (name == "hightoi") {  // MotionQuality::High bodies through processHighToi

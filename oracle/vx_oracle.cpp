@@ -386,6 +386,15 @@ uint32_t vxo_find_contacts(const vx_body* b, uint32_t n, const vx_soup* s, float
     for (uint32_t k = 0; k < cs.size() && k < cap; ++k) out[k] = cs[k];
     return (uint32_t)cs.size();
 }
+// The host libstdc++'s std::sort of (key, value) pairs by key (the sort
+// world.cpp:4518-4521 orders the warm-start cache with).
+void vxo_std_sort(uint64_t* keys, uint32_t* vals, uint32_t n) {
+    struct E { uint64_t k; uint32_t v; };
+    std::vector<E> e(n);
+    for (uint32_t i = 0; i < n; ++i) e[i] = {keys[i], vals[i]};
+    std::sort(e.begin(), e.end(), [](const E& x, const E& y) { return x.k < y.k; });
+    for (uint32_t i = 0; i < n; ++i) { keys[i] = e[i].k; vals[i] = e[i].v; }
+}
 void vxo_warm_start_match(vx_contact* cur, uint32_t m, const vx_contact* prev, uint32_t mp) {
     warmStartMatch(cur, m, prev, mp);
 }

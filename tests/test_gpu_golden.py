@@ -61,7 +61,7 @@ def test_trajectory_matches_reference(ctx, name):
             assert np.array_equal(contact_bytes(ctx.contacts()), contact_bytes(want_c))
 
 
-NARROW = ["nlattice_0", "nlattice_45", "npile_40", "nterrain_30"]
+NARROW = ["nlattice_0", "nlattice_45", "npile_40", "nterrain_30", "nslab_2"]
 
 
 @pytest.mark.parametrize("name", NARROW)

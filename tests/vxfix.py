@@ -105,6 +105,7 @@ def oracle():
             "vxo_collide_pair": (i32, [vp, vp, u32, u32, f32, vp]),
             "vxo_find_contacts": (u32, [vp, u32, vp, f32, vp, u32]),
             "vxo_warm_start_match": (None, [vp, u32, vp, u32]),
+            "vxo_std_sort": (None, [vp, vp, u32]),
             "vxo_color": (u32, [vp, u32, vp, u32, vp]),
             "vxo_solve": (None, [vp, vp, u32, f32, i32, i32, vp]),
             "vxo_position_correction": (None, [vp, vp, u32, f32, vp]),

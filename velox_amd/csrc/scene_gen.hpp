@@ -259,18 +259,20 @@ inline void boxStack(Scene& s, int nx, int ny, int nz, float h, float cx, float 
 }
 
 // The crowded regime of a large settled pile, reached at step 1: one island
-// of about 35,000 contacts (a 24 x 5 x 24 stack of 0.5 m boxes), a 6 m slab
+// of 55,000-90,000 contacts (a 24 x 5 x 24 stack of 0.5 m boxes), a 6 m slab
 // resting on its top layer with a 10 x 10 grid of spheres on the slab (the
 // slab touches ~680 contacts, so the greedy colouring runs to ~700 colours,
-// far past the 64 colours of one mask word), a separate 8 x 4 x 8 stack of
-// about 2,800 contacts, and 64 lone spheres on the plane.  Friction 0.6,
-// restitution 0.
-inline Scene slabStack() {
+// far past the 64 colours of one mask word), a separate 8 x 3 x 8 stack of
+// about 3,300 contacts, and 64 lone spheres on the plane.  Friction 0.6,
+// restitution 0.  `small`: a 12 x 1 x 12 layer under the slab and spheres
+// only (the slab still takes ~680 colours), for per-stage snapshots.
+inline Scene slabStack(bool small = false) {
     Scene s;
     s.bodies.push_back(plane(0.6f, 0.0f));
     const float h = 0.25f, gap = 5e-4f, pitch = 2.0f * h + gap;
-    boxStack(s, 24, 5, 24, h, 0.0f, 0.0f, 125.0f);
-    const float top = gap + 5.0f * pitch - gap;  // upper face of the fifth layer
+    const int layers = small ? 1 : 5;
+    boxStack(s, small ? 12 : 24, layers, small ? 12 : 24, h, 0.0f, 0.0f, 125.0f);
+    const float top = gap + (float)layers * pitch - gap;  // upper face of the top layer
     BodySpec slab;
     slab.kind = Box;
     slab.half[0] = 3.0f; slab.half[1] = 0.15f; slab.half[2] = 3.0f;
@@ -293,7 +295,8 @@ inline Scene slabStack() {
             b.restitution = 0.0f;
             s.bodies.push_back(b);
         }
-    boxStack(s, 8, 4, 8, h, 20.0f, 0.0f, 125.0f);
+    if (small) return s;
+    boxStack(s, 8, 3, 8, h, 20.0f, 0.0f, 125.0f);
     for (int i = 0; i < 64; ++i) {
         BodySpec b;
         b.kind = Sphere;

@@ -1,6 +1,5 @@
 // Device kernels of the HIP device step (one body, pair or contact per lane).
 #pragma once
-#include "vx_stdsort.hpp"
 #include "vxd_solver.hpp"
 
 namespace vxk {
@@ -489,6 +488,91 @@ __global__ __launch_bounds__(kThreads) void kWarmMatch(vx_contact* __restrict__ 
     cur[k] = c;
 }
 
+// The reference matches against its cache in the order std::sort leaves it
+// (world.cpp:4518-4521), and std::sort is not stable: the first of several
+// cached contacts with the contact's feature key, or the first of several at
+// exactly the nearest distance, depends on that order.  Counts the contacts
+// whose match such a tie decides (the cache here is in another order among a
+// pair's contacts); the host then puts the cache in the exact order first.
+__global__ __launch_bounds__(kThreads) void kWarmTies(const vx_contact* __restrict__ cur, uint32_t m,
+                                                      const vx_contact* __restrict__ prev, uint32_t mp,
+                                                      uint32_t* __restrict__ ties) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    bool tie = false;
+    if (k < m && mp) {
+        const vx_contact c = cur[k];
+        const uint64_t key = pairKeyOf(c);
+        uint32_t lo = 0, hi = mp;
+        while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (pairKeyOf(prev[mid]) < key) lo = mid + 1; else hi = mid; }
+        uint32_t same = 0;
+        if (c.featureKey != 0)
+            for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t) same += prev[t].featureKey == c.featureKey;
+        if (same > 1) {
+            tie = true;
+        } else if (same == 0) {
+            float best = 2.5e-3f;
+            uint32_t atBest = 0;
+            for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t) {
+                const float d = lengthSq(v3(prev[t].point) - v3(c.point));
+                if (d < best) { best = d; atBest = 1; }
+                else if (d == best && atBest) ++atBest;
+            }
+            tie = atBest > 1;
+        }
+    }
+    const uint64_t any = __ballot(tie);
+    if (any && (threadIdx.x & 63) == 0) atomicAdd(ties, (uint32_t)__popcll(any));
+}
+
+// Cache contacts (canonical order) with their positions in the solve order
+// they came from: (pair key, canonical index) at that position.
+__global__ __launch_bounds__(kThreads) void kPrevSolveKeys(const vx_contact* __restrict__ prev, uint32_t mp,
+                                                           const uint32_t* __restrict__ rank, int nb,
+                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    uint32_t j = blockIdx.x * kThreads + threadIdx.x;
+    if (j >= mp) return;
+    const uint32_t r = rank[j];
+    keys[r] = ((uint64_t)prev[j].a << nb) | prev[j].b;
+    vals[r] = j;
+}
+
+// rank[order[i]] = i: the solve position of each canonical contact.
+__global__ __launch_bounds__(kThreads) void kRankBack(const uint32_t* __restrict__ order, uint32_t m,
+                                                      uint32_t* __restrict__ rank) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < m) rank[order[i]] = i;
+}
+
+// out[t] = in[vals[t]].
+__global__ __launch_bounds__(kThreads) void kGatherBy(const vx_contact* __restrict__ in, const uint32_t* __restrict__ vals,
+                                                      uint32_t n, vx_contact* __restrict__ out) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t < n) out[t] = in[vals[t]];
+}
+
+// World of each entry (by the first body of its pair key), for a stable
+// world-major arrangement; per-world entry counts.
+__global__ __launch_bounds__(kThreads) void kEntryWorlds(const uint64_t* __restrict__ keys, uint32_t n, int nb,
+                                                         const uint32_t* __restrict__ worldOf,
+                                                         uint64_t* __restrict__ wkeys, uint32_t* __restrict__ idx,
+                                                         uint32_t* __restrict__ worldCount) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t w = worldOf[(uint32_t)(keys[i] >> nb)];
+    wkeys[i] = w;
+    idx[i] = i;
+    atomicAdd(&worldCount[w], 1u);
+}
+
+__global__ __launch_bounds__(kThreads) void kPermuteKV(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
+                                                       const uint32_t* __restrict__ idx, uint32_t n,
+                                                       uint64_t* __restrict__ keysOut, uint32_t* __restrict__ valsOut) {
+    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    if (i >= n) return;
+    keysOut[i] = keys[idx[i]];
+    valsOut[i] = vals[idx[i]];
+}
+
 // --------------------------------------------------------------- coloring --
 __device__ inline bool constrainedSide(const vx_body* bodies, const vx_contact& c, int side) {
     bool sensor = isSensor(bodies[c.a]) || isSensor(bodies[c.b]);
@@ -891,60 +975,25 @@ __global__ __launch_bounds__(kThreads) void kCarryFlags(const vx_contact* __rest
     carry[t] = keep ? 1u : 0u;
 }
 
-// Sort keys of the cache candidates: this step's contacts (canonical order)
-// then the previous cache, whose entries that are not carried sort last.
-__global__ __launch_bounds__(kThreads) void kCacheKeys(const vx_contact* __restrict__ cur, uint32_t m,
-                                                       const vx_contact* __restrict__ prev, uint32_t mp, int nb,
-                                                       const uint32_t* __restrict__ carry, uint64_t none,
-                                                       uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+// The cache candidates in the order the reference hands them to std::sort
+// (world.cpp:4496-4521): this step's contacts in solve order, then the
+// carried ones in cache order; (pair key, source) with sources below m
+// naming canonical contacts of this step and m + t the cached contact t.
+__global__ __launch_bounds__(kThreads) void kCacheSolveKeys(const vx_contact* __restrict__ solve,
+                                                            const uint32_t* __restrict__ order, uint32_t m,
+                                                            const vx_contact* __restrict__ prev, uint32_t mp, int nb,
+                                                            const uint32_t* __restrict__ carry,
+                                                            const uint32_t* __restrict__ carryRank,
+                                                            uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     uint32_t j = blockIdx.x * kThreads + threadIdx.x;
-    if (j >= m + mp) return;
-    const bool own = j < m;
-    const vx_contact& c = own ? cur[j] : prev[j - m];
-    keys[j] = own || carry[j - m] ? ((uint64_t)c.a << nb) | c.b : none;
-    vals[j] = j;
-}
-
-// In a step where a body fell asleep, some pair may turn inert with several
-// contacts cached; the order std::sort leaves them in then decides what the
-// next steps report and keep.  The cache candidates are put in the order the
-// reference hands to std::sort (this step's contacts in solve order, then the
-// carried ones in cache order) and sorted by the libstdc++ restatement: one
-// thread, in LDS when they fit.
-constexpr uint32_t kStdSortLds = 10240;  // 120 KB of (key, value) pairs
-__global__ __launch_bounds__(kThreads) void kCacheStdSort(const vx_contact* __restrict__ solve,
-                                                          const uint32_t* __restrict__ order, uint32_t m,
-                                                          const vx_contact* __restrict__ prev, uint32_t mp, int nb,
-                                                          const uint32_t* __restrict__ carry,
-                                                          const uint32_t* __restrict__ carryRank,
-                                                          const uint32_t* __restrict__ carried,
-                                                          const uint32_t* __restrict__ fellAsleep,
-                                                          uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    __shared__ uint64_t lk[kStdSortLds];
-    __shared__ uint32_t lw[kStdSortLds];
-    if (*fellAsleep == 0) return;
-    const uint32_t n = m + *carried;
-    const bool lds = n <= kStdSortLds;
-    uint64_t* K = lds ? lk : keys;
-    uint32_t* V = lds ? lw : vals;
-    for (uint32_t j = threadIdx.x; j < m; j += kThreads) {
-        K[j] = ((uint64_t)solve[j].a << nb) | solve[j].b;
-        V[j] = order[j];
+    if (j < m) {
+        keys[j] = ((uint64_t)solve[j].a << nb) | solve[j].b;
+        vals[j] = order[j];
+    } else if (j < m + mp && carry[j - m]) {
+        const uint32_t t = j - m, r = m + carryRank[t];
+        keys[r] = ((uint64_t)prev[t].a << nb) | prev[t].b;
+        vals[r] = m + t;
     }
-    for (uint32_t t = threadIdx.x; t < mp; t += kThreads)
-        if (carry[t]) {
-            const uint32_t r = m + carryRank[t];
-            K[r] = ((uint64_t)prev[t].a << nb) | prev[t].b;
-            V[r] = m + t;
-        }
-    __syncthreads();
-    if (threadIdx.x == 0) vxsort::sort({K, V}, n);
-    __syncthreads();
-    if (lds)
-        for (uint32_t j = threadIdx.x; j < n; j += kThreads) {
-            keys[j] = K[j];
-            vals[j] = V[j];
-        }
 }
 
 __global__ __launch_bounds__(kThreads) void kCacheGather(const vx_contact* __restrict__ cur, uint32_t m,

@@ -8,7 +8,9 @@
 //    Stability per pass is what makes the multi-pass LSD sort correct.
 //  * Exclusive scan of u32 counts: one workgroup up to 16384 counts, else
 //    tile sums, scan of tile sums, add-back.
+#include <algorithm>
 #include "vx_internal.hpp"
+#include "vx_stdsort.hpp"
 
 namespace vxk {
 
@@ -196,7 +198,172 @@ __global__ __launch_bounds__(kScanOneThreads) void kScanOne(const uint32_t* in, 
     if (t == kScanOneThreads - 1 && total) *total = waveOffset + x;
 }
 
+// ------------------------------------------------ exact std::sort order --
+// libstdc++'s std::sort (introsort: median-of-three pivot moved to the front,
+// unguarded Hoare partition of the rest, 16-element threshold, heap sort
+// after 2*floor(log2 n) levels, final insertion sort) leaves equal keys in an
+// order that depends on the whole input.  The pending ranges of the
+// introsort touch disjoint elements, so they are processed level by level,
+// one workgroup per range, and each Hoare partition is computed from ranks:
+// with L_k the k-th position (from the left) whose key is not below the
+// pivot and R_k the k-th position (from the right) whose key is not above
+// it, the partition swaps (L_k, R_k) exactly for the k with L_k < R_k (a
+// prefix, since L rises and R falls) and returns min(L_s, R_{s-1}), s the
+// number of swaps.  Between two swaps each scanner only reads elements no
+// earlier swap has touched, which is what makes the ranks of the input
+// enough.  After the levels, every range of at most 16 elements lies between
+// smaller and larger ranges, so the final insertion sort moves elements only
+// within their range: a stable insertion sort per range finishes the order.
+constexpr int kIntroThreads = 256;
+constexpr int kIntroItems = 4;
+constexpr uint32_t kIntroTile = kIntroThreads * kIntroItems;
+constexpr uint32_t kIntroSorted = 0x80000000u;  // leaf already sorted (heap sort)
+
+struct IntroSeg {
+    uint32_t first, last, depth, pad;
+};
+
+// Exclusive scan of one packed u32 per thread over the workgroup.
+__device__ inline uint32_t introScan(uint32_t v, uint32_t* waveSums, uint32_t& total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) waveSums[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kIntroThreads / 64; ++w) {
+        const uint32_t t = waveSums[w];
+        before += w < wave ? t : 0u;
+        all += t;
+    }
+    __syncthreads();
+    total = all;
+    return before + x - v;
+}
+
+__device__ inline void introEmit(uint32_t a, uint32_t b, uint32_t depth, IntroSeg* next, uint32_t* nextCount,
+                                 uint2* leaves, uint32_t* leafCount) {
+    if (b - a > vxsort::kThreshold) {
+        next[atomicAdd(nextCount, 1u)] = IntroSeg{a, b, depth, 0u};
+    } else if (b - a > 1) {
+        leaves[atomicAdd(leafCount, 1u)] = make_uint2(a, b);
+    }
+}
+
+__global__ void kIntroClear(uint32_t* counts, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counts[i] = 0u;
+}
+
+// Level 0: one range per segment, with std::sort's depth limit for its size.
+__global__ void kIntroInit(const uint32_t* __restrict__ segStart, uint32_t segments, uint32_t n,
+                           IntroSeg* __restrict__ level0, uint32_t* __restrict__ count0,
+                           uint2* __restrict__ leaves, uint32_t* __restrict__ leafCount) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= segments) return;
+    const uint32_t f = segStart ? segStart[w] : 0u, l = segStart ? segStart[w + 1] : n;
+    if (l <= f) return;
+    introEmit(f, l, 2u * vxsort::floorLog2(l - f), level0, count0, leaves, leafCount);
+}
+
+__global__ __launch_bounds__(kIntroThreads) void kIntroLevel(uint64_t* __restrict__ K, uint32_t* __restrict__ V,
+                                                             const IntroSeg* __restrict__ segs,
+                                                             const uint32_t* __restrict__ segCount,
+                                                             IntroSeg* __restrict__ next, uint32_t* __restrict__ nextCount,
+                                                             uint32_t* __restrict__ posL, uint32_t* __restrict__ posR,
+                                                             uint2* __restrict__ leaves, uint32_t* __restrict__ leafCount) {
+    __shared__ uint32_t waveSums[kIntroThreads / 64];
+    __shared__ uint64_t pivotS;
+    __shared__ uint32_t swapsS;
+    const vxsort::Seq seq{K, V};
+    const uint32_t count = *segCount;
+    for (uint32_t si = blockIdx.x; si < count; si += gridDim.x) {
+        const IntroSeg g = segs[si];
+        const uint32_t f = g.first, l = g.last;
+        if (g.depth == 0) {  // __partial_sort(first, last, last)
+            if (threadIdx.x == 0) {
+                vxsort::heapSort(seq, f, l);
+                leaves[atomicAdd(leafCount, 1u)] = make_uint2(f, l | kIntroSorted);
+            }
+            __syncthreads();
+            continue;
+        }
+        if (threadIdx.x == 0) {  // __move_median_to_first(first, first + 1, mid, last - 1)
+            vxsort::medianToFirst(seq, f, f + 1, f + (l - f) / 2, l - 1);
+            pivotS = K[f];
+        }
+        __syncthreads();
+        const uint64_t P = pivotS;
+        // Stops of both scanners over [f + 1, l), in increasing position.
+        uint32_t cl = 0, cr = 0;
+        for (uint32_t base = f + 1; base < l; base += kIntroTile) {
+            const uint32_t p0 = base + threadIdx.x * kIntroItems;
+            uint32_t fl = 0, fr = 0;
+#pragma unroll
+            for (int j = 0; j < kIntroItems; ++j) {
+                const uint32_t p = p0 + j;
+                if (p < l) {
+                    const uint64_t k = K[p];
+                    fl |= (k < P ? 0u : 1u) << j;
+                    fr |= (P < k ? 0u : 1u) << j;
+                }
+            }
+            uint32_t total;
+            const uint32_t ex = introScan((uint32_t)__popc(fl) | ((uint32_t)__popc(fr) << 16), waveSums, total);
+            uint32_t rl = cl + (ex & 0xffffu), rr = cr + (ex >> 16);
+#pragma unroll
+            for (int j = 0; j < kIntroItems; ++j) {
+                if ((fl >> j) & 1u) posL[f + rl++] = p0 + j;
+                if ((fr >> j) & 1u) posR[f + rr++] = p0 + j;
+            }
+            cl += total & 0xffffu;
+            cr += total >> 16;
+        }
+        __syncthreads();
+        // s = #{k : L_k < R_k}, R_k = posR[f + cr - 1 - k]: a prefix of k.
+        if (threadIdx.x == 0) {
+            uint32_t lo = 0, hi = cl < cr ? cl : cr;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) / 2;
+                if (posL[f + mid] < posR[f + cr - 1 - mid]) lo = mid + 1;
+                else hi = mid;
+            }
+            swapsS = lo;
+        }
+        __syncthreads();
+        const uint32_t swaps = swapsS;
+        for (uint32_t k = threadIdx.x; k < swaps; k += kIntroThreads)
+            seq.swap(posL[f + k], posR[f + cr - 1 - k]);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t cut = swaps ? posR[f + cr - swaps] : l;
+            if (swaps < cl && posL[f + swaps] < cut) cut = posL[f + swaps];
+            introEmit(f, cut, g.depth - 1, next, nextCount, leaves, leafCount);
+            introEmit(cut, l, g.depth - 1, next, nextCount, leaves, leafCount);
+        }
+        __syncthreads();
+    }
+}
+
+// The final insertion sort, range by range (stable insertion sort).
+__global__ void kIntroLeaves(uint64_t* __restrict__ K, uint32_t* __restrict__ V, const uint2* __restrict__ leaves,
+                             const uint32_t* __restrict__ leafCount) {
+    const vxsort::Seq seq{K, V};
+    const uint32_t count = *leafCount;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+        const uint2 r = leaves[i];
+        if (r.y & kIntroSorted) continue;
+        vxsort::insertionSort(seq, r.x, r.y);
+    }
+}
+
 }  // namespace vxk
+
 
 namespace vxhost {
 
@@ -261,4 +428,55 @@ hipError_t exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t
     return hipGetLastError();
 }
 
+namespace {
+struct IntroLayout {
+    uint32_t maxSegs, levels;
+    size_t posL, posR, segA, segB, counts, leaves, bytes;
+};
+IntroLayout introLayout(uint32_t n, uint32_t segments) {
+    IntroLayout L{};
+    L.maxSegs = std::max<uint32_t>(segments, n / (vxsort::kThreshold + 1)) + 2;
+    L.levels = 2 * vxsort::floorLog2(n ? n : 1) + 2;
+    size_t at = 0;
+    auto take = [&](size_t bytes) { const size_t p = at; at += (bytes + 255) & ~(size_t)255; return p; };
+    L.posL = take((size_t)n * 4);
+    L.posR = take((size_t)n * 4);
+    L.segA = take((size_t)L.maxSegs * sizeof(IntroSeg));
+    L.segB = take((size_t)L.maxSegs * sizeof(IntroSeg));
+    L.counts = take((size_t)(L.levels + 2) * 4);
+    L.leaves = take((size_t)(n / 2 + segments + 2) * sizeof(uint2));
+    L.bytes = at;
+    return L;
+}
+}  // namespace
+
+size_t stdSortTempBytes(uint32_t n, uint32_t segments) { return introLayout(n, segments).bytes; }
+
+// Orders (keys, vals) exactly as std::sort would, segment by segment.
+hipError_t stdSort(uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart, uint32_t segments,
+                   void* temp, hipStream_t s) {
+    if (n <= 1 || segments == 0) return hipSuccess;
+    const IntroLayout L = introLayout(n, segments);
+    char* t = (char*)temp;
+    uint32_t* posL = (uint32_t*)(t + L.posL);
+    uint32_t* posR = (uint32_t*)(t + L.posR);
+    IntroSeg* seg[2] = {(IntroSeg*)(t + L.segA), (IntroSeg*)(t + L.segB)};
+    uint32_t* counts = (uint32_t*)(t + L.counts);  // per level, then the leaf count
+    uint2* leaves = (uint2*)(t + L.leaves);
+    uint32_t* leafCount = counts + L.levels + 1;
+    const uint32_t nc = L.levels + 2;
+    kIntroClear<<<(nc + 255) / 256, 256, 0, s>>>(counts, nc);
+    kIntroInit<<<(segments + 255) / 256, 256, 0, s>>>(segStart, segments, n, seg[0], counts, leaves, leafCount);
+    for (uint32_t level = 0; level + 1 < L.levels; ++level) {
+        const uint64_t wide = (uint64_t)segments << std::min<uint32_t>(level, 24);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(wide, L.maxSegs), 4096);
+        kIntroLevel<<<grid, kIntroThreads, 0, s>>>(keys, vals, seg[level & 1], counts + level, seg[(level + 1) & 1],
+                                                   counts + level + 1, posL, posR, leaves, leafCount);
+    }
+    const uint32_t leafGrid = std::min<uint32_t>((n / 2 + segments + 257) / 256, 2048);
+    kIntroLeaves<<<leafGrid, 256, 0, s>>>(keys, vals, leaves, leafCount);
+    return hipGetLastError();
+}
+
 }  // namespace vxhost
+

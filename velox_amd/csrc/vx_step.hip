@@ -156,6 +156,13 @@ struct vxhip_ctx {
     Buf<vx_contact> prevStage, prevNext;
     Buf<uint32_t> carry, carryRank, cacheVals, cacheValsTmp;
     Buf<uint64_t> cacheKeys, cacheKeysTmp;
+    // The warm-start cache is matched in the order std::sort leaves it.
+    // prevExact: `prev` is in that order; otherwise it is canonical and
+    // prevRank holds each entry's position in the solve order it came from.
+    bool prevExact = false;
+    Buf<uint32_t> prevRank;
+    Buf<uint64_t> exWorld, exWorldTmp, exKeys;
+    Buf<uint32_t> exIdx, exIdxTmp, exVals, exCount, exStart;
     Buf<float4> sVel, sAng, sInvI, sFrame, sRows, sImp;
     Buf<uint4> sMeta, sMetaK;
     Buf<float4> sSide;
@@ -232,7 +239,7 @@ enum Counter : uint32_t {
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
     cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55,
-    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cCount = 64
+    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cWarmTies = 64, cCount = 72
 };
 static_assert(cCount <= 128, "kResetCounters runs one 128-lane workgroup");
 
@@ -296,11 +303,11 @@ void collectProfile(vxhip_ctx* c) {
     c->timings.steps += 1;
 }
 
-// Resets the per-stage counters; the sticky error word cBatchError is
-// cleared only at the start of a step (keepSticky false).
+// Resets the per-stage counters; the sticky words (cBatchError, cWarmTies)
+// are cleared only at the start of a step (keepSticky false).
 __global__ void kResetCounters(uint32_t* k, bool keepSticky = true) {
     uint32_t t = threadIdx.x;
-    if (t >= cCount || (keepSticky && t == cBatchError)) return;
+    if (t >= cCount || (keepSticky && (t == cBatchError || t == cWarmTies))) return;
     uint32_t v = 0;
     if (t >= cBounds && t < cBounds + 3) v = 0x7fffffffu;       // min cell
     else if (t >= cBounds + 3 && t < cBounds + 6) v = 0x80000000u;  // max cell
@@ -551,10 +558,12 @@ uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     if (m <= kColorSingleLimit)
         kGreedyColor<<<1, kColorThreads, 0, s>>>(c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p,
                                                  c->color.p, c->colorCount.p, k + cColors, k + cColorError);
-    else
+    else {
+        ++c->stats.launches[VXHIP_LAUNCH_GREEDY_DEVICE];
         kGreedyColorDevice<<<std::min<uint32_t>(blocks(m), kColorBlocks), kThreads, 0, s>>>(
             c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p, c->color.p, c->colorCount.p, k + cColors,
             k + cColorError);
+    }
     HIPCK(hipGetLastError());
     readCounters(c);
     if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: coloring did not converge");
@@ -722,6 +731,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
             // Contacts visited: every contact of the LDS batches, once per sweep.
             const uint64_t visits = (uint64_t)c->ldsContactCount * sweeps;
             solveLaunchBegin(c);
+            ++c->stats.launches[VXHIP_LAUNCH_BATCH];
             kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, s>>>(
                 sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
                 c->bodyList.p, c->counters.p + cRanges, sweeps, c->counters.p + cBatchError);
@@ -729,6 +739,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
         }
         if (global) {
             solveLaunchBegin(c);
+            ++c->stats.launches[VXHIP_LAUNCH_ISLAND];
             kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, s>>>(
                 sa, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
             solveLaunchEnd(c, VXHIP_KERNEL_ISLAND, (uint64_t)(c->smallCount - c->ldsContactCount) * sweeps);
@@ -742,6 +753,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 if (!cnt) { ++col; continue; }
                 if (kMode != 2 && col == c->tailColor) {
                     solveLaunchBegin(c);
+                    ++c->stats.launches[VXHIP_LAUNCH_TAIL];
                     kSweepTailPair<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, K,
                                                                          c->tailRowSlot.p, c->tailSlotBody.p,
                                                                          c->counters.p + cTailSlots);
@@ -750,6 +762,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 }
                 if (cnt > kNarrowSweepColor) {
                     solveLaunchBegin(c);
+                    ++c->stats.launches[VXHIP_LAUNCH_COLOR_PAIR];
                     kSweepColorPair<kMode><<<blocks(2 * cnt), kThreads, 0, bs>>>(sa, c->smallCount, cs[col], cnt);
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
                     ++col;
@@ -759,6 +772,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 while (end < K && (kMode == 2 || end != c->tailColor) && cs[end + 1] - cs[end] <= kNarrowSweepColor)
                     ++end;
                 solveLaunchBegin(c);
+                ++c->stats.launches[VXHIP_LAUNCH_COLOR_RUN];
                 kSweepColorRun<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, end);
                 solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[end] - cs[col]);
                 col = end;
@@ -1114,6 +1128,50 @@ void sleepUpdate(vxhip_ctx* c, uint32_t m, float dt) {
     HIPCK(hipGetLastError());
 }
 
+// Orders `n` (pair key, value) entries, given in the order the reference
+// hands them to std::sort, exactly as that std::sort leaves them: one
+// std::sort per world (a stable world-major arrangement first when worlds
+// are batched), by the parallel restatement of vx_sort.hip.
+void exactOrder(vxhip_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t n) {
+    if (n <= 1) return;
+    hipStream_t s = c->stream;
+    const bool batched = c->worldCount > 1 && c->worldN == c->n;
+    const uint32_t W = batched ? c->worldCount : 1u;
+    const uint32_t* segStart = nullptr;
+    if (batched) {
+        const int nb = bodyBits(c->n);
+        c->exWorld.reserve(n); c->exWorldTmp.reserve(n); c->exIdx.reserve(n); c->exIdxTmp.reserve(n);
+        c->exKeys.reserve(n); c->exVals.reserve(n); c->exCount.reserve(W); c->exStart.reserve(W + 1);
+        fill32(c, c->exCount.p, W, 0);
+        kEntryWorlds<<<blocks(n), kThreads, 0, s>>>(keys, n, nb, c->worldOf.p, c->exWorld.p, c->exIdx.p, c->exCount.p);
+        HIPCK(vxhost::radixSort(c->exWorld.p, c->exIdx.p, c->exWorldTmp.p, c->exIdxTmp.p, n,
+                                vxhost::bitsFor(W - 1), tempFor(c, vxhost::radixTempBytes(n)), s));
+        HIPCK(vxhost::exclusiveScan(c->exCount.p, c->exStart.p, W, c->exStart.p + W,
+                                    tempFor(c, vxhost::scanTempBytes(W)), s));
+        kPermuteKV<<<blocks(n), kThreads, 0, s>>>(keys, vals, c->exIdx.p, n, c->exKeys.p, c->exVals.p);
+        HIPCK(hipMemcpyAsync(keys, c->exKeys.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s));
+        HIPCK(hipMemcpyAsync(vals, c->exVals.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        segStart = c->exStart.p;
+    }
+    HIPCK(vxhost::stdSort(keys, vals, n, segStart, W, tempFor(c, vxhost::stdSortTempBytes(n, W)), s));
+}
+
+// Puts the canonical warm-start cache into the reference's order: the order
+// std::sort leaves the previous step's contacts in, taken in solve order.
+void exactPrev(vxhip_ctx* c) {
+    const uint32_t mp = c->prevCount;
+    hipStream_t s = c->stream;
+    const int nb = bodyBits(c->n);
+    c->cacheKeys.reserve(mp + 1); c->cacheVals.reserve(mp + 1); c->prevNext.reserve(mp + 1);
+    kPrevSolveKeys<<<blocks(mp), kThreads, 0, s>>>(c->prev.p, mp, c->prevRank.p, nb, c->cacheKeys.p, c->cacheVals.p);
+    exactOrder(c, c->cacheKeys.p, c->cacheVals.p, mp);
+    kGatherBy<<<blocks(mp), kThreads, 0, s>>>(c->prev.p, c->cacheVals.p, mp, c->prevNext.p);
+    HIPCK(hipGetLastError());
+    std::swap(c->prev.p, c->prevNext.p);
+    std::swap(c->prev.cap, c->prevNext.cap);
+    c->prevExact = true;
+}
+
 // The warm-start cache for the next step (world.cpp:4492-4521): this step's
 // contacts plus the first cached contact of every pair kept by an inert
 // Persist event, ordered by pair key; in a step where a body fell asleep, in
@@ -1126,23 +1184,21 @@ void buildCache(vxhip_ctx* c, uint32_t m) {
     c->prevStage.reserve(m ? m : 1);
     if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prevStage.p);
     c->carry.reserve(mp + 1); c->carryRank.reserve(mp + 1);
-    c->cacheKeys.reserve(total + 1); c->cacheKeysTmp.reserve(total + 1);
-    c->cacheVals.reserve(total + 1); c->cacheValsTmp.reserve(total + 1);
+    c->cacheKeys.reserve(total + 1); c->cacheVals.reserve(total + 1);
     if (mp) {
         kCarryFlags<<<blocks(mp), kThreads, 0, s>>>(c->prev.p, mp, nb, c->evAppKeys.p, k + cEvInert, c->carry.p);
         HIPCK(vxhost::exclusiveScan(c->carry.p, c->carryRank.p, mp, k + cCarried, tempFor(c, vxhost::scanTempBytes(mp)), s));
     } else {
         kFill32<<<1, 64, 0, s>>>(k + cCarried, 1, 0u);
     }
+    // Ordered as std::sort orders them; the carried count is read first (a
+    // sleeping world is small, so the round trip is cheap).
     if (total) {
-        const uint64_t none = 1ull << (2 * nb);
-        kCacheKeys<<<blocks(total), kThreads, 0, s>>>(c->prevStage.p, m, c->prev.p, mp, nb, c->carry.p, none,
-                                                      c->cacheKeys.p, c->cacheVals.p);
-        HIPCK(vxhost::radixSort(c->cacheKeys.p, c->cacheVals.p, c->cacheKeysTmp.p, c->cacheValsTmp.p, total, 2 * nb + 1,
-                                tempFor(c, vxhost::radixTempBytes(total)), s));
-        kCacheStdSort<<<1, kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p, mp, nb, c->carry.p,
-                                             c->carryRank.p, k + cCarried, k + cSlDynamic + 4, c->cacheKeys.p,
-                                             c->cacheVals.p);
+        kCacheSolveKeys<<<blocks(total), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p, mp, nb, c->carry.p,
+                                                           c->carryRank.p, c->cacheKeys.p, c->cacheVals.p);
+        HIPCK(hipMemcpyAsync(c->hostCounters + cCarried, k + cCarried, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        exactOrder(c, c->cacheKeys.p, c->cacheVals.p, m + c->hostCounters[cCarried]);
     }
     c->prevNext.reserve(total ? total : 1);
     if (total)
@@ -1155,6 +1211,7 @@ void finishCache(vxhip_ctx* c, uint32_t m) {
     std::swap(c->prev.p, c->prevNext.p);
     std::swap(c->prev.cap, c->prevNext.cap);
     c->prevCount = m + c->hostCounters[cCarried];
+    c->prevExact = true;
 }
 
 void finishSleep(vxhip_ctx* c) {
@@ -1185,6 +1242,7 @@ void worldStep(vxhip_ctx* c, float dt) {
     hipStream_t s = c->stream;
     c->prevPos.reserve(n); c->prevQ.reserve(n);
     uint32_t* k = c->counters.p;
+    for (uint32_t& l : c->stats.launches) l = 0;
     markStage(c, sBegin);
     kResetCounters<<<1, 128, 0, s>>>(k, false);
     kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
@@ -1199,9 +1257,17 @@ void worldStep(vxhip_ctx* c, float dt) {
     const uint32_t m = jointContactsOff(c, np, narrowPhase(c, np, dt));
     markStage(c, sMatch);
     if (c->anyAsleep && m) kWakePass<<<1, kWakeThreads, 0, s>>>(c->bodies.p, c->contacts.p, m);
-    if (m && c->prevCount) kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
+    // Ties of the match are counted here and read with the colouring's
+    // counters (colours do not depend on impulses); the match itself runs
+    // after, against a cache in the reference's exact order when a tie needs it.
+    const bool tieCheck = m && c->prevCount && !c->prevExact;
+    if (tieCheck) kWarmTies<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount, k + cWarmTies);
     markStage(c, sColor);
     const uint32_t K = colorContacts(c, c->contacts.p, m);
+    if (m && c->prevCount) {
+        if (tieCheck && c->hostCounters[cWarmTies]) exactPrev(c);
+        kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
+    }
     colorOrder(c, m, K);
     c->solve.reserve(m ? m : 1);
     if (m) kGather<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, c->order.p, m, c->solve.p);
@@ -1237,7 +1303,9 @@ void worldStep(vxhip_ctx* c, float dt) {
         buildCache(c, m);
     } else {
         c->prev.reserve(m ? m : 1);
+        c->prevRank.reserve(m ? m : 1);
         if (m) kScatterBack<<<blocks(m), kThreads, 0, s>>>(c->solve.p, c->order.p, m, c->prev.p);
+        if (m) kRankBack<<<blocks(m), kThreads, 0, s>>>(c->order.p, m, c->prevRank.p);
     }
     HIPCK(hipGetLastError());
     c->solveCount = m;
@@ -1248,8 +1316,13 @@ void worldStep(vxhip_ctx* c, float dt) {
     if (c->hostCounters[cColorError]) fail(VXHIP_ERR_DEVICE_LOST, "vxhip: CCD dataflow did not converge");
     if (events) finishEvents(c);
     if (!c->eventsEnabled) c->eventCount = 0;
-    if (carry) finishCache(c, m);
-    else c->prevCount = m;
+    if (carry) {
+        finishCache(c, m);
+    } else {
+        c->prevCount = m;
+        c->prevExact = false;
+    }
+    c->stats.warmTies = c->hostCounters[cWarmTies];
     if (c->sleepOn) finishSleep(c);
     c->jointBreakCount = 0;
     if (c->jointCount) {
@@ -1358,6 +1431,7 @@ int vxhip_world_load(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_s
         uploadSoup(c, soup);
         setBodies(c, bodies, n);
         c->prevCount = 0;
+        c->prevExact = false;
         c->solveCount = 0;
         c->worldCount = 1;
         c->worldN = 0;
@@ -1587,8 +1661,11 @@ constexpr uint32_t kStateMagic = 0x54535856u;  // "VXST"
 size_t stateBytes(const vxhip_ctx* c) {
     return sizeof(StateHeader) + (size_t)c->n * sizeof(vx_body) + (size_t)c->prevCount * sizeof(vx_contact) +
            (size_t)c->evPrevCount * sizeof(uint64_t) + (size_t)c->eventCount * sizeof(vx_contact_event) +
-           (size_t)c->solveCount * sizeof(vx_contact);
+           (size_t)c->solveCount * sizeof(vx_contact) + (c->prevExact ? 0 : (size_t)c->prevCount * sizeof(uint32_t));
 }
+// Version 2: the cache's order flag in `reserved` (bit 0: in the reference's
+// std::sort order), and, for a canonical cache, each entry's solve position.
+constexpr uint32_t kStateVersion = 2u;
 }  // namespace
 
 int vxhip_world_state_size(vxhip_ctx* c, size_t* bytes) {
@@ -1604,7 +1681,8 @@ int vxhip_world_save_state(vxhip_ctx* c, void* out, size_t cap) {
         if (cap < stateBytes(c)) fail(VXHIP_ERR_CAPACITY, "vxhip: state buffer too small");
         hipStream_t s = c->stream;
         uint8_t* p = (uint8_t*)out;
-        const StateHeader h{kStateMagic, 1u, c->n, c->prevCount, c->evPrevCount, c->eventCount, c->solveCount, 0u};
+        const StateHeader h{kStateMagic, kStateVersion, c->n, c->prevCount, c->evPrevCount, c->eventCount,
+                            c->solveCount, c->prevExact ? 1u : 0u};
         std::memcpy(p, &h, sizeof h);
         p += sizeof h;
         auto down = [&](const void* src, size_t bytes) {
@@ -1616,6 +1694,7 @@ int vxhip_world_save_state(vxhip_ctx* c, void* out, size_t cap) {
         down(c->evPrevKeys.p, (size_t)c->evPrevCount * sizeof(uint64_t));
         down(c->events.p, (size_t)c->eventCount * sizeof(vx_contact_event));
         down(c->solve.p, (size_t)c->solveCount * sizeof(vx_contact));
+        if (!c->prevExact) down(c->prevRank.p, (size_t)c->prevCount * sizeof(uint32_t));
         HIPCK(hipStreamSynchronize(s));
     });
 }
@@ -1625,11 +1704,14 @@ int vxhip_world_restore_state(vxhip_ctx* c, const void* in, size_t size) {
         if (!c || !c->loaded || !in || size < sizeof(StateHeader)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
         StateHeader h;
         std::memcpy(&h, in, sizeof h);
-        if (h.magic != kStateMagic || h.version != 1u) fail(VXHIP_ERR_INVALID, "vxhip: not a saved world state");
+        if (h.magic != kStateMagic || h.version != kStateVersion)
+            fail(VXHIP_ERR_INVALID, "vxhip: not a saved world state");
+        const bool exact = (h.reserved & 1u) != 0;
         if (h.n != c->n) fail(VXHIP_ERR_INVALID, "vxhip: saved state belongs to a world of another size");
         const size_t want = sizeof h + (size_t)h.n * sizeof(vx_body) + (size_t)h.prevCount * sizeof(vx_contact) +
                             (size_t)h.evPrevCount * sizeof(uint64_t) + (size_t)h.eventCount * sizeof(vx_contact_event) +
-                            (size_t)h.solveCount * sizeof(vx_contact);
+                            (size_t)h.solveCount * sizeof(vx_contact) +
+                            (exact ? 0 : (size_t)h.prevCount * sizeof(uint32_t));
         if (size != want) fail(VXHIP_ERR_INVALID, "vxhip: saved state has the wrong size");
         const uint8_t* p = (const uint8_t*)in + sizeof h;
         std::vector<vx_body> bodies(h.n);
@@ -1646,6 +1728,8 @@ int vxhip_world_restore_state(vxhip_ctx* c, const void* in, size_t size) {
         up(c->evPrevKeys, h.evPrevCount, sizeof(uint64_t));
         up(c->events, h.eventCount, sizeof(vx_contact_event));
         up(c->solve, h.solveCount, sizeof(vx_contact));
+        if (!exact) up(c->prevRank, h.prevCount, sizeof(uint32_t));
+        c->prevExact = exact;
         c->prevCount = h.prevCount;
         c->evPrevCount = h.evPrevCount;
         c->eventCount = h.eventCount;
@@ -1741,6 +1825,7 @@ int vxhip_invalidate_caches(vxhip_ctx* c) {
         if (!c) fail(VXHIP_ERR_INVALID, "vxhip: null context");
         c->solveCount = 0;
         c->prevCount = 0;
+        c->prevExact = false;
         for (size_t& s : c->soupSig) s = (size_t)-1;
     });
 }
@@ -1835,6 +1920,32 @@ int vxhip_probe_warm_start(vxhip_ctx* c, vx_contact* cur, uint32_t m, const vx_c
         HIPCK(hipMemcpyAsync(cur, c->contacts.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
         c->prevCount = 0;
+        c->prevExact = false;
+    });
+}
+
+int vxhip_probe_std_sort(vxhip_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart,
+                         uint32_t segments) {
+    return guarded([&] {
+        if (!c || (n && (!keys || !vals)) || (segStart && segments == 0)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (segStart) {
+            for (uint32_t w = 0; w < segments; ++w)
+                if (segStart[w] > segStart[w + 1]) fail(VXHIP_ERR_INVALID, "vxhip: segment starts must not decrease");
+            if (segStart[0] != 0 || segStart[segments] != n) fail(VXHIP_ERR_INVALID, "vxhip: segments must cover [0, n)");
+        }
+        if (n <= 1) return;
+        hipStream_t s = c->stream;
+        const uint32_t W = segStart ? segments : 1u;
+        c->exKeys.reserve(n); c->exVals.reserve(n); c->exStart.reserve(W + 1);
+        HIPCK(hipMemcpyAsync(c->exKeys.p, keys, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(c->exVals.p, vals, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        if (segStart)
+            HIPCK(hipMemcpyAsync(c->exStart.p, segStart, (size_t)(W + 1) * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        HIPCK(vxhost::stdSort(c->exKeys.p, c->exVals.p, n, segStart ? c->exStart.p : nullptr, W,
+                              tempFor(c, vxhost::stdSortTempBytes(n, W)), s));
+        HIPCK(hipMemcpyAsync(keys, c->exKeys.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(vals, c->exVals.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
     });
 }
 

@@ -33,10 +33,17 @@ class Stats(C.Structure):
                 ("oversize", C.c_uint32), ("cellSize", C.c_float), ("islands", C.c_uint32),
                 ("islandContacts", C.c_uint32), ("bigIslandContacts", C.c_uint32),
                 ("largestIsland", C.c_uint32), ("maxBatchPasses", C.c_uint32), ("multiToiEvents", C.c_uint32),
-                ("ccdFlagged", C.c_uint32), ("ccdRelevant", C.c_uint32)]
+                ("ccdFlagged", C.c_uint32), ("ccdRelevant", C.c_uint32), ("warmTies", C.c_uint32),
+                ("launches", C.c_uint32 * 6)]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["launches"] = dict(zip(LAUNCH_KINDS, self.launches))
+        return d
+
+
+# vxhip_stats.launches indices (VXHIP_LAUNCH_*)
+LAUNCH_KINDS = ("batch", "island", "color_pair", "color_run", "tail", "greedy_device")
 
 
 class SleepConfig(C.Structure):
@@ -119,6 +126,7 @@ SIGNATURES = {
     "vxhip_probe_pairs": (_i32, [_vp, _vp, _u32, _vp, _f32, _vp, _u32, _vp]),
     "vxhip_probe_color": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "vxhip_probe_warm_start": (_i32, [_vp, _vp, _u32, _vp, _u32]),
+    "vxhip_probe_std_sort": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
     "vxhip_world_profile": (_i32, [_vp, _i32]),
     "vxhip_world_timings": (_i32, [_vp, _vp]),
     "vxhip_scene_build": (_i32, [C.c_char_p, _u32, _vp]),
@@ -454,8 +462,19 @@ class Context:
                                           ptr(colors), C.byref(k)))
         return colors[:len(contacts)], k.value
 
+    def std_sort(self, keys, vals, seg_start=None):
+        """(keys, vals) in the order libstdc++'s std::sort leaves them (per
+        segment when seg_start is given), computed on the device."""
+        keys = np.ascontiguousarray(keys, np.uint64).copy()
+        vals = np.ascontiguousarray(vals, np.uint32).copy()
+        seg = None if seg_start is None else np.ascontiguousarray(seg_start, np.uint32)
+        check(library().vxhip_probe_std_sort(self._h, ptr(keys), ptr(vals), len(keys),
+                                             ptr(seg) if seg is not None else None,
+                                             0 if seg is None else len(seg) - 1))
+        return keys, vals
+
     def warm_start_match(self, cur, prev):
-        cur = np.ascontiguousarray(cur).copy() if False else np.ascontiguousarray(cur)
+        cur = np.ascontiguousarray(cur)
         prev = np.ascontiguousarray(prev)
         check(library().vxhip_probe_warm_start(self._h, ptr(cur), len(cur), ptr(prev), len(prev)))
         return cur
