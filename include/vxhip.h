@@ -349,6 +349,53 @@ typedef struct {
 int vxhip_world_profile(vxhip_ctx* ctx, int enable);
 int vxhip_world_timings(vxhip_ctx* ctx, vxhip_timings* out);
 
+/* ------------------------------------------------- differentiable form -- */
+/* Recording of the frozen active set: while on, each vxhip_world_step keeps
+ * its contacts in solve order with the warm-start impulses they started
+ * from, and for each the previous step's solve position whose final
+ * impulses it took (0xffffffff: none); vxhip_world_download_record copies
+ * the last step's (count first; nothing is copied when cap is too small).
+ * No reference counterpart: the record is the differentiable form's. */
+int vxhip_world_record(vxhip_ctx* ctx, int enable);
+int vxhip_world_download_record(vxhip_ctx* ctx, vx_contact* contacts, uint32_t* match, uint32_t cap, uint32_t* count);
+
+
+/* Gradient rollouts of the resident world (no reference counterpart: the
+ * differentiable form is this build's own).  The rollout is the ordinary
+ * device step; its gradients come from hand-written adjoint kernels of the
+ * substepped contact solve, the transform advance and the positional
+ * correction, with the active set frozen per step (each step's contacts,
+ * their geometry, colours and warm-start matches are the rollout's; the
+ * branches the solve took are replayed).  A checkpoint of the state a step
+ * reads from the one before is kept on the device every `every` steps; the
+ * backward replays one segment at a time.  Admitted worlds: dynamic bodies of
+ * isotropic inertia (spheres, cubes), geometric-mean friction without rolling
+ * or spinning resistance, no sleeping, joints or multi-TOI bodies, the
+ * two-axis friction model, each batched world's bodies contiguous
+ * (VXHIP_ERR_UNAVAILABLE otherwise).
+ *
+ * vxhip_diff_forward: `steps` device steps from the current state; `loss`
+ * (one per world, may be NULL) receives |c_w - t_w|^2 for the mass-weighted
+ * centre c_w of world w's dynamic bodies after the last step and the target
+ * t_w = targets[3w..3w+2]. */
+int vxhip_diff_forward(vxhip_ctx* ctx, float dt, uint32_t steps, uint32_t every, const float* targets, float* loss);
+
+typedef struct {
+    uint32_t steps, checkpoints, worlds;
+    uint32_t ccdRescued;      /* pairs the CCD safety net rescued during the rollout: not differentiated */
+    uint32_t replayMismatch;  /* body-steps whose tape replay differed from the rollout (0: bit for bit) */
+    uint32_t maxContacts;     /* largest contact count of a step */
+} vxhip_diff_stats;
+
+/* Gradients of sum_w lossWeight[w] * loss_w with respect to the rollout's
+ * initial linear and angular velocities (gV, gW: 3 per body), friction
+ * coefficients (gFriction) and masses (gMass: through the inverse mass, the
+ * inverse inertia it scales and the centres of mass), for every body (zero
+ * for static ones).  Leaves the world at the rollout's end state with an
+ * empty warm-start cache.  Any output may be NULL. */
+int vxhip_diff_backward(vxhip_ctx* ctx, const float* lossWeight, float* gV, float* gW, float* gFriction, float* gMass,
+                        vxhip_diff_stats* stats);
+
 /* ------------------------------------------------------- stage probes -- */
 /* Single stages on caller data, for parity checks of one stage at a time. */
 

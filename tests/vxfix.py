@@ -106,6 +106,7 @@ def oracle():
             "vxo_find_contacts": (u32, [vp, u32, vp, f32, vp, u32]),
             "vxo_warm_start_match": (None, [vp, u32, vp, u32]),
             "vxo_std_sort": (None, [vp, vp, u32]),
+            "vxo_diff_rollout": (C.c_double, [vp, u32, vp, vp, vp, u32, f32, i32, i32, i32, vp, vp, vp, vp, vp]),
             "vxo_color": (u32, [vp, u32, vp, u32, vp]),
             "vxo_solve": (None, [vp, vp, u32, f32, i32, i32, vp]),
             "vxo_position_correction": (None, [vp, vp, u32, f32, vp]),

@@ -384,6 +384,7 @@ vxscene::Scene vxSceneByName(const std::string& name, uint32_t param, bool& ok) 
     if (name == "settle") return vxscene::settle();
     if (name == "hightoi") return vxscene::highToi();
     if (name == "slabstack") return vxscene::slabStack();
+    if (name == "diff") return vxscene::diffWorlds(param ? param : 1024);
     // The exact descriptions behind the golden trajectory fixtures.
     if (name == "fixture-pile") return vxscene::pile(160, 5.0f, 6.0f);
     if (name == "fixture-terrain") return vxscene::terrain(16, 0.5f, 0.8f, 40);

@@ -419,6 +419,44 @@ inline Scene batchWorlds(uint32_t worlds = 4096, uint32_t seed = kSeedBatch) {
     return s;
 }
 
+// Differentiable worlds: `worlds` worlds, each a static plane (friction 0.6)
+// and 64 cubes (half-extent 0.2 m) in an 8 x 8 grid at 0.6 m pitch resting
+// 0.5 mm above it, with mass U(0.5, 5) kg, friction U(0.2, 1.0) (geometric-mean
+// combine) and restitution 0, each pushed by an initial impulse: linear
+// velocity (U(0.5, 1.5), 0, U(-0.25, 0.25)) m/s and a spin about the vertical
+// of U(-0.5, 0.5) rad/s.  The cubes slide and stop within about 0.35 m, so
+// neighbours meet gently if at all (no pair comes near the CCD net's
+// penetration threshold).  Cubes have isotropic inertia, which the adjoint
+// assumes.
+inline Scene diffWorlds(uint32_t worlds = 1024, uint32_t boxes = 64, uint32_t seed = kSeedDiff) {
+    Scene s;
+    s.worlds = worlds;
+    Rng r(seed);
+    const uint32_t side = (uint32_t)std::ceil(std::sqrt((float)boxes));
+    for (uint32_t w = 0; w < worlds; ++w) {
+        BodySpec pl = plane(0.6f, 0.0f);
+        pl.world = w;
+        s.bodies.push_back(pl);
+        for (uint32_t k = 0; k < boxes; ++k) {
+            BodySpec b;
+            b.kind = Box;
+            b.world = w;
+            b.half[0] = b.half[1] = b.half[2] = 0.2f;
+            b.mass = r.range(0.5f, 5.0f);
+            b.friction = r.range(0.2f, 1.0f);
+            b.restitution = 0.0f;
+            b.pos[0] = 0.6f * ((float)(k % side) - 0.5f * (float)(side - 1));
+            b.pos[1] = 0.2f + 5e-4f;
+            b.pos[2] = 0.6f * ((float)(k / side) - 0.5f * (float)(side - 1));
+            b.vel[0] = r.range(0.5f, 1.5f);
+            b.vel[2] = r.range(-0.25f, 0.25f);
+            b.angVel[1] = r.range(-0.5f, 0.5f);
+            s.bodies.push_back(b);
+        }
+    }
+    return s;
+}
+
 // Sleep and wake: eight towers of three unit boxes (1 mm gaps, U(-0.02,0.02)
 // x/z jitter) and eight resting spheres (r = 0.4) on a plane settle and fall
 // asleep island by island; the middle box of tower 7 has sleeping disabled, so

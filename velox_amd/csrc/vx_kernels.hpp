@@ -458,8 +458,13 @@ __device__ inline void copyImpulses(vx_contact& c, const vx_contact& p) {
 }
 
 // world.cpp:4133-4184: feature-key match, vn0 carry, nearest point within 5 cm.
+// With `prevPos` (each cache entry's position in the previous step's solve
+// order), `match` receives the position of the entry whose impulses the
+// contact took, or ~0 (the differentiable form's record of the warm start).
 __global__ __launch_bounds__(kThreads) void kWarmMatch(vx_contact* __restrict__ cur, uint32_t m,
-                                                       const vx_contact* __restrict__ prev, uint32_t mp) {
+                                                       const vx_contact* __restrict__ prev, uint32_t mp,
+                                                       const uint32_t* __restrict__ prevPos = nullptr,
+                                                       uint32_t* __restrict__ match = nullptr) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= m || mp == 0) return;
     vx_contact c = cur[k];
@@ -467,12 +472,14 @@ __global__ __launch_bounds__(kThreads) void kWarmMatch(vx_contact* __restrict__ 
     uint32_t lo = 0, hi = mp;
     while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (pairKeyOf(prev[mid]) < key) lo = mid + 1; else hi = mid; }
     bool matched = false;
+    uint32_t from = 0xffffffffu;
     if (c.featureKey != 0)
         for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t)
             if (prev[t].featureKey == c.featureKey) {
                 copyImpulses(c, prev[t]);
                 if (c.vn0 < -1.0f && prev[t].vn0 < c.vn0) c.vn0 = prev[t].vn0;
                 matched = true;
+                from = t;
                 break;
             }
     if (!matched) {
@@ -482,10 +489,11 @@ __global__ __launch_bounds__(kThreads) void kWarmMatch(vx_contact* __restrict__ 
         float best = 2.5e-3f;
         for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t) {
             float d = lengthSq(v3(prev[t].point) - v3(c.point));
-            if (d < best) { best = d; copyImpulses(c, prev[t]); }
+            if (d < best) { best = d; copyImpulses(c, prev[t]); from = t; }
         }
     }
     cur[k] = c;
+    if (match) match[k] = from == 0xffffffffu ? from : prevPos[from];
 }
 
 // The reference matches against its cache in the order std::sort leaves it
@@ -546,6 +554,13 @@ __global__ __launch_bounds__(kThreads) void kRankBack(const uint32_t* __restrict
 // out[t] = in[vals[t]].
 __global__ __launch_bounds__(kThreads) void kGatherBy(const vx_contact* __restrict__ in, const uint32_t* __restrict__ vals,
                                                       uint32_t n, vx_contact* __restrict__ out) {
+    uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (t < n) out[t] = in[vals[t]];
+}
+
+// out[t] = in[vals[t]] for u32 words.
+__global__ __launch_bounds__(kThreads) void kGatherU32(const uint32_t* __restrict__ in, const uint32_t* __restrict__ vals,
+                                                       uint32_t n, uint32_t* __restrict__ out) {
     uint32_t t = blockIdx.x * kThreads + threadIdx.x;
     if (t < n) out[t] = in[vals[t]];
 }

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 #include "vx_ccd.hpp"
@@ -12,6 +13,7 @@
 #include "vx_sleep.hpp"
 #include "vx_toi.hpp"
 #include "vx_joints.hpp"
+#include "vx_diff.hpp"
 #include "vx_internal.hpp"
 #include "../../include/vxhip.h"
 
@@ -99,6 +101,25 @@ struct Buf {
 };
 
 inline uint32_t blocks(uint64_t n, int t = kThreads) { return (uint32_t)((n + t - 1) / t); }
+
+// One step of a differentiable rollout as its adjoint replays it.
+struct DiffStepRec {
+    uint32_t m = 0, prevCount = 0, ccdRescued = 0;
+    Buf<vx_body> start;     // bodies at the step's start
+    Buf<vx_contact> cs;     // contacts in solve order, with their warm-start impulses
+    Buf<uint32_t> match;    // per contact: the previous step's solve position it took impulses from, or ~0
+    Buf<uint32_t> colorOf;  // per contact: colour
+    Buf<uint32_t> matchCanon;
+};
+
+// The state a step reads from the one before it: bodies and warm-start cache.
+struct DiffCheckpoint {
+    uint32_t prevCount = 0;
+    bool prevExact = false;
+    Buf<vx_body> bodies;
+    Buf<vx_contact> prev;
+    Buf<uint32_t> prevRank;
+};
 
 }  // namespace
 
@@ -229,6 +250,29 @@ struct vxhip_ctx {
     std::vector<uint64_t> launchContacts;
     std::vector<int> launchKind;
     vxhip_timings timings{};
+    // Differentiable rollouts (vxhip_diff_*): the step being recorded, the
+    // checkpoints of the last rollout and the adjoint's buffers.
+    DiffStepRec* diffCap = nullptr;
+    bool recordOn = false;  // vxhip_world_record: the last step's frozen data kept for download
+    DiffStepRec record;
+    uint32_t diffSteps = 0, diffEvery = 8, diffW = 0;
+    float diffDt = 0.0f;
+    bool diffReady = false;
+    std::vector<std::unique_ptr<DiffCheckpoint>> diffCps;
+    std::vector<std::unique_ptr<DiffStepRec>> diffSeg;
+    std::vector<uint32_t> diffBodyStartHost;
+    Buf<uint32_t> diffBodyStart;
+    Buf<float> diffTarget, diffLoss, diffLossBar, diffAdjMass, diffG;
+    Buf<vx_body> diffFinal, diffWork, diffPost;
+    Buf<vx_body> diffSub[8], diffPre[8];  // per substep: bodies before the solve, and before the transform
+    Buf<float4> diffRows[8], diffTape[8], diffImp0;
+    Buf<float2> diffCTape;
+    Buf<float4> diffAdjX, diffAdjQ, diffAdjV, diffAdjW, diffAdjParam, diffLamBar, diffLamPrev, diffRowBar;
+    Buf<float4> dVel, dAng, dInvI, dFrame, dImp, dSide;
+    Buf<uint4> dMeta;
+    Buf<uint32_t> dIota, dList, dListTmp, dWStart, dWCount;
+    Buf<uint64_t> dWKeys, dWKeysTmp;
+    Buf<uint32_t> diffOverflow;
 };
 
 namespace {
@@ -1158,17 +1202,22 @@ void exactOrder(vxhip_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t n) {
 
 // Puts the canonical warm-start cache into the reference's order: the order
 // std::sort leaves the previous step's contacts in, taken in solve order.
+// prevRank follows its entries (each keeps its solve position).
 void exactPrev(vxhip_ctx* c) {
     const uint32_t mp = c->prevCount;
     hipStream_t s = c->stream;
     const int nb = bodyBits(c->n);
     c->cacheKeys.reserve(mp + 1); c->cacheVals.reserve(mp + 1); c->prevNext.reserve(mp + 1);
+    c->carryRank.reserve(mp + 1);
     kPrevSolveKeys<<<blocks(mp), kThreads, 0, s>>>(c->prev.p, mp, c->prevRank.p, nb, c->cacheKeys.p, c->cacheVals.p);
     exactOrder(c, c->cacheKeys.p, c->cacheVals.p, mp);
     kGatherBy<<<blocks(mp), kThreads, 0, s>>>(c->prev.p, c->cacheVals.p, mp, c->prevNext.p);
+    kGatherU32<<<blocks(mp), kThreads, 0, s>>>(c->prevRank.p, c->cacheVals.p, mp, c->carryRank.p);
     HIPCK(hipGetLastError());
     std::swap(c->prev.p, c->prevNext.p);
     std::swap(c->prev.cap, c->prevNext.cap);
+    std::swap(c->prevRank.p, c->carryRank.p);
+    std::swap(c->prevRank.cap, c->carryRank.cap);
     c->prevExact = true;
 }
 
@@ -1245,6 +1294,12 @@ void worldStep(vxhip_ctx* c, float dt) {
     for (uint32_t& l : c->stats.launches) l = 0;
     markStage(c, sBegin);
     kResetCounters<<<1, 128, 0, s>>>(k, false);
+    DiffStepRec* cap = c->diffCap ? c->diffCap : (c->recordOn ? &c->record : nullptr);
+    if (cap) {
+        cap->start.reserve(n);
+        HIPCK(hipMemcpyAsync(cap->start.p, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+        cap->prevCount = c->prevCount;
+    }
     kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h, c->prevPos.p, c->prevQ.p, 1);
     HIPCK(hipGetLastError());
     if (c->anyHigh && c->toiSettings.enabled && c->toiSettings.splitting && n) {  // world.cpp:4015-4025
@@ -1264,13 +1319,24 @@ void worldStep(vxhip_ctx* c, float dt) {
     if (tieCheck) kWarmTies<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount, k + cWarmTies);
     markStage(c, sColor);
     const uint32_t K = colorContacts(c, c->contacts.p, m);
+    if (cap) {
+        cap->matchCanon.reserve(m ? m : 1);
+        fill32(c, cap->matchCanon.p, m, 0xffffffffu);
+    }
     if (m && c->prevCount) {
         if (tieCheck && c->hostCounters[cWarmTies]) exactPrev(c);
-        kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount);
+        kWarmMatch<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount,
+                                                  cap ? c->prevRank.p : nullptr, cap ? cap->matchCanon.p : nullptr);
     }
     colorOrder(c, m, K);
     c->solve.reserve(m ? m : 1);
     if (m) kGather<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, c->order.p, m, c->solve.p);
+    if (cap && m) {  // the step as the adjoint replays it
+        cap->cs.reserve(m); cap->match.reserve(m); cap->colorOf.reserve(m);
+        HIPCK(hipMemcpyAsync(cap->cs.p, c->solve.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToDevice, s));
+        kGatherU32<<<blocks(m), kThreads, 0, s>>>(cap->matchCanon.p, c->order.p, m, cap->match.p);
+        kColorOfSolve<<<blocks(m), kThreads, 0, s>>>(c->color.p, c->order.p, m, cap->colorOf.p);
+    }
     c->solveCount = m;
     markStage(c, sSolve);
     if (c->jointCount) kJointWake<<<1, 1, 0, s>>>(c->bodies.p, c->joints.p, c->jointCount);  // backend_cuda.cu:704-708
@@ -1354,6 +1420,232 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->stats.largestIsland = m ? c->largestIsland : 0;
     c->stats.maxBatchPasses = m ? c->maxBatchPasses : 0;
     c->stats.multiToiEvents = c->toiEventCount;
+    if (cap) {
+        cap->m = m;
+        cap->ccdRescued = c->hostCounters[cCcdRescued];
+    }
+}
+
+// -------------------------------------------------- differentiable form --
+// Rollouts of the resident world whose gradients the adjoint kernels of
+// vx_diff.hpp compute.  The forward runs the ordinary device step, keeping a
+// checkpoint of everything a step reads from the one before (bodies and the
+// warm-start cache) every `diffEvery` steps.  The backward goes over the
+// checkpoint segments from the last: it replays a segment with the ordinary
+// step, recording each step's contacts (solve order, warm-start impulses),
+// colours and warm-start matches; then, step by step from the segment's end,
+// it replays the step's substeps with the per-world tape kernels (the same
+// rows, so bit for bit the recorded trajectory) and runs the adjoint kernels
+// over them in reverse.
+constexpr uint32_t kDiffMaxSubsteps = 8;
+
+__global__ void kFillF4(float4* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+__global__ void kFillF(float* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+// Bodies whose replayed state (pose and velocities) differs from the
+// recorded one, bitwise.
+__global__ void kDiffCompare(const vx_body* __restrict__ a, const vx_body* __restrict__ b, uint32_t n,
+                             uint32_t* __restrict__ count) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t* x = reinterpret_cast<const uint32_t*>(&a[i]);
+    const uint32_t* y = reinterpret_cast<const uint32_t*>(&b[i]);
+    bool same = true;
+    for (int w = 0; w < 13; ++w) same &= x[w] == y[w];  // position, orientation, velocity, angular velocity
+    if (!same) atomicAdd(count, 1u);
+}
+
+void zeroF4(vxhip_ctx* c, Buf<float4>& b, uint32_t n) {
+    b.reserve(n ? n : 1);
+    if (n) kFillF4<<<blocks(n), kThreads, 0, c->stream>>>(b.p, n);
+}
+
+// What the differentiable form admits: dynamic bodies of isotropic inverse
+// inertia in their principal frame, geometric-mean friction, no rolling or
+// spinning resistance, no kinematic, locked, high-quality or sleeping bodies,
+// no joints, the two-axis friction model; worlds in contiguous body ranges.
+void diffValidate(vxhip_ctx* c) {
+    if (!c->loaded) fail(VXHIP_ERR_INVALID, "vxhip: no world is loaded");
+    if (c->sleepOn || c->jointCount || c->anyHigh)
+        fail(VXHIP_ERR_UNAVAILABLE, "vxhip: differentiable rollouts support no sleeping, joints or multi-TOI bodies");
+    if (c->cfg.options.frictionModel != VX_FRICTION_TWO_AXIS)
+        fail(VXHIP_ERR_UNAVAILABLE, "vxhip: differentiable rollouts support the two-axis friction model only");
+    if (c->cfg.substeps < 1 || (uint32_t)c->cfg.substeps > kDiffMaxSubsteps)
+        fail(VXHIP_ERR_UNAVAILABLE, "vxhip: differentiable rollouts support 1 to 8 substeps");
+    const uint32_t n = c->n;
+    std::vector<vx_body> b(n);
+    HIPCK(hipMemcpyAsync(b.data(), c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
+    std::vector<uint32_t> worldOf;
+    const bool batched = c->worldCount > 1 && c->worldN == n;
+    if (batched) {
+        worldOf.resize(n);
+        HIPCK(hipMemcpyAsync(worldOf.data(), c->worldOf.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    }
+    HIPCK(hipStreamSynchronize(c->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        const vx_body& x = b[i];
+        if (x.frictionCombine != 1 || x.rollingFriction != 0.0f || x.spinningFriction != 0.0f)
+            fail(VXHIP_ERR_UNAVAILABLE, "vxhip: differentiable rollouts need geometric-mean friction without rolling "
+                                        "or spinning resistance");
+        if (x.motionType == 0) continue;  // static
+        const bool iso = x.invInertia.x == x.invInertia.y && x.invInertia.y == x.invInertia.z &&
+                         x.inertiaOrientation.x == 0.0f && x.inertiaOrientation.y == 0.0f &&
+                         x.inertiaOrientation.z == 0.0f && x.inertiaOrientation.w == 1.0f;
+        if (x.motionType != 2 || !iso || x.fixedRotation || x.ccdTuning.quality == 3 || x.ccdTuning.quality == 2 ||
+            x.invMass <= 0.0f || x.asleep)
+            fail(VXHIP_ERR_UNAVAILABLE, "vxhip: differentiable rollouts need dynamic bodies of isotropic inertia "
+                                        "(spheres, cubes) of Medium or Low motion quality");
+    }
+    c->diffW = batched ? c->worldCount : 1u;
+    c->diffBodyStartHost.assign(c->diffW + 1, 0u);
+    if (batched) {
+        for (uint32_t i = 0; i < n; ++i) {
+            if (worldOf[i] >= c->diffW || (i && worldOf[i] < worldOf[i - 1]))
+                fail(VXHIP_ERR_UNAVAILABLE, "vxhip: differentiable rollouts need each world's bodies contiguous");
+            c->diffBodyStartHost[worldOf[i] + 1] = i + 1;
+        }
+        for (uint32_t w = 1; w <= c->diffW; ++w)
+            c->diffBodyStartHost[w] = std::max(c->diffBodyStartHost[w], c->diffBodyStartHost[w - 1]);
+    } else {
+        c->diffBodyStartHost[1] = n;
+    }
+    c->diffBodyStart.reserve(c->diffW + 1);
+    HIPCK(hipMemcpyAsync(c->diffBodyStart.p, c->diffBodyStartHost.data(), (c->diffW + 1) * sizeof(uint32_t),
+                         hipMemcpyHostToDevice, c->stream));
+}
+
+void diffSave(vxhip_ctx* c, DiffCheckpoint& cp) {
+    hipStream_t s = c->stream;
+    cp.bodies.reserve(c->n);
+    HIPCK(hipMemcpyAsync(cp.bodies.p, c->bodies.p, (size_t)c->n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+    cp.prevCount = c->prevCount;
+    cp.prevExact = c->prevExact;
+    cp.prev.reserve(c->prevCount ? c->prevCount : 1);
+    cp.prevRank.reserve(c->prevCount ? c->prevCount : 1);
+    if (c->prevCount) {
+        HIPCK(hipMemcpyAsync(cp.prev.p, c->prev.p, (size_t)c->prevCount * sizeof(vx_contact), hipMemcpyDeviceToDevice, s));
+        HIPCK(hipMemcpyAsync(cp.prevRank.p, c->prevRank.p, (size_t)c->prevCount * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, s));
+    }
+}
+
+void diffRestore(vxhip_ctx* c, const DiffCheckpoint& cp) {
+    hipStream_t s = c->stream;
+    HIPCK(hipMemcpyAsync(c->bodies.p, cp.bodies.p, (size_t)c->n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+    c->prev.reserve(cp.prevCount ? cp.prevCount : 1);
+    c->prevRank.reserve(cp.prevCount ? cp.prevCount : 1);
+    if (cp.prevCount) {
+        HIPCK(hipMemcpyAsync(c->prev.p, cp.prev.p, (size_t)cp.prevCount * sizeof(vx_contact), hipMemcpyDeviceToDevice, s));
+        HIPCK(hipMemcpyAsync(c->prevRank.p, cp.prevRank.p, (size_t)cp.prevCount * sizeof(uint32_t),
+                             hipMemcpyDeviceToDevice, s));
+    }
+    c->prevCount = cp.prevCount;
+    c->prevExact = cp.prevExact;
+}
+
+// The solve positions of one recorded step grouped by world (kDiffSweeps'
+// `list` and `wStart`).
+void diffGroup(vxhip_ctx* c, const DiffStepRec& r) {
+    hipStream_t s = c->stream;
+    const uint32_t m = r.m, W = c->diffW;
+    c->dList.reserve(m ? m : 1); c->dListTmp.reserve(m ? m : 1); c->dWStart.reserve(W + 1); c->dWCount.reserve(W);
+    if (!m) { fill32(c, c->dWStart.p, W + 1, 0u); return; }
+    if (W == 1) {
+        kIota<<<blocks(m), kThreads, 0, s>>>(c->dList.p, m);
+        const uint32_t ws[2] = {0u, m};
+        HIPCK(hipMemcpyAsync(c->dWStart.p, ws, sizeof ws, hipMemcpyHostToDevice, s));
+        HIPCK(hipStreamSynchronize(s));
+        return;
+    }
+    c->dWKeys.reserve(m); c->dWKeysTmp.reserve(m);
+    fill32(c, c->dWCount.p, W, 0u);
+    kDiffWorldKeys<<<blocks(m), kThreads, 0, s>>>(r.cs.p, m, c->worldOf.p, c->dWKeys.p, c->dList.p, c->dWCount.p);
+    HIPCK(vxhost::radixSort(c->dWKeys.p, c->dList.p, c->dWKeysTmp.p, c->dListTmp.p, m, vxhost::bitsFor(W - 1),
+                            tempFor(c, vxhost::radixTempBytes(m)), s));
+    HIPCK(vxhost::exclusiveScan(c->dWCount.p, c->dWStart.p, W, c->dWStart.p + W, tempFor(c, vxhost::scanTempBytes(W)), s));
+}
+
+// Reverse of one recorded step: on entry the adjoints hold those of the
+// step's end state and diffLamBar those of its contacts' final impulses; on
+// return, those of its start state, and diffLamPrev those of the previous
+// step's final impulses.
+void diffBackStep(vxhip_ctx* c, const DiffStepRec& r, const vx_body* next) {
+    hipStream_t s = c->stream;
+    const uint32_t n = c->n, m = r.m, W = c->diffW;
+    const int S = c->cfg.substeps;
+    const float h = c->diffDt / (float)S;
+    const V3 g{c->cfg.gravity[0], c->cfg.gravity[1], c->cfg.gravity[2]};
+    const vx_solver_options& opt = c->cfg.options;
+    const int sweeps = 2 * opt.velocityIterations;
+    uint32_t* over = c->diffOverflow.p;
+    diffGroup(c, r);
+    // Forward replay with the tape.
+    c->diffWork.reserve(n); c->diffPost.reserve(n);
+    c->dVel.reserve(n); c->dAng.reserve(n); c->dInvI.reserve(n); c->dFrame.reserve(n);
+    c->dMeta.reserve(m ? m : 1); c->dImp.reserve(2 * (size_t)(m ? m : 1)); c->dSide.reserve(8 * (size_t)(m ? m : 1));
+    c->dIota.reserve(m ? m : 1);
+    c->diffCTape.reserve((size_t)(m ? m : 1) * (opt.positionIterations > 0 ? opt.positionIterations : 1));
+    c->contactsTmp.reserve(m ? m : 1);
+    if (m) {
+        kIota<<<blocks(m), kThreads, 0, s>>>(c->dIota.p, m);
+        HIPCK(hipMemcpyAsync(c->contactsTmp.p, r.cs.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToDevice, s));
+    }
+    HIPCK(hipMemcpyAsync(c->diffWork.p, r.start.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+    kBeginStep<<<blocks(n), kThreads, 0, s>>>(c->diffWork.p, n, g, h, nullptr, nullptr, 1);
+    SolveArrays sa{c->dVel.p, c->dAng.p, c->dInvI.p, c->dFrame.p, nullptr, c->dMeta.p, c->dImp.p, c->dSide.p, c->dIota.p};
+    for (int st = 0; st < S; ++st) {
+        if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->diffWork.p, n, g, h);
+        c->diffSub[st].reserve(n); c->diffPre[st].reserve(n);
+        c->diffRows[st].reserve(8 * (size_t)(m ? m : 1));
+        c->diffTape[st].reserve(4 * (size_t)(sweeps + 1) * (m ? m : 1));
+        HIPCK(hipMemcpyAsync(c->diffSub[st].p, c->diffWork.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+        kGatherSolveBodies<<<blocks(n), kThreads, 0, s>>>(c->diffWork.p, n, sa);
+        sa.rows = c->diffRows[st].p;
+        if (m) {
+            kPrepareRows<<<blocks(m), kThreads, 0, s>>>(c->diffWork.p, c->contactsTmp.p, m, h, sa);
+            if (st == 0) kLoadImpulses<<<blocks(m), kThreads, 0, s>>>(r.cs.p, m, sa.imp, sa.pos);
+            kDiffSweeps<<<W, kDiffThreads, 0, s>>>(sa, c->dList.p, c->dWStart.p, r.colorOf.p, m, sweeps, st == 0,
+                                                   c->diffTape[st].p, over);
+            kScatterVelocities<<<blocks(n), kThreads, 0, s>>>(c->diffWork.p, n, sa);
+        }
+        HIPCK(hipMemcpyAsync(c->diffPre[st].p, c->diffWork.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+        kTransforms<<<blocks(n), kThreads, 0, s>>>(c->diffWork.p, n, h);
+    }
+    HIPCK(hipMemcpyAsync(c->diffPost.p, c->diffWork.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+    if (m && opt.positionIterations > 0)
+        kDiffCorrection<<<W, kDiffThreads, 0, s>>>(c->diffWork.p, c->contactsTmp.p, c->dList.p, c->dWStart.p,
+                                                   r.colorOf.p, m, opt.positionIterations, h, opt, c->diffCTape.p, over);
+    kDiffCompare<<<blocks(n), kThreads, 0, s>>>(c->diffWork.p, next, n, over + 1);
+    HIPCK(hipGetLastError());
+    // Reverse.
+    float4 *aX = c->diffAdjX.p, *aQ = c->diffAdjQ.p, *aV = c->diffAdjV.p, *aW = c->diffAdjW.p, *aP = c->diffAdjParam.p;
+    c->diffRowBar.reserve(4 * (size_t)(m ? m : 1));
+    if (m) zeroF4(c, c->diffRowBar, 4 * m);
+    if (m && opt.positionIterations > 0)
+        kDiffCorrectionAdj<<<W, kDiffThreads, 0, s>>>(c->diffPost.p, c->contactsTmp.p, c->dList.p, c->dWStart.p,
+                                                      r.colorOf.p, m, opt.positionIterations, c->diffCTape.p, aX, aQ,
+                                                      aP, over);
+    for (int st = S - 1; st >= 0; --st) {
+        kDiffTransformAdj<<<blocks(n), kThreads, 0, s>>>(c->diffPre[st].p, n, h, aX, aQ, aV, aW);
+        if (!m) continue;
+        kGatherSolveBodies<<<blocks(n), kThreads, 0, s>>>(c->diffSub[st].p, n, sa);
+        sa.rows = c->diffRows[st].p;
+        kDiffSweepsAdj<<<W, kDiffThreads, 0, s>>>(sa, c->dList.p, c->dWStart.p, r.colorOf.p, m, sweeps, st == 0, h,
+                                                  c->diffTape[st].p, aV, aW, c->diffLamBar.p, c->diffRowBar.p, over);
+        kDiffPrefixAdj<<<blocks(m), kThreads, 0, s>>>(sa, r.cs.p, m, c->diffSub[st].p, h, c->diffRowBar.p, aX, aQ, aP);
+    }
+    kDiffNormalizeAdj<<<blocks(n), kThreads, 0, s>>>(r.start.p, n, aQ);
+    // The warm-start impulses were the matched contacts' final impulses.
+    zeroF4(c, c->diffLamPrev, r.prevCount);
+    if (m && r.prevCount)
+        kDiffLambdaCarry<<<blocks(m), kThreads, 0, s>>>(r.match.p, m, c->diffLamBar.p, c->diffLamPrev.p);
+    HIPCK(hipGetLastError());
 }
 
 template <class F>
@@ -1817,6 +2109,145 @@ int vxhip_world_timings(vxhip_ctx* c, vxhip_timings* out) {
     return guarded([&] {
         if (!c || !out) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
         *out = c->timings;
+    });
+}
+
+int vxhip_world_record(vxhip_ctx* c, int enable) {
+    return guarded([&] {
+        if (!c) fail(VXHIP_ERR_INVALID, "vxhip: null context");
+        c->recordOn = enable != 0;
+        c->record.m = 0;
+    });
+}
+
+int vxhip_world_download_record(vxhip_ctx* c, vx_contact* contacts, uint32_t* match, uint32_t cap, uint32_t* count) {
+    return guarded([&] {
+        if (!c || !count) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (!c->recordOn) fail(VXHIP_ERR_INVALID, "vxhip: recording is off");
+        const uint32_t m = c->record.m;
+        *count = m;
+        if (!m || cap < m) return;
+        if (contacts)
+            HIPCK(hipMemcpyAsync(contacts, c->record.cs.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+        if (match)
+            HIPCK(hipMemcpyAsync(match, c->record.match.p, (size_t)m * sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+    });
+}
+
+int vxhip_diff_forward(vxhip_ctx* c, float dt, uint32_t steps, uint32_t every, const float* targets, float* loss) {
+    return guarded([&] {
+        if (!c || !targets || steps == 0 || every == 0) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        diffValidate(c);
+        hipStream_t s = c->stream;
+        const uint32_t W = c->diffW, n = c->n;
+        c->diffReady = false;
+        c->diffSteps = steps;
+        c->diffEvery = every;
+        c->diffDt = dt;
+        const uint32_t segs = (steps + every - 1) / every;
+        c->diffCps.resize(segs);
+        for (auto& cp : c->diffCps)
+            if (!cp) cp.reset(new DiffCheckpoint());
+        for (uint32_t t = 0; t < steps; ++t) {
+            if (t % every == 0) diffSave(c, *c->diffCps[t / every]);
+            worldStep(c, dt);
+        }
+        c->diffFinal.reserve(n);
+        HIPCK(hipMemcpyAsync(c->diffFinal.p, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+        c->diffTarget.reserve(3 * W); c->diffLoss.reserve(W);
+        HIPCK(hipMemcpyAsync(c->diffTarget.p, targets, 3 * (size_t)W * sizeof(float), hipMemcpyHostToDevice, s));
+        kDiffLoss<<<blocks(W), kThreads, 0, s>>>(c->diffFinal.p, c->diffBodyStart.p, W, c->diffTarget.p,
+                                                 c->diffLoss.p, nullptr, nullptr, nullptr, nullptr);
+        HIPCK(hipGetLastError());
+        if (loss) HIPCK(hipMemcpyAsync(loss, c->diffLoss.p, (size_t)W * sizeof(float), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        c->diffReady = true;
+    });
+}
+
+int vxhip_diff_backward(vxhip_ctx* c, const float* lossWeight, float* gV, float* gW, float* gFriction, float* gMass,
+                        vxhip_diff_stats* stats) {
+    return guarded([&] {
+        if (!c || !lossWeight) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (!c->diffReady) fail(VXHIP_ERR_INVALID, "vxhip: no differentiable rollout to take gradients of");
+        hipStream_t s = c->stream;
+        const uint32_t n = c->n, W = c->diffW, T = c->diffSteps, E = c->diffEvery;
+        for (Buf<float4>* b : {&c->diffAdjX, &c->diffAdjQ, &c->diffAdjV, &c->diffAdjW, &c->diffAdjParam})
+            zeroF4(c, *b, n);
+        c->diffAdjMass.reserve(n);
+        kFillF<<<blocks(n), kThreads, 0, s>>>(c->diffAdjMass.p, n);
+        c->diffOverflow.reserve(2);
+        fill32(c, c->diffOverflow.p, 2, 0u);
+        c->diffLossBar.reserve(W);
+        HIPCK(hipMemcpyAsync(c->diffLossBar.p, lossWeight, (size_t)W * sizeof(float), hipMemcpyHostToDevice, s));
+        kDiffLoss<<<blocks(W), kThreads, 0, s>>>(c->diffFinal.p, c->diffBodyStart.p, W, c->diffTarget.p, nullptr,
+                                                 nullptr, c->diffLossBar.p, c->diffAdjX.p, c->diffAdjMass.p);
+        uint32_t ccd = 0, maxContacts = 0;
+        bool last = true;
+        const uint32_t segs = (T + E - 1) / E;
+        for (uint32_t seg = segs; seg-- > 0;) {
+            const uint32_t t0 = seg * E, t1 = std::min(T, t0 + E);
+            diffRestore(c, *c->diffCps[seg]);
+            c->diffSeg.resize(t1 - t0);
+            for (uint32_t j = 0; j < t1 - t0; ++j) {
+                if (!c->diffSeg[j]) c->diffSeg[j].reset(new DiffStepRec());
+                c->diffCap = c->diffSeg[j].get();
+                try {
+                    worldStep(c, c->diffDt);
+                } catch (...) {
+                    c->diffCap = nullptr;
+                    throw;
+                }
+                c->diffCap = nullptr;
+                ccd += c->diffSeg[j]->ccdRescued;
+                maxContacts = std::max(maxContacts, c->diffSeg[j]->m);
+            }
+            // The state after the segment (the replay's reference): the next
+            // checkpoint's bodies, or the final bodies.
+            const vx_body* after = seg + 1 < segs ? c->diffCps[seg + 1]->bodies.p : c->diffFinal.p;
+            for (uint32_t j = t1 - t0; j-- > 0;) {
+                const DiffStepRec& r = *c->diffSeg[j];
+                if (last) {
+                    zeroF4(c, c->diffLamBar, r.m);
+                    last = false;
+                } else {
+                    std::swap(c->diffLamBar.p, c->diffLamPrev.p);
+                    std::swap(c->diffLamBar.cap, c->diffLamPrev.cap);
+                    c->diffLamBar.reserve(r.m ? r.m : 1);
+                }
+                diffBackStep(c, r, j + 1 < t1 - t0 ? c->diffSeg[j + 1]->start.p : after);
+            }
+        }
+        c->diffG.reserve(8 * (size_t)n);
+        float* g = c->diffG.p;
+        kDiffParams<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->diffAdjV.p, c->diffAdjW.p, c->diffAdjParam.p,
+                                                   c->diffAdjMass.p, g, g + 3 * (size_t)n, g + 6 * (size_t)n,
+                                                   g + 7 * (size_t)n);
+        HIPCK(hipGetLastError());
+        if (gV) HIPCK(hipMemcpyAsync(gV, g, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (gW) HIPCK(hipMemcpyAsync(gW, g + 3 * (size_t)n, 3 * (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (gFriction)
+            HIPCK(hipMemcpyAsync(gFriction, g + 6 * (size_t)n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+        if (gMass) HIPCK(hipMemcpyAsync(gMass, g + 7 * (size_t)n, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, s));
+        uint32_t over[2] = {0, 0};
+        HIPCK(hipMemcpyAsync(over, c->diffOverflow.p, sizeof over, hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        // The replays leave the world at the end of the first segment: put it
+        // back at the rollout's end.
+        HIPCK(hipMemcpyAsync(c->bodies.p, c->diffFinal.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToDevice, s));
+        c->prevCount = 0;
+        c->prevExact = false;
+        HIPCK(hipStreamSynchronize(s));
+        if (stats) {
+            stats->steps = T;
+            stats->checkpoints = segs;
+            stats->ccdRescued = ccd;
+            stats->replayMismatch = over[1];
+            stats->maxContacts = maxContacts;
+            stats->worlds = W;
+        }
+        if (over[0]) fail(VXHIP_ERR_CAPACITY, "vxhip: a world has more colour runs than the adjoint kernels hold");
     });
 }
 

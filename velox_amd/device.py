@@ -76,6 +76,15 @@ class MultiToiSettings(C.Structure):
         super().__init__(**defaults)
 
 
+class DiffStats(C.Structure):
+    """vxhip_diff_stats (include/vxhip.h)."""
+    _fields_ = [(name, C.c_uint32) for name in ("steps", "checkpoints", "worlds", "ccdRescued", "replayMismatch",
+                                                "maxContacts")]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class SleepStats(C.Structure):
     _fields_ = [(name, C.c_uint32) for name in ("totalDynamicBodies", "awakeBodies", "drowsyBodies", "sleepingBodies",
                                                 "islandCount", "sleepingIslandCount", "drowsyIslandCount",
@@ -127,6 +136,10 @@ SIGNATURES = {
     "vxhip_probe_color": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "vxhip_probe_warm_start": (_i32, [_vp, _vp, _u32, _vp, _u32]),
     "vxhip_probe_std_sort": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
+    "vxhip_world_record": (_i32, [_vp, _i32]),
+    "vxhip_world_download_record": (_i32, [_vp, _vp, _vp, _u32, _vp]),
+    "vxhip_diff_forward": (_i32, [_vp, _f32, _u32, _u32, _vp, _vp]),
+    "vxhip_diff_backward": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "vxhip_world_profile": (_i32, [_vp, _i32]),
     "vxhip_world_timings": (_i32, [_vp, _vp]),
     "vxhip_scene_build": (_i32, [C.c_char_p, _u32, _vp]),
@@ -415,6 +428,43 @@ class Context:
         buf = (C.c_uint8 * len(blob)).from_buffer_copy(blob)
         check(library().vxhip_world_restore_state(self._h, buf, len(blob)))
         return self
+
+    # -- differentiable form --------------------------------------------
+    def record(self, enable=True):
+        """Keep each step's frozen active set for download_record()."""
+        check(library().vxhip_world_record(self._h, 1 if enable else 0))
+        return self
+
+    def download_record(self):
+        """The last step's contacts in solve order (warm-start impulses) and,
+        per contact, the previous step's solve position it started from
+        (0xffffffff: none)."""
+        count = C.c_uint32(0)
+        check(library().vxhip_world_download_record(self._h, None, None, 0, C.byref(count)))
+        cs = np.zeros(max(1, count.value), CONTACT)
+        match = np.zeros(max(1, count.value), np.uint32)
+        check(library().vxhip_world_download_record(self._h, ptr(cs), ptr(match), len(cs), C.byref(count)))
+        return cs[:count.value], match[:count.value]
+
+    def diff_forward(self, dt, steps, every, targets):
+        """Rollout of `steps` steps keeping a checkpoint every `every`; per-world
+        loss |centre of mass - target|^2 after it (targets: W x 3)."""
+        targets = np.ascontiguousarray(targets, np.float32).reshape(-1)
+        loss = np.zeros(len(targets) // 3, np.float32)
+        check(library().vxhip_diff_forward(self._h, float(dt), int(steps), int(every), ptr(targets), ptr(loss)))
+        return loss
+
+    def diff_backward(self, loss_weight):
+        """Gradients of sum_w loss_weight[w] * loss_w: (dv0 (n, 3), dw0 (n, 3),
+        dfriction (n,), dmass (n,), stats)."""
+        lw = np.ascontiguousarray(loss_weight, np.float32)
+        gv = np.zeros((self.n, 3), np.float32)
+        gw = np.zeros((self.n, 3), np.float32)
+        gf = np.zeros(self.n, np.float32)
+        gm = np.zeros(self.n, np.float32)
+        st = DiffStats()
+        check(library().vxhip_diff_backward(self._h, ptr(lw), ptr(gv), ptr(gw), ptr(gf), ptr(gm), C.byref(st)))
+        return gv, gw, gf, gm, st.as_dict()
 
     def sleep_stats(self):
         st = SleepStats()
