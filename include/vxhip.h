@@ -421,6 +421,18 @@ int vxhip_probe_warm_start(vxhip_ctx* ctx, vx_contact* cur, uint32_t m, const vx
 int vxhip_probe_std_sort(vxhip_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart,
                          uint32_t segments);
 
+/* The same sort on caller-owned device memory, allocating nothing (no
+ * reference counterpart: for callers that cannot allocate, such as code
+ * under HIP graph capture or tight serving loops).  keys, vals and segStart
+ * are device pointers; the working memory is a fixed 4 MiB device array per
+ * GPU, so n <= 262,144 and segments <= 4,096 (VXHIP_ERR_CAPACITY beyond).
+ * Segment offsets that do not rise from 0 to n fail with VXHIP_ERR_INVALID
+ * and leave the keys unsorted.  Runs on the context's stream and returns
+ * when the sort is done.  Calls that overlap on one GPU share the array and
+ * race; the context must belong to the current device. */
+int vxhip_std_sort_noalloc(vxhip_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart,
+                           uint32_t segments);
+
 /* ------------------------------------------------------ synthetic scenes -- */
 /* Host utility: builds body records and flat geometry for the synthetic
  * workloads the device step is measured on (no reference counterpart; the

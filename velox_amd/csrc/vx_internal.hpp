@@ -16,9 +16,11 @@ hipError_t exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t
 // and in parallel (vx_sort.hip).  The input is cut into independent segments
 // [segStart[w], segStart[w + 1]) (one per world; segStart == nullptr: one
 // segment [0, n)), each sorted as its own std::sort call would sort it.
+// With `bad` (a zeroed device word), segment offsets that do not rise from
+// 0 to n are detected on the device: *bad is set and nothing is sorted.
 size_t stdSortTempBytes(uint32_t n, uint32_t segments);
 hipError_t stdSort(uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart, uint32_t segments,
-                   void* temp, hipStream_t s);
+                   void* temp, hipStream_t s, uint32_t* bad = nullptr);
 
 inline int bitsFor(uint64_t maxValue) {  // bits needed to represent 0..maxValue
     int b = 0;

@@ -261,11 +261,23 @@ __global__ void kIntroClear(uint32_t* counts, uint32_t n) {
 }
 
 // Level 0: one range per segment, with std::sort's depth limit for its size.
-__global__ void kIntroInit(const uint32_t* __restrict__ segStart, uint32_t segments, uint32_t n,
-                           IntroSeg* __restrict__ level0, uint32_t* __restrict__ count0,
-                           uint2* __restrict__ leaves, uint32_t* __restrict__ leafCount) {
+// Segment offsets that do not rise from 0 to n without falling (a device
+// array the host has not seen) set *bad; kIntroInit then emits nothing.
+__global__ void kIntroCheck(const uint32_t* __restrict__ segStart, uint32_t segments, uint32_t n,
+                            uint32_t* __restrict__ bad) {
     const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= segments) return;
+    const bool ok = segStart[w] <= segStart[w + 1] && (w > 0 || segStart[0] == 0) &&
+                    (w + 1 < segments || segStart[segments] == n);
+    if (!ok) atomicOr(bad, 1u);
+}
+
+__global__ void kIntroInit(const uint32_t* __restrict__ segStart, uint32_t segments, uint32_t n,
+                           IntroSeg* __restrict__ level0, uint32_t* __restrict__ count0,
+                           uint2* __restrict__ leaves, uint32_t* __restrict__ leafCount,
+                           const uint32_t* __restrict__ bad) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= segments || (bad && *bad)) return;
     const uint32_t f = segStart ? segStart[w] : 0u, l = segStart ? segStart[w + 1] : n;
     if (l <= f) return;
     introEmit(f, l, 2u * vxsort::floorLog2(l - f), level0, count0, leaves, leafCount);
@@ -454,7 +466,7 @@ size_t stdSortTempBytes(uint32_t n, uint32_t segments) { return introLayout(n, s
 
 // Orders (keys, vals) exactly as std::sort would, segment by segment.
 hipError_t stdSort(uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart, uint32_t segments,
-                   void* temp, hipStream_t s) {
+                   void* temp, hipStream_t s, uint32_t* bad) {
     if (n <= 1 || segments == 0) return hipSuccess;
     const IntroLayout L = introLayout(n, segments);
     char* t = (char*)temp;
@@ -466,7 +478,9 @@ hipError_t stdSort(uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* s
     uint32_t* leafCount = counts + L.levels + 1;
     const uint32_t nc = L.levels + 2;
     kIntroClear<<<(nc + 255) / 256, 256, 0, s>>>(counts, nc);
-    kIntroInit<<<(segments + 255) / 256, 256, 0, s>>>(segStart, segments, n, seg[0], counts, leaves, leafCount);
+    if (bad && segStart) kIntroCheck<<<(segments + 255) / 256, 256, 0, s>>>(segStart, segments, n, bad);
+    kIntroInit<<<(segments + 255) / 256, 256, 0, s>>>(segStart, segments, n, seg[0], counts, leaves, leafCount,
+                                                      segStart ? bad : nullptr);
     for (uint32_t level = 0; level + 1 < L.levels; ++level) {
         const uint64_t wide = (uint64_t)segments << std::min<uint32_t>(level, 24);
         const uint32_t grid = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(wide, L.maxSegs), 4096);

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "vx_ccd.hpp"
@@ -125,6 +126,7 @@ struct DiffCheckpoint {
 
 struct vxhip_ctx {
     hipStream_t stream = nullptr;
+    int device = 0;  // the device current at vxhip_create
     // resident scene
     uint32_t n = 0;
     Buf<vx_body> bodies;
@@ -1663,10 +1665,48 @@ int guarded(F&& f) {
 
 vxhip_ctx* newContext() {
     vxhip_ctx* c = new vxhip_ctx();
+    HIPCK(hipGetDevice(&c->device));
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     c->counters.reserve(cCount);
     HIPCK(hipHostMalloc((void**)&c->hostCounters, cCount * sizeof(uint32_t), hipHostMallocDefault));
     return c;
+}
+
+}  // namespace
+
+// ------------------------------------------------ allocation-free sort --
+// vxhip_std_sort_noalloc works in a fixed device array instead of memory it
+// allocates: 256 bytes for its flag word, then the sort's working memory.
+// Every device holds its own copy at its own address, found once per device
+// with hipGetSymbolAddress; the array's name is never used as an address in
+// host code.  Calls on one device share the array (and keep the last call's
+// contents): calls that overlap on one device race.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(4) << 20;
+constexpr uint32_t kNoAllocMaxKeys = 262144, kNoAllocMaxSegments = 4096, kNoAllocFlagSlot = cCount - 1;
+static_assert(kNoAllocFlagSlot > cWarmTies, "the flag word's host slot is free");
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+void* gScratch[kMaxScratchDevices] = {};
+std::mutex gScratchMutex;
+
+// The device address of `scratch` on device `dev` (current), once per device.
+uint8_t* scratchOn(int dev) {
+    if (dev < 0 || dev >= kMaxScratchDevices) fail(VXHIP_ERR_UNAVAILABLE, "vxhip: device index out of range");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratch[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) fail(classify(e), std::string("vxhip: scratch array address: ") + hipGetErrorString(e));
+        gScratch[dev] = p;
+    }
+    return (uint8_t*)gScratch[dev];
 }
 
 }  // namespace
@@ -2352,6 +2392,31 @@ int vxhip_probe_warm_start(vxhip_ctx* c, vx_contact* cur, uint32_t m, const vx_c
         HIPCK(hipStreamSynchronize(c->stream));
         c->prevCount = 0;
         c->prevExact = false;
+    });
+}
+
+int vxhip_std_sort_noalloc(vxhip_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t n, const uint32_t* segStart,
+                           uint32_t segments) {
+    return guarded([&] {
+        if (!c || (n && (!keys || !vals)) || (segStart && segments == 0)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (n > kNoAllocMaxKeys || (segStart && segments > kNoAllocMaxSegments))
+            fail(VXHIP_ERR_CAPACITY, "vxhip: the allocation-free sort takes at most 262,144 keys and 4,096 segments");
+        const uint32_t W = segStart ? segments : 1u;
+        if (vxhost::stdSortTempBytes(n, W) + 256 > kNoAllocScratchBytes)
+            fail(VXHIP_ERR_CAPACITY, "vxhip: the allocation-free sort's working memory exceeds its 4 MiB array");
+        int dev = 0;
+        HIPCK(hipGetDevice(&dev));
+        if (dev != c->device) fail(VXHIP_ERR_INVALID, "vxhip: the context belongs to another device");
+        uint8_t* base = scratchOn(dev);
+        uint32_t* bad = reinterpret_cast<uint32_t*>(base);
+        if (n <= 1) return;
+        hipStream_t s = c->stream;
+        kFill32<<<1, 64, 0, s>>>(bad, 1, 0u);
+        HIPCK(vxhost::stdSort(keys, vals, n, segStart, W, base + 256, s, segStart ? bad : nullptr));
+        HIPCK(hipMemcpyAsync(c->hostCounters + kNoAllocFlagSlot, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
+        if (c->hostCounters[kNoAllocFlagSlot])
+            fail(VXHIP_ERR_INVALID, "vxhip: segment offsets must rise from 0 to n without falling");
     });
 }
 

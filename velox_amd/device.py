@@ -136,6 +136,7 @@ SIGNATURES = {
     "vxhip_probe_color": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "vxhip_probe_warm_start": (_i32, [_vp, _vp, _u32, _vp, _u32]),
     "vxhip_probe_std_sort": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
+    "vxhip_std_sort_noalloc": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
     "vxhip_world_record": (_i32, [_vp, _i32]),
     "vxhip_world_download_record": (_i32, [_vp, _vp, _vp, _u32, _vp]),
     "vxhip_diff_forward": (_i32, [_vp, _f32, _u32, _u32, _vp, _vp]),
@@ -160,6 +161,14 @@ def library():
     if _lib is None:
         if not os.path.exists(LIBRARY):
             raise RuntimeError(f"{LIBRARY} is missing: run velox_amd.build.build() (or __graft_entry__.build())")
+        # One HIP runtime per process: when PyTorch is installed it is loaded
+        # first, and the library binds to the runtime PyTorch brings, so device
+        # buffers and streams can pass between the two (the differentiable
+        # form's autograd binding, caller-owned device buffers).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(LIBRARY)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)
