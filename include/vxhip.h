@@ -327,15 +327,22 @@ int vxhip_world_sleep_stats(vxhip_ctx* ctx, vx_sleep_stats* out);
 
 /* Stage timings measured with HIP events on the context's own stream while
  * profiling is enabled (accumulated over steps; StepStats timings,
- * world.cpp:4523-4575).  Every contact-solve launch is bracketed by its own
- * event pair, giving each solver kernel's device time, launch count and the
- * contact visits (contacts x sweeps) it performed, indexed by VXHIP_KERNEL_*. */
+ * world.cpp:4523-4575).  Every launch of the kernels below is bracketed by
+ * its own event pair, giving each one's device time, launch count and the
+ * work units it processed, indexed by VXHIP_KERNEL_*: contact visits
+ * (contacts x sweeps) for the solve kernels, candidate pairs for the narrow
+ * phase and the CCD check, walked pairs for the CCD dataflow, bodies for the
+ * pair emission. */
 enum {
-    VXHIP_KERNEL_BATCH = 0,      /* kBatchSweeps, velocity sweeps (island batches in LDS) */
-    VXHIP_KERNEL_BATCH_WARM = 1, /* kBatchSweeps, warm start */
-    VXHIP_KERNEL_ISLAND = 2,     /* kIslandSweeps (one larger island per workgroup) */
-    VXHIP_KERNEL_COLOR = 3,      /* big-island colours: kSweepColorPair, kSweepColorRun, kSweepTailPair */
-    VXHIP_KERNEL_COUNT = 4
+    VXHIP_KERNEL_BATCH = 0,        /* kBatchSweeps, velocity sweeps (island batches in LDS) */
+    VXHIP_KERNEL_BATCH_WARM = 1,   /* kBatchSweeps, warm start */
+    VXHIP_KERNEL_ISLAND = 2,       /* kIslandSweeps (one larger island per workgroup) */
+    VXHIP_KERNEL_COLOR = 3,        /* big-island colours: kSweepColorPair, kSweepColorRun, kSweepTailPair */
+    VXHIP_KERNEL_NARROW = 4,       /* kNarrow (units: candidate pairs) */
+    VXHIP_KERNEL_CCD_CHECK = 5,    /* kCcdCheck (units: pairs checked) */
+    VXHIP_KERNEL_CCD_DATAFLOW = 6, /* kCcdDataflow (units: pairs walked in order) */
+    VXHIP_KERNEL_GRID_PAIRS = 7,   /* kGridPairs (units: bodies) */
+    VXHIP_KERNEL_COUNT = 8
 };
 typedef struct {
     uint32_t steps;

@@ -904,6 +904,56 @@ int bench(const std::string& name, int steps, int workers, int warmup) {
 }
 
 
+// The reference CPU backend timed on sampled steps of a trajectory: steps
+// stride, 2 stride, ... up to `end` of the scene's run.  The first pass steps
+// the whole trajectory, timing the sampled steps and keeping a World snapshot
+// before each; `repeats - 1` more passes restore each snapshot and time its
+// step again.  Prints one JSON line: the rate of every pass
+// (bodies x sampled steps / their summed time), the worker count the backend
+// resolved (setWorkerCount(0) is hardware concurrency capped at 64,
+// solver.cpp:132-137) and std::thread::hardware_concurrency().
+int sample(const std::string& name, int end, int stride, int workers, int repeats) {
+    vxscene::Scene s = sceneByName(name);
+    World w(BackendType::Cpu);
+    build(w, s);
+    w.setWorkerCount((uint32_t)workers);
+    std::vector<WorldSnapshot> snaps;
+    std::vector<double> rates;
+    double total = 0.0;
+    int count = 0;
+    for (int t = 1; t <= end; ++t) {
+        const bool timed = t % stride == 0;
+        if (timed) snaps.push_back(w.saveSnapshot());
+        auto a = std::chrono::steady_clock::now();
+        w.step(s.dt);
+        if (timed) {
+            total += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+            ++count;
+            std::fprintf(stderr, "[ref] step %d contacts=%zu\n", t, w.contacts_.size());
+        }
+    }
+    rates.push_back((double)w.bodies_.size() * count / total);
+    for (int r = 1; r < repeats; ++r) {
+        double sum = 0.0;
+        for (const WorldSnapshot& snap : snaps) {
+            w.restoreSnapshot(snap);
+            auto a = std::chrono::steady_clock::now();
+            w.step(s.dt);
+            sum += std::chrono::duration<double>(std::chrono::steady_clock::now() - a).count();
+        }
+        rates.push_back((double)w.bodies_.size() * count / sum);
+    }
+    std::vector<double> sorted = rates;
+    std::sort(sorted.begin(), sorted.end());
+    std::string list;
+    for (double r : rates) list += (list.empty() ? "" : ", ") + std::to_string(r);
+    std::printf("{\"scene\": \"%s\", \"bodies\": %zu, \"sampled_steps\": %d, \"stride\": %d, \"end\": %d, "
+                "\"workers\": %u, \"hardware_concurrency\": %u, \"rates\": [%s], \"median\": %.3f}\n",
+                name.c_str(), w.bodies_.size(), count, stride, end, w.backend_->workerCount(),
+                std::thread::hardware_concurrency(), list.c_str(), sorted[sorted.size() / 2]);
+    return 0;
+}
+
 // Per-step body states and contacts of a reference trajectory (debugging aid
 // for locating the first step where a replay departs from the reference).
 int trace(const std::string& out, const std::string& which, int steps) {
@@ -1008,7 +1058,10 @@ int main(int argc, char** argv) {
     if (mode == "trace" && argc > 4) return trace(argv[2], argv[3], std::atoi(argv[4]));
     if (mode == "fixtures" && argc > 2) return fixtures(argv[2]);
     if (mode == "bench" && argc > 4) return bench(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), argc > 5 ? std::atoi(argv[5]) : 1);
-    std::fprintf(stderr, "usage: %s fixtures <outdir> | fixture <outdir> <case> | bench <scene> <steps> <workers> [warmup]\n",
+    if (mode == "sample" && argc > 6)
+        return sample(argv[2], std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), std::atoi(argv[6]));
+    std::fprintf(stderr, "usage: %s fixtures <outdir> | fixture <outdir> <case> | bench <scene> <steps> <workers> [warmup]"
+                         " | sample <scene> <end> <stride> <workers> <repeats>\n",
                  argv[0]);
     return 2;
 }

@@ -305,7 +305,7 @@ void markStage(vxhip_ctx* c, int stage) {
     if (c->profile) HIPCK(hipEventRecord(c->stageEv[stage], c->stream));
 }
 
-// Events bracket every colour-solve launch while profiling.
+// Events bracket every launch of the event-timed kernels while profiling.
 void solveLaunchBegin(vxhip_ctx* c) {
     if (!c->profile) return;
     if (c->launchUsed + 2 > c->launchEv.size()) {
@@ -515,9 +515,11 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
     for (;;) {
         c->pairs.reserve(c->pairCap);
         kResetCounters<<<1, 128, 0, s>>>(k);  // only cPairs matters here; others re-derived
+        solveLaunchBegin(c);
         kGridPairs<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->keys.p, c->idx.p,
                                                   table ? c->cellRange.p : nullptr, n, g, nb, c->pairs.p, k + cPairs,
                                                   c->pairCap);
+        solveLaunchEnd(c, VXHIP_KERNEL_GRID_PAIRS, n);
         if (oversizeCount && batched)
             kOversizePairsWorld<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, c->worldOf.p,
                                                                c->worldStart.p, c->oversizeByWorld.p, n, g, nb,
@@ -557,8 +559,10 @@ uint32_t narrowPhase(vxhip_ctx* c, uint32_t np, float dt) {
         kPairClass<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, np, nb, c->pairClass.p, c->pairOrder.p);
         HIPCK(vxhost::radixSort(c->pairClass.p, c->pairOrder.p, c->pairClassTmp.p, c->pairOrderTmp.p, np,
                                 kPairClassBits, tempFor(c, vxhost::radixTempBytes(np)), s));
+        solveLaunchBegin(c);
         kNarrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, np, nb, c->pairOrder.p,
                                                 c->slots.p, c->counts.p);
+        solveLaunchEnd(c, VXHIP_KERNEL_NARROW, np);
         HIPCK(hipGetLastError());
     }
     HIPCK(vxhost::exclusiveScan(c->counts.p, c->offsets.p, np, c->counters.p + cContacts,
@@ -1040,9 +1044,11 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     c->ccdFlag.reserve(np);
     c->ccdGap.reserve(np);
     const uint8_t* toiDone = c->toiProcessed ? c->toiEligible.p : nullptr;
+    solveLaunchBegin(c);
     kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb,
                                               c->pairOrder.p, c->ccdFlag.p,
                                               c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst, toiDone);
+    solveLaunchEnd(c, VXHIP_KERNEL_CCD_CHECK, np);
     HIPCK(hipGetLastError());
     readCounters(c);
     if (c->hostCounters[cCcdFlagged] == 0) return;
@@ -1084,9 +1090,13 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
                             tempFor(c, vxhost::radixTempBytes(e)), s));
     if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, pbits, c->pred.p);
     const uint32_t grid = std::min<uint32_t>(blocks(nr, kThreads / 64), kCcdBlocks);  // one entry per wavefront first
-    if (nr) kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p, c->ccdList.p,
-                                           nr, nb, c->ccdFlag.p, c->ccdGap.p, c->pred.p, c->ccdDone.p,
-                                           k + cCcdExamined, k + cCcdRescued, k + cColorError, toiDone);
+    if (nr) {
+        solveLaunchBegin(c);
+        kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p,
+                                               c->ccdList.p, nr, nb, c->ccdFlag.p, c->ccdGap.p, c->pred.p, c->ccdDone.p,
+                                               k + cCcdExamined, k + cCcdRescued, k + cColorError, toiDone);
+        solveLaunchEnd(c, VXHIP_KERNEL_CCD_DATAFLOW, nr);
+    }
     HIPCK(hipGetLastError());
 }
 

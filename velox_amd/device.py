@@ -191,11 +191,12 @@ def device_count():
 class Timings(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("beginMs", C.c_float), ("broadMs", C.c_float), ("narrowMs", C.c_float),
                 ("matchMs", C.c_float), ("colorMs", C.c_float), ("solveMs", C.c_float), ("ccdMs", C.c_float),
-                ("positionMs", C.c_float), ("finishMs", C.c_float), ("kernelMs", C.c_float * 4),
-                ("kernelLaunches", C.c_uint32 * 4), ("kernelVisits", C.c_uint64 * 4)]
+                ("positionMs", C.c_float), ("finishMs", C.c_float), ("kernelMs", C.c_float * 8),
+                ("kernelLaunches", C.c_uint32 * 8), ("kernelVisits", C.c_uint64 * 8)]
 
-    # Solver kernel families, in the order of the per-kernel arrays.
-    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorPair/Run/TailPair")
+    # Event-timed kernels, in the order of the per-kernel arrays (VXHIP_KERNEL_*).
+    KERNELS = ("kBatchSweeps<0>", "kBatchSweeps<2>", "kIslandSweeps", "kSweepColorPair/Run/TailPair", "kNarrow",
+               "kCcdCheck", "kCcdDataflow", "kGridPairs")
 
     def as_dict(self):
         d = {}
