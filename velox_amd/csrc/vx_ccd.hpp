@@ -15,11 +15,11 @@ struct Gap { float gap; V3 normal; };
 
 // `coop`: the whole wavefront evaluates this same query (hull supports are
 // then shared across the lanes; see hullSupportIndexWave).
-__device__ inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup& soup, float search,
-                                bool coop = false) {
+__device__ __attribute__((always_inline)) inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup& soup,
+                                                               float search, bool coop = false) {
     if (tb.shape == VX_SHAPE_PLANE) {
         Convex c = makeConvex(ta, soup);
-        c.coop = coop;
+        if (coop) c.makeCoop();
         V3 pn = v3(tb.planeNormal);
         V3 deep = c.support(-pn);
         return {dot(pn, deep) - tb.planeOffset - c.radius, pn};
@@ -30,46 +30,44 @@ __device__ inline Gap gapSimple(const vx_body& ta, const vx_body& tb, const Soup
         return best;
     }
     Convex ca = makeConvex(ta, soup), cb = makeConvex(tb, soup);
-    ca.coop = coop;
-    cb.coop = coop;
+    if (coop) {
+        ca.makeCoop();
+        cb.makeCoop();
+    }
     GjkResult r = gjkDistance(ca, cb);
     return {r.distance, r.normal};
 }
 
-// world.cpp:2916-2958 (compound children expanded without recursion).
-__device__ inline Gap gapAt(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0, V3 pb, Q4 qb, const Soup& soup,
-                            float search, bool coop = false) {
+// world.cpp:2916-2958 (compound children expanded without recursion): the
+// child pairs are visited in the reference's nested order (children of a,
+// then of b) and the first smallest gap is kept, which is what the nested
+// strict-minimum loops keep; a pair of simple shapes returns its gap as it
+// is.  One gapSimple call site, so the query inlines as one copy: a call per
+// query would save and restore the callee's registers through scratch memory
+// on every conservative-advancement iteration.
+__device__ __attribute__((always_inline)) inline Gap gapAtInline(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0,
+                                                                 V3 pb, Q4 qb, const Soup& soup, float search,
+                                                                 bool coop = false) {
     vx_body ta = a0; ta.position = raw(pa); ta.orientation = raw(qa);
     vx_body tb = b0; tb.position = raw(pb); tb.orientation = raw(qb);
-    if (ta.shape == VX_SHAPE_COMPOUND) {
-        Gap best{1e30f, {0, 1, 0}};
-        for (uint32_t i = 0; i < ta.compoundCount; ++i) {
-            vx_body ch = childBody(ta, soup.children[ta.compoundFirst + i]);
-            Gap g;
-            if (tb.shape == VX_SHAPE_COMPOUND) {
-                g = {1e30f, {0, 1, 0}};
-                for (uint32_t k = 0; k < tb.compoundCount; ++k) {
-                    vx_body cb = childBody(tb, soup.children[tb.compoundFirst + k]);
-                    Gap h = gapSimple(ch, cb, soup, search, coop);
-                    if (h.gap < g.gap) g = h;
-                }
-            } else {
-                g = gapSimple(ch, tb, soup, search, coop);
-            }
+    const bool ca = ta.shape == VX_SHAPE_COMPOUND, cb = tb.shape == VX_SHAPE_COMPOUND;
+    const uint32_t na = ca ? ta.compoundCount : 1u, nb = cb ? tb.compoundCount : 1u;
+    Gap best{1e30f, {0, 1, 0}};
+    for (uint32_t i = 0; i < na; ++i)
+        for (uint32_t k = 0; k < nb; ++k) {
+            vx_body x = ta, y = tb;
+            if (ca) x = childBody(ta, soup.children[ta.compoundFirst + i]);
+            if (cb) y = childBody(tb, soup.children[tb.compoundFirst + k]);
+            const Gap g = gapSimple(x, y, soup, search, coop);
+            if (!ca && !cb) return g;
             if (g.gap < best.gap) best = g;
         }
-        return best;
-    }
-    if (tb.shape == VX_SHAPE_COMPOUND) {
-        Gap best{1e30f, {0, 1, 0}};
-        for (uint32_t i = 0; i < tb.compoundCount; ++i) {
-            vx_body ch = childBody(tb, soup.children[tb.compoundFirst + i]);
-            Gap g = gapSimple(ta, ch, soup, search, coop);
-            if (g.gap < best.gap) best = g;
-        }
-        return best;
-    }
-    return gapSimple(ta, tb, soup, search, coop);
+    return best;
+}
+
+__device__ inline Gap gapAt(const vx_body& a0, V3 pa, Q4 qa, const vx_body& b0, V3 pb, Q4 qb, const Soup& soup,
+                            float search, bool coop = false) {
+    return gapAtInline(a0, pa, qa, b0, pb, qb, soup, search, coop);
 }
 
 __device__ inline Q4 orientationAt(const vx_body& body, Q4 start, V3 L, float elapsed) {
@@ -116,7 +114,7 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
         const vx_body a = bodies[i], b = bodies[j];
         // Bodies the multi-TOI scheduler replayed are left alone (world.cpp:4268-4269).
         if (ccdEligible(a, b) && !(toiDone && (toiDone[i] || toiDone[j]))) {
-            Gap end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup,
+            Gap end = gapAtInline(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup,
                             ccdSearch(a, dt));
             if (end.gap < -8e-3f) {
                 f = 1;
@@ -136,31 +134,50 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
 // has moved since (same state, same result).
 // With `coop` the whole wavefront runs the same rescue (every lane computes
 // the same values) and only the `writer` lane stores the bodies.
-__device__ inline bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const vx_quat* prevQ, uint32_t i, uint32_t j,
-                                 const Soup& soup, float dt, const float4* known, bool coop = false,
-                                 bool writer = true) {
+__device__ __attribute__((noinline)) bool ccdRescue(vx_body* bodies, const vx_vec3* prevPos, const vx_quat* prevQ,
+                                                    uint32_t i, uint32_t j, const Soup& soup, float dt,
+                                                    const float4* known, bool coop = false, bool writer = true) {
     vx_body a = bodies[i], b = bodies[j];
     if (!ccdEligible(a, b)) return false;
     const float search = ccdSearch(a, dt);
-    Gap end;
-    if (known) end = {known->w, {known->x, known->y, known->z}};
-    else end = gapAt(a, v3(a.position), q4(a.orientation), b, v3(b.position), q4(b.orientation), soup, search, coop);
-    if (end.gap >= -8e-3f) return false;
     V3 pa0 = v3(prevPos[i]), da = v3(a.position) - pa0;
     Q4 qa0 = q4(prevQ[i]);
     V3 pb0 = isStatic(b) ? v3(b.position) : v3(prevPos[j]);
     V3 db = v3(b.position) - pb0;
     Q4 qb0 = isStatic(b) ? q4(b.orientation) : q4(prevQ[j]);
     float bound = maxPointSpeed(a) * dt + (isStatic(b) ? 0.0f : maxPointSpeed(b) * dt);
-    if (bound < 1e-9f) return false;
     V3 La = angularMomentum(a);
     V3 Lb = isDynamic(b) ? angularMomentum(b) : V3{};
+    // Pass -1 measures the end-of-step gap (when the parallel check's value
+    // cannot be reused): a pair at most 8 mm deep, or whose bodies barely
+    // move, is left alone.  Passes 0..23 advance conservatively from the
+    // step's start.  One gap query call site for all passes.
+    Gap end{};
+    if (known) end = {known->w, {known->x, known->y, known->z}};
     float t = 0.0f;
-    V3 n = end.normal;
-    for (int it = 0; it < 24 && t < 1.0f; ++it) {
-        Q4 qa = orientationAt(a, qa0, La, dt * t);
-        Q4 qb = isStatic(b) ? qb0 : orientationAt(b, qb0, Lb, dt * t);
-        Gap g = gapAt(a, pa0 + da * t, qa, b, pb0 + db * t, qb, soup, search, coop);
+    V3 n{};
+    for (int it = known ? 0 : -1;; ++it) {
+        if (it == 0) {
+            if (end.gap >= -8e-3f || bound < 1e-9f) return false;
+            n = end.normal;
+        }
+        if (it >= 0 && !(it < 24 && t < 1.0f)) break;
+        V3 pA, pB;
+        Q4 qA, qB;
+        if (it < 0) {
+            pA = v3(a.position); qA = q4(a.orientation);
+            pB = v3(b.position); qB = q4(b.orientation);
+        } else {
+            qA = orientationAt(a, qa0, La, dt * t);
+            qB = isStatic(b) ? qb0 : orientationAt(b, qb0, Lb, dt * t);
+            pA = pa0 + da * t;
+            pB = pb0 + db * t;
+        }
+        const Gap g = gapAtInline(a, pA, qA, b, pB, qB, soup, search, coop);
+        if (it < 0) {
+            end = g;
+            continue;
+        }
         n = g.normal;
         if (g.gap < 1e-3f) break;
         t = fmn(1.0f, t + g.gap / bound);

@@ -620,6 +620,9 @@ struct PairItem {
     float4 i0, i1;                 // accumulated impulses
 };
 
+// kImp: also load the accumulated impulses (kernels that keep them in LDS
+// read them there at the row's turn instead).
+template <bool kImp = true>
 __device__ __attribute__((always_inline)) inline PairItem loadPairItemSlots(SolveArrays sa, uint32_t slots, uint32_t j,
                                                                            bool sideB) {
     PairItem it;
@@ -630,23 +633,66 @@ __device__ __attribute__((always_inline)) inline PairItem loadPairItemSlots(Solv
     for (int q = 0; q < 8; ++q) it.r[q] = row[q];
     const float4* sd = sa.side + 8 * (size_t)j + (sideB ? 4 : 0);
     it.frame = sd[0]; it.invI = sd[1]; it.vel = sd[2]; it.ang = sd[3];
-    it.i0 = sa.imp[2 * j];
-    it.i1 = sa.imp[2 * j + 1];
+    if (kImp) {
+        it.i0 = sa.imp[2 * j];
+        it.i1 = sa.imp[2 * j + 1];
+    }
     return it;
 }
 
+template <bool kImp = true>
 __device__ __attribute__((always_inline)) inline PairItem loadPairItem(SolveArrays sa, const uint32_t* slotPair,
                                                                       uint32_t j, bool sideB) {
-    return loadPairItemSlots(sa, slotPair[j], j, sideB);
+    return loadPairItemSlots<kImp>(sa, slotPair[j], j, sideB);
+}
+
+// The row data one lane of an LDS-resident pass reads, loaded at exactly the
+// widths it uses (the flags word of meta, the arm, direction and mass terms
+// the mode reads, the side's frame, inertia and xyz velocities): a loaded
+// register that no instruction reads may be reused while its load is still in
+// flight, and the wait for that load would stall the pass on the loads of the
+// next one.  The impulses are read from LDS at the row's turn.
+template <int kMode>
+__device__ __attribute__((always_inline)) inline PairItem loadLdsRow(SolveArrays sa, uint32_t slots, uint32_t j,
+                                                                    bool sideB) {
+    PairItem it;
+    it.meta = make_uint4(0u, 0u, reinterpret_cast<const uint32_t*>(sa.meta + j)[2], 0u);
+    it.slots = slots;
+    const float4* row = sa.rows + 8 * (size_t)j;
+    if (kMode == 2) {  // warm start: arms and the three directions
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+            const float* f = reinterpret_cast<const float*>(row + q);
+            it.r[q] = make_float4(f[0], f[1], f[2], 0.0f);
+        }
+#pragma unroll
+        for (int q = 5; q < 8; ++q) it.r[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 7; ++q) it.r[q] = row[q];
+        if (kMode == 1) it.r[7] = row[7];
+        else it.r[7] = make_float4(reinterpret_cast<const float*>(row + 7)[0], 0.0f, 0.0f, 0.0f);  // ks only
+    }
+    const float4* sd = sa.side + 8 * (size_t)j + (sideB ? 4 : 0);
+    it.frame = sd[0];
+    it.invI = sd[1];
+    const float* v = reinterpret_cast<const float*>(sd + 2);
+    const float* w = reinterpret_cast<const float*>(sd + 3);
+    it.vel = make_float4(v[0], v[1], v[2], 0.0f);
+    it.ang = make_float4(w[0], w[1], w[2], 0.0f);
+    return it;
 }
 
 // kGlobal: the lane's body velocities are read from and written back to the
-// global arrays (slot and lv unused) instead of LDS slots.
-template <int kMode, uint32_t kStride = kLdsBodies, bool kGlobal = false>  // 0 two-axis, 1 cone-block, 2 warm start
-__device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, uint32_t j, PairItem& it, bool sideB,
+// global arrays (slot and lv unused) instead of LDS slots.  kStoreImp: the
+// new impulses are stored to the global array; without it they are only
+// left in it.i0/it.i1, and the return value says whether they changed
+// (a solved row, mode 0 or 1) so the caller can keep them elsewhere.
+template <int kMode, uint32_t kStride = kLdsBodies, bool kGlobal = false, bool kStoreImp = true>
+__device__ __attribute__((always_inline)) inline bool pairRow(SolveArrays sa, uint32_t j, PairItem& it, bool sideB,
                                                               float* __restrict__ lv) {
     const uint32_t flags = it.meta.z;
-    if (kMode == 2 ? (flags & kSensorRow) != 0 : (flags & kSkip) != 0) return;
+    if (kMode == 2 ? (flags & kSensorRow) != 0 : (flags & kSkip) != 0) return false;
     const bool dyn = flags & (sideB ? kDynB : kDynA);
     const bool rot = flags & (sideB ? kRotB : kRotA);
     const uint32_t slot = sideB ? it.slots >> 16 : it.slots & 0xffffu;
@@ -785,8 +831,10 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
         // reload of this contact follows its own store.
         it.i0 = make_float4(cN, cT1, cT2, cR1);
         it.i1 = make_float4(cR2, cS, 0.0f, 0.0f);
-        sa.imp[2 * j] = it.i0;
-        sa.imp[2 * j + 1] = it.i1;
+        if (kStoreImp) {
+            sa.imp[2 * j] = it.i0;
+            sa.imp[2 * j + 1] = it.i1;
+        }
     }
     if (dyn && kGlobal) {
         sa.vel[body] = make_float4(v.x, v.y, v.z, 0.0f);
@@ -795,6 +843,7 @@ __device__ __attribute__((always_inline)) inline void pairRow(SolveArrays sa, ui
         lv[slot] = v.x; lv[kStride + slot] = v.y; lv[2 * kStride + slot] = v.z;
         lv[3 * kStride + slot] = w.x; lv[4 * kStride + slot] = w.y; lv[5 * kStride + slot] = w.z;
     }
+    return kMode != 2;
 }
 
 // Medium islands, one workgroup each on the global velocity arrays: every
@@ -874,16 +923,19 @@ __global__ __launch_bounds__(kThreads) void kSweepColorPair(SolveArrays sa, uint
     pairRow<kMode, kLdsBodies, true>(sa, k, it, sideB, nullptr);
 }
 
-// LDS batches, one workgroup each: the batch's dynamic bodies are loaded
-// into LDS, then every sweep of every colour present in the batch runs with
-// a barrier between colours (the batch's islands share no dynamic body with
-// any other contact, and contacts of one colour share none with each other,
-// so this is the colour order exactly), and the bodies are written back.
-// A pass is up to kThreads/2 contacts of one colour; each lane loads its
-// next pass's contact while it solves the current one.  A lane meets the
-// same contact again one sweep later, which is the next pass only when a
-// sweep is a single pass: then the impulses it just computed are carried
-// over instead of the prefetched (older) ones.
+// LDS batches, one workgroup each: the batch's dynamic bodies and its
+// contacts' accumulated impulses are loaded into LDS, then every sweep of
+// every colour present in the batch runs with a barrier between colours (the
+// batch's islands share no dynamic body with any other contact, and contacts
+// of one colour share none with each other, so this is the colour order
+// exactly), and bodies and impulses are written back.  A pass is up to
+// kThreads/2 contacts of one colour; each lane loads its next pass's row data
+// while it solves the current one.  The sweeps make no global store and
+// every lane issues the same loads on every pass (lanes without a row in the
+// next pass load the batch's first row and drop it), so the wait for a
+// pass's data never has to wait for the loads of the pass after it.
+constexpr uint32_t kBatchRows = kBatch + kLdsIsland;  // more than any batch's contact count
+
 template <int kMode>
 __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ slotPair,
                                                          const uint32_t* __restrict__ runStart,
@@ -893,6 +945,8 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
                                                          const uint32_t* __restrict__ batchRange, int sweeps,
                                                          uint32_t* __restrict__ error) {
     __shared__ float lv[6 * kLdsBodies];
+    __shared__ float4 li4[kBatchRows];  // cN, cT1, cT2, cR1
+    __shared__ float2 li2[kBatchRows];  // cR2, cS
     const uint32_t first = batchRange[0], last = batchRange[1];
     const uint32_t t = threadIdx.x;
     const bool sideB = t & 1;
@@ -900,7 +954,9 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
     for (uint32_t q = first + blockIdx.x; q < last; q += gridDim.x) {
         const uint2 bb = batchBodies[q];
         const uint32_t nb = bb.y - bb.x;
-        if (nb > kLdsBodies) {  // excluded by the packing bounds; reported, never overrun
+        const uint32_t r0 = batchFirstRun[q], r1 = batchFirstRun[q + 1];
+        const uint32_t jb = runStart[r0], nc = runStart[r1] - jb;  // the batch's contacts, contiguous
+        if (nb > kLdsBodies || nc > kBatchRows) {  // excluded by the packing bounds; reported, never overrun
             if (t == 0) atomicOr(error, 1u);
             continue;
         }
@@ -910,15 +966,23 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
             lv[i] = v.x; lv[kLdsBodies + i] = v.y; lv[2 * kLdsBodies + i] = v.z;
             lv[3 * kLdsBodies + i] = w.x; lv[4 * kLdsBodies + i] = w.y; lv[5 * kLdsBodies + i] = w.z;
         }
-        const uint32_t r0 = batchFirstRun[q], r1 = batchFirstRun[q + 1];
-        // Pass cursor: sweep sw, run r, chunk start base, run end end.
+        for (uint32_t i = t; i < nc; i += kThreads) {
+            const float4 a = sa.imp[2 * (jb + i)], b = sa.imp[2 * (jb + i) + 1];
+            li4[i] = a;
+            li2[i] = make_float2(b.x, b.y);
+        }
+        // Pass cursor: sweep sw, run r, chunk start base, run end end.  Two
+        // row buffers alternate (the loop body is written out twice), so a
+        // pass's data stays where it was loaded.
         int sw = 0;
-        uint32_t r = r0, base = runStart[r0], end = runStart[r0 + 1];
+        uint32_t r = r0, base = jb, end = runStart[r0 + 1];
         uint32_t j = base + (t >> 1);
-        PairItem cur;
-        if (j < end) cur = loadPairItem(sa, slotPair, j, sideB);
+        const uint32_t j0 = j < end ? j : jb;
+        PairItem bufA = loadLdsRow<kMode>(sa, slotPair[j0], j0, sideB), bufB;
         __syncthreads();
-        for (;;) {
+        // Solves the pass in `cur` after starting the next pass's loads into
+        // `nxt`; false after the last pass.
+        auto pass = [&](PairItem& cur, PairItem& nxt) {
             int nsw = sw;
             uint32_t nr = r, nbase = base + kPass, nend = end;
             const bool runDone = nbase >= end;
@@ -930,20 +994,35 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
             }
             const bool more = nsw < sweeps;
             const uint32_t nj = nbase + (t >> 1);
-            PairItem nxt;
-            if (more && nj < nend) nxt = loadPairItem(sa, slotPair, nj, sideB);
-            if (j < end) pairRow<kMode>(sa, j, cur, sideB, lv);
+            const uint32_t jl = more && nj < nend ? nj : jb;
+            nxt = loadLdsRow<kMode>(sa, slotPair[jl], jl, sideB);
+            if (j < end) {
+                const uint32_t k = j - jb;
+                const float2 e = li2[k];
+                cur.i0 = li4[k];
+                cur.i1 = make_float4(e.x, e.y, 0.0f, 0.0f);
+                if (pairRow<kMode, kLdsBodies, false, false>(sa, j, cur, sideB, lv)) {
+                    li4[k] = cur.i0;
+                    li2[k] = make_float2(cur.i1.x, cur.i1.y);
+                }
+            }
             if (runDone) ldsBarrier();
-            if (!more) break;
-            if (nj == j && kMode != 2) { nxt.i0 = cur.i0; nxt.i1 = cur.i1; }
             sw = nsw; r = nr; base = nbase; end = nend; j = nj;
-            cur = nxt;
+            return more;
+        };
+        while (pass(bufA, bufB) && pass(bufB, bufA)) {
         }
         for (uint32_t i = t; i < nb; i += kThreads) {
             const uint32_t body = bodyList[bb.x + i];
             sa.vel[body] = make_float4(lv[i], lv[kLdsBodies + i], lv[2 * kLdsBodies + i], 0.0f);
             sa.ang[body] = make_float4(lv[3 * kLdsBodies + i], lv[4 * kLdsBodies + i], lv[5 * kLdsBodies + i], 0.0f);
         }
+        if (kMode != 2)
+            for (uint32_t i = t; i < nc; i += kThreads) {
+                const float2 e = li2[i];
+                sa.imp[2 * (jb + i)] = li4[i];
+                sa.imp[2 * (jb + i) + 1] = make_float4(e.x, e.y, 0.0f, 0.0f);
+            }
         __syncthreads();
     }
 }

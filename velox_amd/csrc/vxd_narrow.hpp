@@ -49,6 +49,33 @@ VXD uint32_t hullSupportIndexWave(const vx_vec3* pts, uint32_t count, V3 d) {
     return best < count ? best : 0u;  // lane 0 holds a valid candidate, so lane 63 does too
 }
 
+// The same support point when every lane holds one hull point in registers
+// (lane k point k, count <= 64): no memory access per query.
+VXD V3 hullSupportPointLanes(V3 lp, uint32_t count, V3 d) {
+    auto lanePoint = [&](uint32_t k) {
+        return V3{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(lp.x), (int)k)),
+                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lp.y), (int)k)),
+                  __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lp.z), (int)k))};
+    };
+    const V3 p0 = lanePoint(0);
+    if (!(dot(d, p0) == dot(d, p0))) return p0;
+    const uint32_t lane = __lane_id();
+    float t = 0.0f;
+    uint32_t i = kNoPoint;
+    if (lane < count) {
+        const float tk = dot(d, lp);
+        if (tk == tk) { t = tk; i = lane; }
+    }
+    waveBestStep<0x111, 0xf>(t, i);  // row_shr:1
+    waveBestStep<0x112, 0xf>(t, i);  // row_shr:2
+    waveBestStep<0x114, 0xf>(t, i);  // row_shr:4
+    waveBestStep<0x118, 0xf>(t, i);  // row_shr:8
+    waveBestStep<0x142, 0xa>(t, i);  // row_bcast:15
+    waveBestStep<0x143, 0xc>(t, i);  // row_bcast:31
+    const uint32_t best = (uint32_t)__builtin_amdgcn_readlane((int)i, 63);
+    return lanePoint(best < count ? best : 0u);
+}
+
 struct Convex {                 // gjk.h:13-29
     ConvexKind kind = CPoint;
     V3 position;
@@ -64,6 +91,19 @@ struct Convex {                 // gjk.h:13-29
     float boundingRadius = 0.0f;
     uint8_t quality = VX_GEOM_NORMAL;
     bool coop = false;  // hull support by the whole wavefront (all lanes, same query)
+    bool lanePts = false;  // coop hull of at most 64 points held one per lane in lanePt
+    V3 lanePt;
+
+    // Turns on wavefront-cooperative support; a hull of at most 64 points is
+    // then loaded once, one point per lane.  Called by every lane.
+    VXD void makeCoop() {
+        coop = true;
+        if (kind == CHull && hullCount <= 64) {
+            const uint32_t lane = __lane_id();
+            lanePt = v3(hullPts[lane < hullCount ? lane : 0]);
+            lanePts = true;
+        }
+    }
 
     // gjk.h:30-148
     VXD V3 supportOffset(V3 dir) const {
@@ -87,6 +127,7 @@ struct Convex {                 // gjk.h:13-29
         }
         case CHull: {
             V3 d = rotateInv(orientation, dir);
+            if (lanePts) return rotate(orientation, hullSupportPointLanes(lanePt, hullCount, d));
             if (coop) return rotate(orientation, v3(hullPts[hullSupportIndexWave(hullPts, hullCount, d)]));
             V3 bestP{};
             float bestDot = 0.0f;
@@ -256,59 +297,89 @@ VXD SV supportVertex(const Convex& A, const Convex& B, V3 dir) {
 }
 struct Simplex { SV v[4]; float bary[4] = {0, 0, 0, 0}; int count = 0; };
 
+// Simplex moves are written as per-component value selects, never as copies
+// between slots chosen by a branch: the compiler turns branch-chosen copies
+// into copies through a pointer chosen at run time, which keeps the simplex
+// in private (scratch) memory instead of registers.
+VXD V3 selV3(bool c, V3 x, V3 y) { return {c ? x.x : y.x, c ? x.y : y.y, c ? x.z : y.z}; }
+VXD SV selSV(bool c, const SV& x, const SV& y) { return {selV3(c, x.p, y.p), selV3(c, x.a, y.a), selV3(c, x.b, y.b)}; }
+VXD Simplex selSimplex(bool c, const Simplex& x, const Simplex& y) {
+    Simplex r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        r.v[i] = selSV(c, x.v[i], y.v[i]);
+        r.bary[i] = c ? x.bary[i] : y.bary[i];
+    }
+    r.count = c ? x.count : y.count;
+    return r;
+}
+
 // gjk.h:317-325
 VXD void solveSegment(Simplex& s) {
     V3 a = s.v[0].p, b = s.v[1].p, ab = b - a;
     float t = -dot(a, ab);
-    if (t <= 0.0f) { s.count = 1; s.bary[0] = 1.0f; return; }
-    float denom = dot(ab, ab);
-    if (t >= denom) { s.v[0] = s.v[1]; s.count = 1; s.bary[0] = 1.0f; return; }
-    t /= denom;
-    s.count = 2; s.bary[0] = 1.0f - t; s.bary[1] = t;
+    const float denom = dot(ab, ab);
+    const bool atA = t <= 0.0f, atB = !atA && t >= denom;
+    if (!atA && !atB) t /= denom;
+    s.v[0] = selSV(atB, s.v[1], s.v[0]);
+    s.count = atA || atB ? 1 : 2;
+    s.bary[0] = atA || atB ? 1.0f : 1.0f - t;
+    s.bary[1] = atA || atB ? s.bary[1] : t;
 }
 
 // gjk.h:327-376 (Ericson's closest point on triangle with a degenerate guard)
 VXD void solveTriangle(Simplex& s) {
+    // Region codes: which vertices the reduced simplex keeps (slot 0 from
+    // src0, slot 1 from src1), its size and weights.
+    int src0 = 0, src1 = 1, count = 3;
+    float w0 = 1.0f, w1 = 0.0f, w2 = 0.0f;
     V3 a = s.v[0].p, b = s.v[1].p, c = s.v[2].p;
     V3 ab = b - a, ac = c - a, ap = -a;
     float d1 = dot(ab, ap), d2 = dot(ac, ap);
-    if (d1 <= 0.0f && d2 <= 0.0f) { s.count = 1; s.bary[0] = 1.0f; return; }
+    bool done = false;
+    if (d1 <= 0.0f && d2 <= 0.0f) { count = 1; w0 = 1.0f; done = true; }
     V3 bp = -b;
     float d3 = dot(ab, bp), d4 = dot(ac, bp);
-    if (d3 >= 0.0f && d4 <= d3) { s.v[0] = s.v[1]; s.count = 1; s.bary[0] = 1.0f; return; }
+    if (!done && d3 >= 0.0f && d4 <= d3) { src0 = 1; count = 1; w0 = 1.0f; done = true; }
     float vc = d1 * d4 - d3 * d2;
-    if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    if (!done && vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
         float t = d1 / (d1 - d3);
-        s.count = 2; s.bary[0] = 1.0f - t; s.bary[1] = t; return;
+        count = 2; w0 = 1.0f - t; w1 = t; done = true;
     }
     V3 cp = -c;
     float d5 = dot(ab, cp), d6 = dot(ac, cp);
-    if (d6 >= 0.0f && d5 <= d6) { s.v[0] = s.v[2]; s.count = 1; s.bary[0] = 1.0f; return; }
+    if (!done && d6 >= 0.0f && d5 <= d6) { src0 = 2; count = 1; w0 = 1.0f; done = true; }
     float vb = d5 * d2 - d1 * d6;
-    if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    if (!done && vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
         float t = d2 / (d2 - d6);
-        s.v[1] = s.v[2];
-        s.count = 2; s.bary[0] = 1.0f - t; s.bary[1] = t; return;
+        src1 = 2; count = 2; w0 = 1.0f - t; w1 = t; done = true;
     }
     float va = d3 * d6 - d5 * d4;
-    if (va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
+    if (!done && va <= 0.0f && (d4 - d3) >= 0.0f && (d5 - d6) >= 0.0f) {
         float t = (d4 - d3) / ((d4 - d3) + (d5 - d6));
-        s.v[0] = s.v[1]; s.v[1] = s.v[2];
-        s.count = 2; s.bary[0] = 1.0f - t; s.bary[1] = t; return;
+        src0 = 1; src1 = 2; count = 2; w0 = 1.0f - t; w1 = t; done = true;
     }
-    float sum = va + vb + vc;
-    if (sum < 1e-20f) {
-        // Nearest of the three vertices (first on ties).
-        const float l0 = lengthSq(s.v[0].p), l1 = lengthSq(s.v[1].p), l2 = lengthSq(s.v[2].p);
-        const int best01 = l1 < l0 ? 1 : 0;
-        const float lb = best01 ? l1 : l0;
-        if (l2 < lb) s.v[0] = s.v[2];
-        else if (best01) s.v[0] = s.v[1];
-        s.count = 1; s.bary[0] = 1.0f;
-        return;
+    if (!done) {
+        float sum = va + vb + vc;
+        if (sum < 1e-20f) {
+            // Nearest of the three vertices (first on ties).
+            const float l0 = lengthSq(s.v[0].p), l1 = lengthSq(s.v[1].p), l2 = lengthSq(s.v[2].p);
+            const int best01 = l1 < l0 ? 1 : 0;
+            const float lb = best01 ? l1 : l0;
+            src0 = l2 < lb ? 2 : best01;
+            count = 1; w0 = 1.0f;
+        } else {
+            float inv = 1.0f / sum;
+            count = 3; w0 = va * inv; w1 = vb * inv; w2 = vc * inv;
+        }
     }
-    float inv = 1.0f / sum;
-    s.count = 3; s.bary[0] = va * inv; s.bary[1] = vb * inv; s.bary[2] = vc * inv;
+    const SV v0 = s.v[0], v1 = s.v[1], v2 = s.v[2];
+    s.v[0] = selSV(src0 == 2, v2, selSV(src0 == 1, v1, v0));
+    s.v[1] = selSV(src1 == 2, v2, v1);
+    s.count = count;
+    s.bary[0] = w0;
+    if (count >= 2) s.bary[1] = w1;
+    if (count == 3) s.bary[2] = w2;
 }
 
 // Loops over simplex vertices are unrolled with a count guard so that every
@@ -324,12 +395,11 @@ VXD V3 simplexPoint(const Simplex& s) {
 }
 
 VXD void simplexPush(Simplex& s, const SV& w) {
-    switch (s.count) {
-    case 0: s.v[0] = w; break;
-    case 1: s.v[1] = w; break;
-    case 2: s.v[2] = w; break;
-    default: s.v[3] = w; break;
-    }
+    const int k = s.count;
+    s.v[0] = selSV(k == 0, w, s.v[0]);
+    s.v[1] = selSV(k == 1, w, s.v[1]);
+    s.v[2] = selSV(k == 2, w, s.v[2]);
+    s.v[3] = selSV(k >= 3, w, s.v[3]);
     ++s.count;
 }
 
@@ -353,7 +423,9 @@ VXD bool solveTetrahedron(Simplex& s, float scale) {
         auto visit = [&](Simplex tri) {
             solveTriangle(tri);
             float d = lengthSq(simplexPoint(tri));
-            if (d < bestD) { bestD = d; best = tri; }
+            const bool better = d < bestD;
+            bestD = better ? d : bestD;
+            best = selSimplex(better, tri, best);
         };
         visit(tetFace<0, 1, 2>(s));
         visit(tetFace<0, 1, 3>(s));
@@ -377,14 +449,16 @@ VXD bool solveTetrahedron(Simplex& s, float scale) {
             Simplex t = tri;
             solveTriangle(t);
             float d = lengthSq(simplexPoint(t));
-            if (d < bestD) { bestD = d; best = t; }
+            const bool better = d < bestD;
+            bestD = better ? d : bestD;
+            best = selSimplex(better, t, best);
         }
     };
     face(tetFace<0, 1, 2>(s), s.v[3].p);
     face(tetFace<0, 1, 3>(s), s.v[2].p);
     face(tetFace<0, 2, 3>(s), s.v[1].p);
     face(tetFace<1, 2, 3>(s), s.v[0].p);
-    if (!inside) s = best;
+    s = selSimplex(!inside, best, s);
     return inside;
 }
 
