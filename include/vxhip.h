@@ -114,9 +114,15 @@ int vxhip_integrate(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, const float gra
 int vxhip_find_contacts(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const vx_soup* soup, float dt,
                         vx_contact* out, uint32_t cap, uint32_t* count);
 
-/* Backend::solveVelocities (backend.h:131-134): colored sweeps over `contacts`.
- * With warmStart != 0 the contacts are put into color order (the array is
- * permuted in place, as the reference's device backend does) and warm started. */
+/* Backend::solveVelocities (backend.h:131-134; backend_cuda.cu:569-690):
+ * colored sweeps over `contacts`.  With warmStart != 0 the contacts are sorted
+ * (std::sort on pair key then point), colored, permuted in place into color
+ * order with the impulses they came in with, uploaded and warm started; the
+ * solved impulses stay on the device for vxhip_fetch_impulses.  A call with
+ * warmStart == 0 reuses that device-resident contact set (m must match) and
+ * replays the sweeps with the step and options of the warm-started call, as
+ * the reference replays the graph it captured then.  Bodies are uploaded and
+ * downloaded on every call. */
 int vxhip_solve_velocities(vxhip_ctx* ctx, vx_body* bodies, uint32_t n, vx_contact* contacts, uint32_t m,
                            float dt, int warmStart, const vx_solver_options* options);
 

@@ -44,5 +44,13 @@ $(OUT)/ref_fixtures: $(HDEPS) $(OUT)/libvelox_strict.a
 $(OUT)/ref_bench: $(HDEPS) $(OUT)/libvelox_relaxed.a
 	$(CXX) $(RELAXED) $(HARNESS) $(OUT)/libvelox_relaxed.a -pthread -o $@
 
+# The HipBackend adapter installed in the strict reference World, linked with
+# this repository's libvxhip.so (built first by velox_amd/build.py).
+REPO    := $(abspath $(dir $(lastword $(MAKEFILE_LIST)))..)
+$(OUT)/ref_hip: $(HDEPS) $(dir $(lastword $(MAKEFILE_LIST)))ref_hip.cpp $(OUT)/libvelox_strict.a $(REPO)/include/vxhip.h
+	$(CXX) $(STRICT) -I$(REPO)/include $(dir $(lastword $(MAKEFILE_LIST)))ref_hip.cpp $(OUT)/libvelox_strict.a \
+	    -L$(REPO)/velox_amd/_build -lvxhip -Wl,-rpath,'$$ORIGIN/../../velox_amd/_build' -pthread -o $@
+
 harness: $(OUT)/ref_fixtures $(OUT)/ref_bench
-.PHONY: harness
+hip: $(OUT)/ref_hip
+.PHONY: harness hip

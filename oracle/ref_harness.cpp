@@ -792,6 +792,42 @@ void narrowSnapshots(const std::string& out, const std::string& name, const vxsc
     }
 }
 
+// Sphere-sphere contacts of the reference's CPU backend (CpuBackend::
+// findContacts, solver.cpp:409-495, whose sphere pairs take the analytic
+// collideSpherePair, solver.cpp:26-52) on the exact-AABB pair set, next to
+// the generic collidePair contacts the device backend computes
+// (narrowphase.h, backend_cuda.cu).  Pairs are given in canonical
+// orientation (lower body index first), as the device forms them.
+void sphereContacts(const std::string& out, const std::string& name, const vxscene::Scene& scene) {
+    World w(BackendType::Cpu);
+    build(w, scene);
+    std::string dir = out + "/" + name;
+    mkdir(dir.c_str(), 0755);
+    writeRecords(dir + "/bodies_in.npy", w.bodies_);
+    writeSoup(dir, w.meshes_);
+    std::vector<uint64_t> pairs;
+    exactPairs(w.bodies_, w.meshes_, scene.dt, pairs);
+    std::unique_ptr<Backend> cpu(createCpuBackend());
+    cpu->setWorkerCount(1);
+    std::vector<Contact> analytic;
+    cpu->findContacts(w.bodies_, w.meshes_, scene.dt, &pairs, analytic);
+    std::vector<Contact> generic;
+    pairContacts(w.bodies_, w.meshes_, scene.dt, pairs, generic);
+    writeNpy(dir + "/pairs.npy", "<u8", {pairs.size()}, pairs.data(), pairs.size() * 8);
+    writeRecords(dir + "/contacts_cpu.npy", analytic);
+    writeRecords(dir + "/contacts.npy", generic);
+    size_t coincident = 0;
+    for (uint64_t p : pairs) {
+        const Body& a = w.bodies_[p >> 32];
+        const Body& b = w.bodies_[uint32_t(p)];
+        const Vec3 d = a.position - b.position;
+        if (dot(d, d) <= 1e-10f) ++coincident;
+    }
+    writeMeta(dir, w, scene.dt, {0}, ", \"coincident\": " + std::to_string(coincident));
+    std::fprintf(stderr, "  %s: %zu pairs (%zu coincident), %zu analytic contacts, %zu generic contacts\n", dir.c_str(),
+                 pairs.size(), coincident, analytic.size(), generic.size());
+}
+
 // Reference integration on edge-case bodies: one velocity integration and one
 // transform advance (CpuBackend::integrate, Body::advanceTransform).
 void integrationCases(const std::string& out) {
@@ -998,6 +1034,10 @@ int fixture(const std::string& out, const std::string& name) {
     }
     if (name == "slabstack") {  // a 55,000+-contact island, ~700 colours (states only: the records are large)
         trajectory(out, name, vxscene::slabStack(), {1, 2, 3}, false, false, false, true);
+        return 0;
+    }
+    if (name == "spheres") {  // sphere pairs: the CPU backend's analytic contact next to collidePair's
+        sphereContacts(out, name, vxscene::sphereCluster());
         return 0;
     }
     if (name == "nslab") {  // per-stage snapshots of a slab on one layer of boxes (~680 colours)

@@ -379,6 +379,32 @@ uint32_t vxo_broadphase(const vx_body* b, uint32_t n, const vx_soup* s, float dt
 int vxo_collide_pair(const vx_body* b, const vx_soup* s, uint32_t i, uint32_t j, float dt, vx_contact* out) {
     return collidePair(b[i], b[j], i, j, soupOf(s), dt, out, kMaxContactsPerPair);
 }
+// The reference CPU backend's sphere-sphere contact (collideSpherePair,
+// solver.cpp:26-52, taken by CpuBackend::findContacts, solver.cpp:478-483):
+// returns -1 where it defers to the generic collidePair (coincident centres),
+// 0 when the pair is beyond its reach ((|va| + |vb|) dt + 1e-3, with a 1e-5
+// relative band), else the count emit() leaves (0 or 1).  Its reach omits the
+// two speculative distances emit() adds, so separated pairs between the two
+// reaches get no contact here but one from the generic path.
+int vxo_sphere_pair_cpu(const vx_body* bodies, uint32_t i, uint32_t j, float dt, vx_contact* out) {
+    const vx_body& a = bodies[i];
+    const vx_body& b = bodies[j];
+    const V3 delta = v3(a.position) - v3(b.position);
+    const float distanceSq = dot(delta, delta);
+    if (distanceSq <= 1e-10f) return -1;
+    const float travel = (maxPointSpeed(a) + maxPointSpeed(b)) * dt + 1e-3f;
+    const float maximumDistance = a.radius + b.radius + travel;
+    const float safeDistance = maximumDistance + 1e-5f * fmx(1.0f, maximumDistance);
+    if (distanceSq > safeDistance * safeDistance) return 0;
+    const float distance = sqrtf(distanceSq);
+    const V3 normal = delta * (1.0f / distance);
+    const float gap = distance - a.radius - b.radius;
+    const V3 pointA = v3(a.position) - normal * a.radius;
+    const V3 pointB = v3(b.position) + normal * b.radius;
+    int count = 0;
+    emit(a, b, i, j, normal, (pointA + pointB) * 0.5f, gap, dt, out, kMaxContactsPerPair, count);
+    return count;
+}
 uint32_t vxo_find_contacts(const vx_body* b, uint32_t n, const vx_soup* s, float dt, vx_contact* out, uint32_t cap) {
     Soup soup = soupOf(s);
     std::vector<vx_contact> cs;

@@ -42,6 +42,9 @@ def main():
     # ~700-900 colours (states, counts and the last step's events only).
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "slabstack"],
                           cwd=ROOT)
+    # Sphere pairs: the CPU backend's analytic sphere-sphere contacts next to
+    # the generic collidePair contacts of the same pairs.
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "spheres"], cwd=ROOT)
     # The 1,000-box lattice with sleeping on over 300 steps (several minutes).
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "lattice1000_sleep"],
                           cwd=ROOT)

@@ -101,6 +101,7 @@ def test_colored_solve_matches_reference(ctx, name):
     h = np.float32(case["meta"]["dt"] / case["meta"]["substeps"])
     ctx.solve_velocities(bodies, cs, h, True)
     assert bits_equal(body_state(bodies), as_state(case["solve_bodies"]))
+    ctx.fetch_impulses(cs)  # the solved impulses stay on the device until fetched
     imp = np.stack([cs["point"][:, 0], cs["point"][:, 1], cs["point"][:, 2], cs["normalImpulse"],
                     cs["tangentImpulse1"], cs["tangentImpulse2"], cs["rollingImpulse1"], cs["rollingImpulse2"],
                     cs["spinningImpulse"]], axis=1).astype(np.float32)

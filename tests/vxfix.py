@@ -103,6 +103,7 @@ def oracle():
             "vxo_body_aabbs": (None, [vp, u32, vp, f32, vp]),
             "vxo_broadphase": (u32, [vp, u32, vp, f32, vp, u32]),
             "vxo_collide_pair": (i32, [vp, vp, u32, u32, f32, vp]),
+            "vxo_sphere_pair_cpu": (i32, [vp, u32, u32, f32, vp]),
             "vxo_find_contacts": (u32, [vp, u32, vp, f32, vp, u32]),
             "vxo_warm_start_match": (None, [vp, u32, vp, u32]),
             "vxo_std_sort": (None, [vp, vp, u32]),

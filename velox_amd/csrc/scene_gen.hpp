@@ -113,7 +113,7 @@ inline double hullVolume(const float* p, uint32_t n) {
 // Velox's demo scenes seed the same way; one seed per workload.
 constexpr uint32_t kSeedLattice = 1001u, kSeedDebris = 2002u, kSeedCcd = 3003u,
                    kSeedBatch = 4004u, kSeedDiff = 5005u,
-                   kSeedSleep = 6006u, kSeedToi = 7007u;
+                   kSeedSleep = 6006u, kSeedToi = 7007u, kSeedSpheres = 8008u;
 
 // 1,000 boxes (half-extent 0.5, density 1000) in a 10x10x10 lattice at 1.1 m
 // spacing with U(-0.05,0.05) jitter and uniform orientations, on a static
@@ -234,6 +234,55 @@ inline Scene freeFlight(uint32_t count = 128, uint32_t seed = 77u) {
 // small volume so that manifolds, EPA and deep stacks all occur.
 inline Scene pile(uint32_t count = 240, float X = 6.0f, float Y = 8.0f, uint32_t seed = 606u) {
     Scene s = debris(count, X, Y, X, seed);
+    return s;
+}
+
+// Sphere pairs at every separation the sphere-sphere contact distinguishes:
+// `count` spheres r~U(.1,.5) at uniform positions in a `side`-metre cube
+// (overlapping, touching and separated neighbours), v~N(0,1) so the
+// speculative reach varies per pair, plus eight pairs of spheres whose centres
+// coincide exactly and, for each of 64 seeded spheres, a partner placed at a
+// distance within a few float steps of the pair's speculative reach.  No plane.
+inline Scene sphereCluster(uint32_t count = 600, float side = 5.0f, uint32_t seed = kSeedSpheres) {
+    Scene s;
+    Rng r(seed);
+    auto sphere = [&](float x, float y, float z, float rad) {
+        BodySpec b;
+        b.kind = Sphere;
+        b.radius = rad;
+        b.mass = 1000.0f * (4.0f / 3.0f) * 3.14159265f * rad * rad * rad;
+        b.pos[0] = x; b.pos[1] = y; b.pos[2] = z;
+        for (int a = 0; a < 3; ++a) b.vel[a] = r.normal();
+        s.bodies.push_back(b);
+        return s.bodies.size() - 1;
+    };
+    for (uint32_t i = 0; i < count; ++i)
+        sphere(r.range(0.0f, side), r.range(0.0f, side), r.range(0.0f, side), r.range(0.1f, 0.5f));
+    for (int k = 0; k < 8; ++k) {  // coincident centres
+        const float x = side + 2.0f + 3.0f * (float)k, y = 1.0f, z = 1.0f;
+        sphere(x, y, z, r.range(0.1f, 0.5f));
+        sphere(x, y, z, r.range(0.1f, 0.5f));
+    }
+    for (int k = 0; k < 64; ++k) {  // partners at the edge of the speculative reach
+        const float x = side + 2.0f + 3.0f * (float)(k % 8), y = 4.0f + 3.0f * (float)(k / 8), z = 1.0f;
+        const size_t a = sphere(x, y, z, r.range(0.1f, 0.5f));
+        const float ra = s.bodies[a].radius, rb = r.range(0.1f, 0.5f);
+        // maxPointSpeed of a sphere is its speed (no angular velocity); the
+        // reach is ra + rb + (|va| + |vb|) * dt + 1e-3, placed along x.
+        BodySpec b;
+        const float va = sqrtf(s.bodies[a].vel[0] * s.bodies[a].vel[0] + s.bodies[a].vel[1] * s.bodies[a].vel[1] +
+                               s.bodies[a].vel[2] * s.bodies[a].vel[2]);
+        float vb[3] = {r.normal(), r.normal(), r.normal()};
+        const float vbl = sqrtf(vb[0] * vb[0] + vb[1] * vb[1] + vb[2] * vb[2]);
+        const float reach = ra + rb + (va + vbl) * s.dt + 1e-3f;
+        const float off = reach * (1.0f + 1e-5f * (float)((int)(k % 9) - 4));
+        b.kind = Sphere;
+        b.radius = rb;
+        b.mass = 1000.0f * (4.0f / 3.0f) * 3.14159265f * rb * rb * rb;
+        b.pos[0] = x + off; b.pos[1] = y; b.pos[2] = z;
+        b.vel[0] = vb[0]; b.vel[1] = vb[1]; b.vel[2] = vb[2];
+        s.bodies.push_back(b);
+    }
     return s;
 }
 

@@ -239,6 +239,8 @@ struct vxhip_ctx {
     uint32_t contactCount = 0;
     uint32_t prevCount = 0;
     uint32_t solveCount = 0;
+    float solveH = 0.0f;               // step of the last warm-started vxhip_solve_velocities
+    vx_solver_options solveOptions{};  // and its options, reused by the calls after it
     uint32_t numColors = 0;
     std::vector<uint32_t> colorStartHost;
     vxhip_stats stats{};
@@ -418,6 +420,19 @@ void validateScene(const vx_body* b, uint32_t n, const vx_soup* s) {
             (!s || x.compoundCount == 0 || (uint64_t)x.compoundFirst + x.compoundCount > s->compoundChildCount))
             fail(VXHIP_ERR_INVALID, "vxhip: compound body references invalid child storage");
     }
+}
+
+// The per-call (Backend-equivalent) entry points share the context's device
+// buffers with the resident world: they detach it first, so a later
+// vxhip_world_step refuses to run on bodies, caches and settings that belong
+// to another call sequence (the world is loaded again to resume).
+void detachWorld(vxhip_ctx* c) {
+    c->loaded = false;
+    c->prevCount = 0;
+    c->prevExact = false;
+    c->evPrevCount = 0;
+    c->eventCount = 0;
+    c->sleepOn = false;
 }
 
 void setBodies(vxhip_ctx* c, const vx_body* b, uint32_t n) {
@@ -1790,6 +1805,8 @@ int vxhip_world_step(vxhip_ctx* c, float dt, uint32_t steps) {
     return guarded([&] {
         if (!c || !c->loaded) fail(VXHIP_ERR_INVALID, "vxhip: no world loaded");
         if (!(dt >= 0.0f) || dt > 1e30f) fail(VXHIP_ERR_INVALID, "vxhip: timestep must be finite and non-negative");
+        if (c->sleepOn && c->n > kSleepMaxBodies)  // bodies replaced since sleeping was switched on
+            fail(VXHIP_ERR_CAPACITY, "vxhip: device sleeping is limited to 256 bodies in this build");
         if (dt == 0.0f) return;
         for (uint32_t i = 0; i < steps; ++i) worldStep(c, dt);
     });
@@ -2322,6 +2339,7 @@ int vxhip_integrate(vxhip_ctx* c, vx_body* bodies, uint32_t n, const float gravi
         injectFailureForTesting();
         if (!c || (!bodies && n) || !gravity) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
         if (n == 0) return;
+        detachWorld(c);
         setBodies(c, bodies, n);
         kIntegrate<<<blocks(n), kThreads, 0, c->stream>>>(c->bodies.p, n, V3{gravity[0], gravity[1], gravity[2]}, dt);
         HIPCK(hipGetLastError());
@@ -2345,6 +2363,7 @@ int vxhip_probe_pairs(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_
         *count = 0;
         if (n < 2) return;
         uploadSoup(c, soup);
+        detachWorld(c);
         setBodies(c, bodies, n);
         const uint32_t np = broadPhase(c, dt);
         *count = np;
@@ -2364,6 +2383,7 @@ int vxhip_find_contacts(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const v
         *count = 0;
         if (n < 2) return;
         uploadSoup(c, soup);
+        detachWorld(c);
         setBodies(c, bodies, n);
         const uint32_t np = broadPhase(c, dt);
         const uint32_t m = narrowPhase(c, np, dt);
@@ -2378,6 +2398,7 @@ int vxhip_probe_color(vxhip_ctx* c, const vx_body* bodies, uint32_t n, const vx_
                       uint32_t* colors, uint32_t* numColors) {
     return guarded([&] {
         if (!c || !numColors || (!bodies && n) || (!contacts && m)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        detachWorld(c);
         setBodies(c, bodies, n);
         c->contacts.reserve(m ? m : 1);
         if (m) HIPCK(hipMemcpyAsync(c->contacts.p, contacts, (size_t)m * sizeof(vx_contact), hipMemcpyHostToDevice, c->stream));
@@ -2463,10 +2484,12 @@ static bool hostCanonicalLess(const vx_contact& x, const vx_contact& y) {  // ba
     return x.point.z < y.point.z;
 }
 
-// First-substep work of solveVelocities: canonical order, colour, colour
-// order, warm start.  Leaves the colour-ordered contacts in c->solve.
+// First-substep work of solveVelocities: canonical order (std::sort with
+// the reference's comparator, so contacts that compare equal end where
+// libstdc++ leaves them there), colour, colour order.  Leaves the
+// colour-ordered contacts in c->solve.
 static void prepareSolve(vxhip_ctx* c, vx_contact* contacts, uint32_t m) {
-    std::stable_sort(contacts, contacts + m, hostCanonicalLess);
+    std::sort(contacts, contacts + m, hostCanonicalLess);
     c->contacts.reserve(m);
     HIPCK(hipMemcpyAsync(c->contacts.p, contacts, (size_t)m * sizeof(vx_contact), hipMemcpyHostToDevice, c->stream));
     const uint32_t K = colorContacts(c, c->contacts.p, m);
@@ -2484,14 +2507,25 @@ int vxhip_solve_velocities(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact
         if (!c || !options || (!bodies && n) || (!contacts && m)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
         c->lastIterations = 0;
         if (m == 0) return;
+        if (!warmStart && c->solveCount != m) fail(VXHIP_ERR_INVALID, "vxhip: solve without a warm-started contact set");
+        detachWorld(c);
         setBodies(c, bodies, n);
-        if (warmStart) prepareSolve(c, contacts, m);
-        else if (c->solveCount != m) fail(VXHIP_ERR_INVALID, "vxhip: solve without a warm-started contact set");
-        substepSolve(c, c->solve.p, c->numColors, dt, *options, warmStart != 0, true);
-        c->lastIterations = (uint32_t)(2 * options->velocityIterations);
-        HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
-        if (warmStart)
+        if (warmStart) {
+            prepareSolve(c, contacts, m);
+            // The caller's contacts come back in colour order with the
+            // impulses they came in with (the reference permutes its host
+            // copy and solves on the device one; fetchImpulses reads those).
             HIPCK(hipMemcpyAsync(contacts, c->solve.p, (size_t)m * sizeof(vx_contact), hipMemcpyDeviceToHost, c->stream));
+            // Later substeps replay the sweeps recorded here, with this
+            // call's step and options (the reference captures them in a
+            // graph at the warm-started call and replays it).
+            c->solveH = dt;
+            c->solveOptions = *options;
+        }
+        substepSolve(c, c->solve.p, c->numColors, c->solveH, c->solveOptions, warmStart != 0, true);
+        // Only the warm-started call reports its sweeps (backend_cuda.cu:573, 665).
+        if (warmStart) c->lastIterations = (uint32_t)(2 * c->solveOptions.velocityIterations);
+        HIPCK(hipMemcpyAsync(bodies, c->bodies.p, (size_t)n * sizeof(vx_body), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
     });
 }
@@ -2520,6 +2554,7 @@ int vxhip_advance_substeps(vxhip_ctx* c, vx_body* bodies, uint32_t n, vx_contact
             a.sleepTimer = b.sleepTimer = 0.0f;
         }
         if (options->iterationPolicy == VX_ITERATION_ADAPTIVE) return;  // backend_cuda.cu:714-715
+        detachWorld(c);
         setBodies(c, bodies, n);
         const V3 g{gravity[0], gravity[1], gravity[2]};
         if (m) prepareSolve(c, contacts, m);

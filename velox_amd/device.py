@@ -564,7 +564,13 @@ class Context:
         return bool(adv.value)
 
     def fetch_impulses(self, contacts):
+        """The accumulated impulses of the last solve into `contacts` (in the
+        order the warm-started solve left them); untouched when the count
+        differs or nothing was solved since the last invalidation."""
         check(library().vxhip_fetch_impulses(self._h, ptr(contacts), len(contacts)))
+
+    def invalidate_caches(self):
+        check(library().vxhip_invalidate_caches(self._h))
 
     def last_velocity_iterations(self):
         return library().vxhip_last_velocity_iterations(self._h)
