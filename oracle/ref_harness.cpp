@@ -792,6 +792,60 @@ void narrowSnapshots(const std::string& out, const std::string& name, const vxsc
     }
 }
 
+// Physical tolerances of a trajectory (stress_demo.cpp:27-133,
+// test_numerical.cpp:202-226): after every step the deepest contact (the
+// most negative gap of the step's contacts) and the total energy (kinetic,
+// rotational about the principal axes, and potential of the dynamic bodies),
+// with the states at `steps`, from the reference World with the device
+// backend's semantics.
+void physicsTrajectory(const std::string& out, const std::string& name, const vxscene::Scene& scene,
+                       const std::vector<int>& steps) {
+    std::string dir = out + "/" + name;
+    mkdir(dir.c_str(), 0755);
+    World w(BackendType::Cpu);
+    build(w, scene);
+    w.backend_.reset(new CudaSemantics());
+    writeRecords(dir + "/bodies_in.npy", w.bodies_);
+    writeSoup(dir, w.meshes_);
+    std::vector<double> minGap, energy;
+    std::vector<uint32_t> contacts;
+    const int last = steps.back();
+    size_t next = 0;
+    for (int k = 1; k <= last; ++k) {
+        w.step(scene.dt);
+        double g = 0.0;
+        for (const Contact& c : w.contacts_) g = std::min(g, (double)c.gap);
+        double e = 0.0;
+        for (const Body& b : w.bodies_) {
+            if (!b.isDynamic() || b.invMass <= 0.0f) continue;
+            const double m = 1.0 / b.invMass;
+            const Vec3 v = b.velocity, om = b.angularVelocity;
+            e += 0.5 * m * ((double)v.x * v.x + (double)v.y * v.y + (double)v.z * v.z);
+            e += m * 9.81 * (double)b.position.y;
+            // I = R diag(1/invI) R^T in the inertia frame at the body's orientation
+            const Quat f = b.inertiaFrameAt(b.orientation);
+            const Vec3 l = rotateInv(f, om);
+            const double ix = b.invInertia.x > 0 ? 1.0 / b.invInertia.x : 0.0;
+            const double iy = b.invInertia.y > 0 ? 1.0 / b.invInertia.y : 0.0;
+            const double iz = b.invInertia.z > 0 ? 1.0 / b.invInertia.z : 0.0;
+            e += 0.5 * (ix * (double)l.x * l.x + iy * (double)l.y * l.y + iz * (double)l.z * l.z);
+        }
+        minGap.push_back(g);
+        energy.push_back(e);
+        contacts.push_back((uint32_t)w.contacts_.size());
+        if (next < steps.size() && k == steps[next]) {
+            writeRecords(dir + "/state_" + std::to_string(k) + ".npy", states(w.bodies_));
+            ++next;
+            std::fprintf(stderr, "  %s step %d: %zu contacts, deepest %.4g m, energy %.6g J\n", name.c_str(), k,
+                         w.contacts_.size(), g, e);
+        }
+    }
+    writeNpy(dir + "/min_gap.npy", "<f8", {minGap.size()}, minGap.data(), minGap.size() * 8);
+    writeNpy(dir + "/energy.npy", "<f8", {energy.size()}, energy.data(), energy.size() * 8);
+    writeNpy(dir + "/contacts_per_step.npy", "<u4", {contacts.size()}, contacts.data(), contacts.size() * 4);
+    writeMeta(dir, w, scene.dt, steps, "");
+}
+
 // Sphere-sphere contacts of the reference's CPU backend (CpuBackend::
 // findContacts, solver.cpp:409-495, whose sphere pairs take the analytic
 // collideSpherePair, solver.cpp:26-52) on the exact-AABB pair set, next to
@@ -1034,6 +1088,10 @@ int fixture(const std::string& out, const std::string& name) {
     }
     if (name == "slabstack") {  // a 55,000+-contact island, ~700 colours (states only: the records are large)
         trajectory(out, name, vxscene::slabStack(), {1, 2, 3}, false, false, false, true);
+        return 0;
+    }
+    if (name == "ccdsmall") {  // the CCD stress scene reduced to a size the strict CPU steps in seconds
+        physicsTrajectory(out, name, vxscene::ccdStressSmall(), {1, 2, 5, 10, 30, 60, 120});
         return 0;
     }
     if (name == "spheres") {  // sphere pairs: the CPU backend's analytic contact next to collidePair's

@@ -42,6 +42,9 @@ def main():
     # ~700-900 colours (states, counts and the last step's events only).
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "slabstack"],
                           cwd=ROOT)
+    # The CCD stress scene reduced to a size the strict CPU steps in seconds:
+    # 120 steps with the deepest contact and total energy of every step.
+    subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "ccdsmall"], cwd=ROOT)
     # Sphere pairs: the CPU backend's analytic sphere-sphere contacts next to
     # the generic collidePair contacts of the same pairs.
     subprocess.check_call([os.path.join(ROOT, "oracle", "_ref", "ref_fixtures"), "fixture", HERE, "spheres"], cwd=ROOT)

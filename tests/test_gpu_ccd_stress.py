@@ -57,3 +57,75 @@ def test_ccd_stress_no_tunnelling():
         assert not below.any(), f"{below.sum()} spinning boxes under the terrain"
     finally:
         c.close()
+
+
+def _energy(bodies):
+    """Kinetic, rotational (principal axes of the inertia frame) and potential
+    energy of the dynamic bodies, in double precision."""
+    from scipy.spatial.transform import Rotation
+    dyn = (bodies["motionType"] == 2) & (bodies["invMass"] > 0)
+    b = bodies[dyn]
+    m = 1.0 / b["invMass"].astype(np.float64)
+    v = b["velocity"].astype(np.float64)
+    e = 0.5 * m * (v * v).sum(1) + m * 9.81 * b["position"][:, 1].astype(np.float64)
+    # inertia frame = orientation * inertiaOrientation (identity for these bodies)
+    q = b["orientation"].astype(np.float64)
+    qi = b["inertiaOrientation"].astype(np.float64)
+    f = Rotation.from_quat(q) * Rotation.from_quat(qi)
+    local = f.inv().apply(b["angularVelocity"].astype(np.float64))
+    inv = b["invInertia"].astype(np.float64)
+    inertia = np.where(inv > 0, 1.0 / np.where(inv > 0, inv, 1.0), 0.0)
+    return float(e.sum() + 0.5 * (inertia * local * local).sum())
+
+
+def test_reduced_ccd_stress_matches_strict_cpu():
+    """The CCD stress scene reduced to a size the strict reference steps in
+    seconds (10 x 10 wall, 256 bullets at 2 km/s, 16 boxes at 1.5 km/s tip
+    speed, 64 x 64 heightfield), all 120 steps: states bit for bit at the
+    fixture's steps, the deepest contact of every step equal to the strict
+    reference's (stress_demo.cpp:27-133 checks penetration this way), the
+    total energy of every step within 1e-6 relative of it (recomputed here in
+    double precision from the same states), no energy gain beyond 1e-3 of the start between
+    steps, and no bullet aimed at the wall through it in the impact step.  (From
+    step 2 on, ricochets off the struck wall leave at up to 1 km/s; some end up
+    past the wall or under the small terrain in the strict reference exactly
+    as on the device, which the bitwise states pin.)"""
+    from velox_amd.device import Context, Scene
+    from vxfix import as_state, bits_equal, body_state, load_case
+    case = load_case("ccdsmall")
+    sc = Scene("ccdsmall")
+    assert np.array_equal(np.ascontiguousarray(sc.bodies).view(np.uint8).reshape(-1),
+                          np.ascontiguousarray(case["bodies_in"]).view(np.uint8).reshape(-1))
+    bullets = sc.bodies["shape"] == 0
+    wall = (sc.bodies["shape"] == 2) & (sc.bodies["position"][:, 0] > 10.0)
+    wall_front, wall_back = 10.0, 11.0
+    wall_lo = (sc.bodies["position"][wall, 1].min() - 0.5, sc.bodies["position"][wall, 2].min() - 0.5)
+    wall_hi = (sc.bodies["position"][wall, 1].max() + 0.5, sc.bodies["position"][wall, 2].max() + 0.5)
+    assert wall.sum() == 100
+    c = Context()
+    try:
+        c.load_scene(sc)
+        gaps, energy = [], []
+        for step in range(1, 121):
+            c.step(sc.dt, 1)
+            cs = c.contacts()
+            gaps.append(min(0.0, float(cs["gap"].min())) if len(cs) else 0.0)
+            b = c.bodies()
+            energy.append(_energy(b))
+            if step == 1:  # the impact step: a bullet past the wall must have missed it (aimed outside its face)
+                past = bullets & (b["position"][:, 0] > wall_back + BULLET_R)
+                p0, v0 = sc.bodies["position"][past].astype(np.float64), sc.bodies["velocity"][past].astype(np.float64)
+                t = (wall_front - p0[:, 0]) / v0[:, 0]
+                y, z = p0[:, 1] + v0[:, 1] * t, p0[:, 2] + v0[:, 2] * t
+                aimed = (y > wall_lo[0] + 0.1) & (y < wall_hi[0] - 0.1) & (z > wall_lo[1] + 0.1) & (z < wall_hi[1] - 0.1)
+                assert not aimed.any(), f"step {step}: {aimed.sum()} bullets through the wall"
+            if step in case["meta"]["steps"]:
+                assert bits_equal(body_state(b), as_state(case[f"state_{step}"])), f"step {step}"
+                print(f"step {step}: {len(cs)} contacts, deepest {gaps[-1]:.4g} m, energy {energy[-1]:.6g} J",
+                      flush=True)
+    finally:
+        c.close()
+    assert np.array_equal(np.array(gaps), case["min_gap"].astype(np.float64))
+    np.testing.assert_allclose(energy, case["energy"], rtol=1e-6)
+    e = np.array(energy)
+    assert (np.diff(e) <= 1e-3 * e[0]).all()

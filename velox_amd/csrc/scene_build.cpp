@@ -381,6 +381,7 @@ vxscene::Scene vxSceneByName(const std::string& name, uint32_t param, bool& ok) 
     if (name == "freeflight") return vxscene::freeFlight(param ? param : 128);
     if (name == "batch") return vxscene::batchWorlds(param ? param : 4096);
     if (name == "ccdstress") return vxscene::ccdStress();
+    if (name == "ccdsmall") return vxscene::ccdStressSmall();
     if (name == "settle") return vxscene::settle();
     if (name == "hightoi") return vxscene::highToi();
     if (name == "slabstack") return vxscene::slabStack();

@@ -620,13 +620,16 @@ inline Scene highToi(uint32_t seed = kSeedToi) {
 // 10-degree cone at a wall of 100 x 100 stacked unit boxes; 256 unit boxes
 // spinning at 1.5 km/s tip speed (1732 rad/s about random axes) above a
 // 256 x 256 heightfield at 0.5 m spacing whose heights are five octaves of
-// seeded sinusoidal noise scaled to 3 m amplitude.
-inline Scene ccdStress(uint32_t seed = kSeedCcd) {
+// seeded sinusoidal noise scaled to 3 m amplitude.  The reduced copy
+// (ccdStressSmall) keeps every ingredient at a size the strict CPU reference
+// steps in seconds: a 10 x 10 wall at x = 10.5, 16 x 16 bullets, 4 x 4
+// spinning boxes and a 64 x 64 heightfield.
+inline Scene ccdStress(uint32_t seed = kSeedCcd, int wall = 100, int bulletSide = 64, int spinSide = 16,
+                       uint32_t side = 256, float wallX = 20.5f) {
     Scene s;
     Rng r(seed);
     const float pi = 3.14159265f;
     HeightfieldSpec hf;
-    const uint32_t side = 256;
     hf.width = hf.depth = side;
     hf.cell = 0.5f;
     hf.origin[0] = hf.origin[2] = -0.5f * hf.cell * (float)(side - 1);
@@ -660,18 +663,21 @@ inline Scene ccdStress(uint32_t seed = kSeedCcd) {
         b.restitution = 0.0f;
         return b;
     };
-    for (int j = 0; j < 100; ++j)  // the wall, x in [20, 21]
-        for (int k = 0; k < 100; ++k) s.bodies.push_back(box(20.5f, 4.5f + 1.002f * (float)j, -49.5f + 1.002f * (float)k));
+    const float W = (float)wall;
+    for (int j = 0; j < wall; ++j)  // the wall, x in [wallX - 0.5, wallX + 0.5]
+        for (int k = 0; k < wall; ++k)
+            s.bodies.push_back(box(wallX, 4.5f + 1.002f * (float)j, -0.5f * (W - 1.0f) + 1.002f * (float)k));
     const float cone = 10.0f * pi / 180.0f;
-    for (int j = 0; j < 64; ++j)  // bullets
-        for (int k = 0; k < 64; ++k) {
+    const float nb1 = (float)(bulletSide - 1);
+    for (int j = 0; j < bulletSide; ++j)  // bullets
+        for (int k = 0; k < bulletSide; ++k) {
             BodySpec b;
             b.kind = Sphere;
             b.radius = 0.05f;
             b.mass = 1000.0f * (4.0f / 3.0f) * pi * 0.05f * 0.05f * 0.05f;
             b.pos[0] = -10.0f;
-            b.pos[1] = 10.0f + 80.0f * (float)j / 63.0f;
-            b.pos[2] = -40.0f + 80.0f * (float)k / 63.0f;
+            b.pos[1] = 0.1f * W + 0.8f * W * (float)j / nb1;
+            b.pos[2] = -0.4f * W + 0.8f * W * (float)k / nb1;
             // Uniform direction in the cone about +x.
             const float cosT = 1.0f - r.next() * (1.0f - std::cos(cone)), sinT = std::sqrt(1.0f - cosT * cosT);
             const float phi = r.range(0.0f, 2.0f * pi);
@@ -683,9 +689,10 @@ inline Scene ccdStress(uint32_t seed = kSeedCcd) {
             s.bodies.push_back(b);
         }
     const float omega = 1500.0f / 0.8660254f;
-    for (int j = 0; j < 16; ++j)  // spinning boxes above the terrain
-        for (int k = 0; k < 16; ++k) {
-            BodySpec b = box(-50.0f + 3.0f * (float)j, 6.0f + r.range(0.0f, 2.0f), -24.0f + 3.0f * (float)k);
+    const float sx = -50.0f * (float)spinSide / 16.0f, sz = -24.0f * (float)spinSide / 16.0f;
+    for (int j = 0; j < spinSide; ++j)  // spinning boxes above the terrain
+        for (int k = 0; k < spinSide; ++k) {
+            BodySpec b = box(sx + 3.0f * (float)j, 6.0f + r.range(0.0f, 2.0f), sz + 3.0f * (float)k);
             r.quat(b.quat);
             float ax = r.normal(), ay = r.normal(), az = r.normal();
             const float l = std::sqrt(ax * ax + ay * ay + az * az);
@@ -694,5 +701,7 @@ inline Scene ccdStress(uint32_t seed = kSeedCcd) {
         }
     return s;
 }
+
+inline Scene ccdStressSmall(uint32_t seed = kSeedCcd) { return ccdStress(seed, 10, 16, 4, 64, 10.5f); }
 
 }  // namespace vxscene
