@@ -248,6 +248,10 @@ struct vxhip_ctx {
     bool loaded = false;
     // profiling (vxhip_world_profile): events on this context's stream
     bool profile = false;
+    // Island batches run beside the big-island colour chain (disjoint bodies
+    // and contacts) on the device's shared side stream, forked and joined
+    // with these events.
+    hipEvent_t forkEv = nullptr, joinEv = nullptr;
     std::vector<hipEvent_t> stageEv;     // one per stage boundary
     std::vector<hipEvent_t> launchEv;    // pairs around colour-solve launches
     uint32_t launchUsed = 0;
@@ -784,12 +788,36 @@ void prepareTail(vxhip_ctx* c, uint32_t K, SolveArrays sa) {
     c->tailColor = t;
 }
 
+// The side stream of the current device: one per device for the whole
+// process (every context forks onto it), so extra contexts take no extra
+// hardware queue.  Default priority; created on first use, never destroyed.
+constexpr int kMaxSideDevices = 64;
+hipStream_t gSide[kMaxSideDevices] = {};
+std::mutex gSideMutex;
+
+hipStream_t sideStream(vxhip_ctx* c) {
+    if (c->device < 0 || c->device >= kMaxSideDevices) return c->stream;
+    std::lock_guard<std::mutex> lock(gSideMutex);
+    if (!gSide[c->device]) HIPCK(hipStreamCreateWithFlags(&gSide[c->device], hipStreamNonBlocking));
+    if (!c->forkEv) HIPCK(hipEventCreateWithFlags(&c->forkEv, hipEventDisableTiming));
+    if (!c->joinEv) HIPCK(hipEventCreateWithFlags(&c->joinEv, hipEventDisableTiming));
+    return gSide[c->device];
+}
+
 // One pass (warm start, or 2*I sweeps) over every contact: small islands in
-// one launch, big islands colour by colour.
+// one launch, big islands colour by colour.  When both exist the island
+// batches go to the side stream and run beside the big-island chain: they
+// touch no big-island body and no big-island contact.  (Not while
+// profiling: the per-launch events time one kernel at a time.)
 template <int kMode>
 void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
     hipStream_t s = c->stream;
-    const hipStream_t bs = s;
+    const bool fork = c->batchCount && c->bigCount && !c->profile;
+    const hipStream_t bs = fork ? sideStream(c) : s;
+    if (fork) {
+        HIPCK(hipEventRecord(c->forkEv, s));
+        HIPCK(hipStreamWaitEvent(bs, c->forkEv, 0));
+    }
     if (c->batchCount) {
         const uint32_t lds = c->ldsBatchCount, global = c->batchCount - c->ldsBatchCount;
         if (lds) {
@@ -797,7 +825,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
             const uint64_t visits = (uint64_t)c->ldsContactCount * sweeps;
             solveLaunchBegin(c);
             ++c->stats.launches[VXHIP_LAUNCH_BATCH];
-            kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, s>>>(
+            kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, bs>>>(
                 sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
                 c->bodyList.p, c->counters.p + cRanges, sweeps, c->counters.p + cBatchError);
             solveLaunchEnd(c, kMode == 2 ? VXHIP_KERNEL_BATCH_WARM : VXHIP_KERNEL_BATCH, visits);
@@ -805,7 +833,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
         if (global) {
             solveLaunchBegin(c);
             ++c->stats.launches[VXHIP_LAUNCH_ISLAND];
-            kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, s>>>(
+            kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, bs>>>(
                 sa, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
             solveLaunchEnd(c, VXHIP_KERNEL_ISLAND, (uint64_t)(c->smallCount - c->ldsContactCount) * sweeps);
         }
@@ -819,7 +847,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 if (kMode != 2 && col == c->tailColor) {
                     solveLaunchBegin(c);
                     ++c->stats.launches[VXHIP_LAUNCH_TAIL];
-                    kSweepTailPair<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, K,
+                    kSweepTailPair<kMode><<<1, kSweepRunThreads, 0, s>>>(sa, c->smallCount, c->bigColorStart.p, col, K,
                                                                          c->tailRowSlot.p, c->tailSlotBody.p,
                                                                          c->counters.p + cTailSlots);
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[K] - cs[col]);
@@ -828,7 +856,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                 if (cnt > kNarrowSweepColor) {
                     solveLaunchBegin(c);
                     ++c->stats.launches[VXHIP_LAUNCH_COLOR_PAIR];
-                    kSweepColorPair<kMode><<<blocks(2 * cnt), kThreads, 0, bs>>>(sa, c->smallCount, cs[col], cnt);
+                    kSweepColorPair<kMode><<<blocks(2 * cnt), kThreads, 0, s>>>(sa, c->smallCount, cs[col], cnt);
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cnt);
                     ++col;
                     continue;
@@ -838,10 +866,14 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                     ++end;
                 solveLaunchBegin(c);
                 ++c->stats.launches[VXHIP_LAUNCH_COLOR_RUN];
-                kSweepColorRun<kMode><<<1, kSweepRunThreads, 0, bs>>>(sa, c->smallCount, c->bigColorStart.p, col, end);
+                kSweepColorRun<kMode><<<1, kSweepRunThreads, 0, s>>>(sa, c->smallCount, c->bigColorStart.p, col, end);
                 solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[end] - cs[col]);
                 col = end;
             }
+    }
+    if (fork) {
+        HIPCK(hipEventRecord(c->joinEv, bs));
+        HIPCK(hipStreamWaitEvent(s, c->joinEv, 0));
     }
     HIPCK(hipGetLastError());
 }
@@ -1773,6 +1805,8 @@ void vxhip_destroy(vxhip_ctx* c) {
     // Device buffers are released by their destructors below.
     for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
+    if (c->forkEv) (void)hipEventDestroy(c->forkEv);
+    if (c->joinEv) (void)hipEventDestroy(c->joinEv);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
