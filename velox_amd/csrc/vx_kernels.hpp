@@ -502,9 +502,22 @@ __global__ __launch_bounds__(kThreads) void kWarmMatch(vx_contact* __restrict__ 
 // exactly the nearest distance, depends on that order.  Counts the contacts
 // whose match such a tie decides (the cache here is in another order among a
 // pair's contacts); the host then puts the cache in the exact order first.
+// A tie whose candidates carry bit-identical warm-start data (the six
+// impulses and vn0) decides nothing and is not counted, unless `anyTie` asks
+// for every tie (the differentiable record keeps which entry was taken).
+__device__ inline bool sameWarmData(const vx_contact& x, const vx_contact& y) {
+    return __float_as_uint(x.normalImpulse) == __float_as_uint(y.normalImpulse) &&
+           __float_as_uint(x.tangentImpulse1) == __float_as_uint(y.tangentImpulse1) &&
+           __float_as_uint(x.tangentImpulse2) == __float_as_uint(y.tangentImpulse2) &&
+           __float_as_uint(x.rollingImpulse1) == __float_as_uint(y.rollingImpulse1) &&
+           __float_as_uint(x.rollingImpulse2) == __float_as_uint(y.rollingImpulse2) &&
+           __float_as_uint(x.spinningImpulse) == __float_as_uint(y.spinningImpulse) &&
+           __float_as_uint(x.vn0) == __float_as_uint(y.vn0);
+}
+
 __global__ __launch_bounds__(kThreads) void kWarmTies(const vx_contact* __restrict__ cur, uint32_t m,
                                                       const vx_contact* __restrict__ prev, uint32_t mp,
-                                                      uint32_t* __restrict__ ties) {
+                                                      uint32_t* __restrict__ ties, bool anyTie) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     bool tie = false;
     if (k < m && mp) {
@@ -512,20 +525,24 @@ __global__ __launch_bounds__(kThreads) void kWarmTies(const vx_contact* __restri
         const uint64_t key = pairKeyOf(c);
         uint32_t lo = 0, hi = mp;
         while (lo < hi) { uint32_t mid = (lo + hi) >> 1; if (pairKeyOf(prev[mid]) < key) lo = mid + 1; else hi = mid; }
-        uint32_t same = 0;
+        uint32_t same = 0, firstSame = 0;
         if (c.featureKey != 0)
-            for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t) same += prev[t].featureKey == c.featureKey;
-        if (same > 1) {
-            tie = true;
-        } else if (same == 0) {
+            for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t)
+                if (prev[t].featureKey == c.featureKey) {
+                    if (same == 0) firstSame = t;
+                    else if (anyTie || !sameWarmData(prev[firstSame], prev[t])) tie = true;
+                    ++same;
+                }
+        if (same == 0) {
             float best = 2.5e-3f;
             uint32_t atBest = 0;
+            bool differ = false;
             for (uint32_t t = lo; t < mp && pairKeyOf(prev[t]) == key; ++t) {
                 const float d = lengthSq(v3(prev[t].point) - v3(c.point));
-                if (d < best) { best = d; atBest = 1; }
-                else if (d == best && atBest) ++atBest;
+                if (d < best) { best = d; atBest = t; differ = false; }
+                else if (d == best && best < 2.5e-3f && (anyTie || !sameWarmData(prev[atBest], prev[t]))) differ = true;
             }
-            tie = atBest > 1;
+            tie = differ;
         }
     }
     const uint64_t any = __ballot(tie);

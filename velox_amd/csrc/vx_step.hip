@@ -1375,7 +1375,8 @@ void worldStep(vxhip_ctx* c, float dt) {
     // counters (colours do not depend on impulses); the match itself runs
     // after, against a cache in the reference's exact order when a tie needs it.
     const bool tieCheck = m && c->prevCount && !c->prevExact;
-    if (tieCheck) kWarmTies<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount, k + cWarmTies);
+    if (tieCheck)
+        kWarmTies<<<blocks(m), kThreads, 0, s>>>(c->contacts.p, m, c->prev.p, c->prevCount, k + cWarmTies, cap != nullptr);
     markStage(c, sColor);
     const uint32_t K = colorContacts(c, c->contacts.p, m);
     if (cap) {
