@@ -74,6 +74,8 @@ typedef struct {
     uint32_t warmTies;         /* contacts whose warm-start match a tie among a pair's cached contacts
                                   decided (the cache was then put in the reference's std::sort order) */
     uint32_t launches[6];      /* launches of the last step by kernel, indexed by VXHIP_LAUNCH_* */
+    uint32_t graphCaptures;    /* substep graphs captured since the context was created */
+    uint32_t graphReplays;     /* substep graph launches beyond the capturing one, since then */
 } vxhip_stats;
 
 /* Kernel launch counts in vxhip_stats.launches (one step). */
@@ -83,7 +85,8 @@ enum {
     VXHIP_LAUNCH_COLOR_PAIR = 2,    /* kSweepColorPair: one big-island colour over the device */
     VXHIP_LAUNCH_COLOR_RUN = 3,     /* kSweepColorRun: a run of narrow colours in one workgroup */
     VXHIP_LAUNCH_TAIL = 4,          /* kSweepTailPair: the colour tail with LDS-resident velocities */
-    VXHIP_LAUNCH_GREEDY_DEVICE = 5  /* kGreedyColorDevice: the colouring dataflow across the device */
+    VXHIP_LAUNCH_GREEDY_DEVICE = 5, /* kGreedyColorDevice: the colouring dataflow across the device */
+    VXHIP_LAUNCH_KINDS = 6
 };
 
 /* ---------------------------------------------------------------- setup -- */
@@ -172,6 +175,31 @@ int vxhip_world_set_worlds(vxhip_ctx* ctx, const uint32_t* worldOf, uint32_t wor
  * one workgroup per island batch (default 4096; results are identical either
  * way, only the schedule changes). */
 int vxhip_world_set_island_limit(vxhip_ctx* ctx, uint32_t limit);
+
+/* Scheduling knobs of the device step (no reference counterpart).  None of
+ * them changes a result: every setting steps a world bit for bit the same,
+ * they only choose launch shapes and schedules.  velox_amd/autotune.py
+ * measures them per workload on the GPU and writes velox_amd/tuning.json,
+ * which the Python host applies when it loads a scene.  Zero fields take the
+ * defaults given here. */
+typedef struct vxhip_tuning {
+    uint32_t islandLimit;        /* islands of more contacts go colour by colour over the device (4096) */
+    uint32_t batchThreads;       /* 256 or 512: threads of an island-batch workgroup, rows per pass = half (256) */
+    uint32_t persistentBatches;  /* 1: island-batch sweeps as one workgroup per CU taking batches from a queue;
+                                    2: one workgroup per batch (1) */
+    uint32_t longestFirst;       /* 1: batches handed out by pass count, longest first; 2: batch order (1) */
+    uint32_t colorLaunchRows;    /* big-island colours with more rows get a launch each, narrower ones run in
+                                    single-workgroup runs (1024) */
+    uint32_t sideStream;         /* 1: island batches beside the big-island colour chain on a second stream;
+                                    2: one stream (1) */
+    uint32_t graphSubsteps;      /* 1: substeps 2..S captured once per step in a hipGraph and replayed (when the
+                                    step has no big island, joints or recording); 2: launched one by one (2) */
+    uint32_t reserved;
+} vxhip_tuning;
+
+int vxhip_world_set_tuning(vxhip_ctx* ctx, const vxhip_tuning* tuning);
+/* The knobs in effect (defaults filled in). */
+int vxhip_world_get_tuning(vxhip_ctx* ctx, vxhip_tuning* out);
 
 /* Advances the resident world `steps` times by dt: the device form of
  * World::stepImpl's physics (world.cpp:3975-4521 without joints, soft bodies

@@ -936,37 +936,51 @@ __global__ __launch_bounds__(kThreads) void kSweepColorPair(SolveArrays sa, uint
 // pass's data never has to wait for the loads of the pass after it.
 constexpr uint32_t kBatchRows = kBatch + kLdsIsland;  // more than any batch's contact count
 
-template <int kMode>
-__global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ slotPair,
-                                                         const uint32_t* __restrict__ runStart,
-                                                         const uint32_t* __restrict__ batchFirstRun,
-                                                         const uint2* __restrict__ batchBodies,
-                                                         const uint32_t* __restrict__ bodyList,
-                                                         const uint32_t* __restrict__ batchRange, int sweeps,
-                                                         uint32_t* __restrict__ error) {
+// Persistent form: a grid of at most one workgroup per CU (the batch's LDS
+// allows no more) takes batches from a device queue, in the order `order`
+// lists them (longest first: the batches' pass counts, so the long chains
+// start early and the short ones fill in behind them), or in batch order
+// without one.  queue[0] hands out slots, queue[1] counts finished
+// workgroups; the last one to finish clears both for the next launch.
+template <int kMode, int kBlock = kThreads>
+__global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ slotPair,
+                                                       const uint32_t* __restrict__ runStart,
+                                                       const uint32_t* __restrict__ batchFirstRun,
+                                                       const uint2* __restrict__ batchBodies,
+                                                       const uint32_t* __restrict__ bodyList,
+                                                       const uint32_t* __restrict__ batchRange,
+                                                       const uint32_t* __restrict__ order, uint32_t* queue, int sweeps,
+                                                       uint32_t* __restrict__ error) {
     __shared__ float lv[6 * kLdsBodies];
     __shared__ float4 li4[kBatchRows];  // cN, cT1, cT2, cR1
     __shared__ float2 li2[kBatchRows];  // cR2, cS
+    __shared__ uint32_t slotS;
     const uint32_t first = batchRange[0], last = batchRange[1];
     const uint32_t t = threadIdx.x;
     const bool sideB = t & 1;
-    constexpr uint32_t kPass = kThreads / 2;
-    for (uint32_t q = first + blockIdx.x; q < last; q += gridDim.x) {
+    constexpr uint32_t kPass = kBlock / 2;
+    for (;;) {
+        if (t == 0) slotS = atomicAdd(&queue[0], 1u);
+        __syncthreads();
+        const uint32_t slot = slotS;
+        if (slot >= last - first) break;
+        const uint32_t q = first + (order ? order[slot] : slot);
         const uint2 bb = batchBodies[q];
         const uint32_t nb = bb.y - bb.x;
         const uint32_t r0 = batchFirstRun[q], r1 = batchFirstRun[q + 1];
         const uint32_t jb = runStart[r0], nc = runStart[r1] - jb;  // the batch's contacts, contiguous
         if (nb > kLdsBodies || nc > kBatchRows) {  // excluded by the packing bounds; reported, never overrun
             if (t == 0) atomicOr(error, 1u);
+            __syncthreads();
             continue;
         }
-        for (uint32_t i = t; i < nb; i += kThreads) {
+        for (uint32_t i = t; i < nb; i += kBlock) {
             const uint32_t body = bodyList[bb.x + i];
             const float4 v = sa.vel[body], w = sa.ang[body];
             lv[i] = v.x; lv[kLdsBodies + i] = v.y; lv[2 * kLdsBodies + i] = v.z;
             lv[3 * kLdsBodies + i] = w.x; lv[4 * kLdsBodies + i] = w.y; lv[5 * kLdsBodies + i] = w.z;
         }
-        for (uint32_t i = t; i < nc; i += kThreads) {
+        for (uint32_t i = t; i < nc; i += kBlock) {
             const float4 a = sa.imp[2 * (jb + i)], b = sa.imp[2 * (jb + i) + 1];
             li4[i] = a;
             li2[i] = make_float2(b.x, b.y);
@@ -1012,19 +1026,42 @@ __global__ __launch_bounds__(kThreads) void kBatchSweeps(SolveArrays sa, const u
         };
         while (pass(bufA, bufB) && pass(bufB, bufA)) {
         }
-        for (uint32_t i = t; i < nb; i += kThreads) {
+        for (uint32_t i = t; i < nb; i += kBlock) {
             const uint32_t body = bodyList[bb.x + i];
             sa.vel[body] = make_float4(lv[i], lv[kLdsBodies + i], lv[2 * kLdsBodies + i], 0.0f);
             sa.ang[body] = make_float4(lv[3 * kLdsBodies + i], lv[4 * kLdsBodies + i], lv[5 * kLdsBodies + i], 0.0f);
         }
         if (kMode != 2)
-            for (uint32_t i = t; i < nc; i += kThreads) {
+            for (uint32_t i = t; i < nc; i += kBlock) {
                 const float2 e = li2[i];
                 sa.imp[2 * (jb + i)] = li4[i];
                 sa.imp[2 * (jb + i) + 1] = make_float4(e.x, e.y, 0.0f, 0.0f);
             }
         __syncthreads();
     }
+    if (t == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[1], 1u) == gridDim.x - 1) {
+            queue[0] = 0u;
+            queue[1] = 0u;
+        }
+    }
+}
+
+// Longest-first order of the LDS batches: key = passes per sweep (runs of L
+// contacts take ceil(L / kPass) passes), inverted so that an ascending stable
+// sort lists the longest first and equal ones in batch order.
+__global__ __launch_bounds__(kThreads) void kBatchCostKeys(const uint32_t* __restrict__ runStart,
+                                                           const uint32_t* __restrict__ batchFirstRun,
+                                                           const uint32_t* __restrict__ ldsBatches, uint32_t kPass,
+                                                           uint64_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+    if (q >= *ldsBatches) return;
+    uint32_t passes = 0;
+    for (uint32_t r = batchFirstRun[q]; r < batchFirstRun[q + 1]; ++r)
+        passes += (runStart[r + 1] - runStart[r] + kPass - 1) / kPass;
+    keys[q] = 0xffffu - (passes < 0xffffu ? passes : 0xffffu);
+    vals[q] = q;
 }
 
 // One sweep of the tail colours c0..c1-1 by one workgroup, two lanes per row,

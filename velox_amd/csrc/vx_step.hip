@@ -202,6 +202,19 @@ struct vxhip_ctx {
              largestIsland = 0,
              maxBatchPasses = 0;
     uint32_t islandLimit = 4096;
+    // Scheduling knobs (vxhip_tuning); none changes a result.
+    uint32_t batchThreads = 256, colorLaunchRows = 1024;
+    bool persistentBatches = true, longestFirst = true, sideOn = true, graphSubsteps = false;
+    uint32_t cuCount = 256;
+    // Island-batch queue (slot counter, finished workgroups) and the batches'
+    // longest-first order of the current step.
+    Buf<uint32_t> batchQueue, batchOrder, batchOrderTmp;
+    Buf<uint64_t> batchKeys, batchKeysTmp;
+    bool batchOrderReady = false;
+    // Substeps 2..S of a step replayed from one captured graph.
+    hipGraphExec_t subExec = nullptr;
+    uint32_t graphReplays = 0, graphCaptures = 0;
+    uint32_t graphLaunches[VXHIP_LAUNCH_KINDS] = {};  // launches of each kind one replay stands for
     // LDS tail run of the big-island colours (colours tailColor..K-1).
     uint32_t tailColor = 0;
     Buf<uint64_t> tailKeys, tailKeysTmp;
@@ -464,7 +477,10 @@ int bodyBits(uint32_t n) { return vxhost::bitsFor(n ? n - 1 : 0) ? vxhost::bitsF
 constexpr uint64_t kCellTableMax = 1ull << 22;
 // Colours with at most this many contacts are solved in runs by one
 // workgroup instead of a launch each.
-constexpr uint32_t kNarrowColor = 2048, kNarrowSweepColor = 1024;
+constexpr uint32_t kNarrowColor = 2048;
+// CUs the persistent island-batch grid leaves to the big-island colour chain
+// beside it (the chain's LDS tail needs a CU of its own).
+constexpr uint32_t kChainCus = 4;
 constexpr uint32_t kSleepMaxBodies = 256;
 // Joint sets this large go to the device; smaller ones stay with the host
 // world, as backend_cuda.cu:696-701 decides.
@@ -753,10 +769,22 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
         c->ldsBatchCount = c->hostCounters[cLdsBatches];
         c->ldsContactCount = c->hostCounters[cLdsContacts];
     }
+    // Longest-first hand-out order of the LDS batches.
+    c->batchOrderReady = false;
+    const uint32_t lb = c->ldsBatchCount;
+    if (c->longestFirst && lb > 1) {
+        c->batchKeys.reserve(lb); c->batchKeysTmp.reserve(lb); c->batchOrder.reserve(lb); c->batchOrderTmp.reserve(lb);
+        kBatchCostKeys<<<blocks(lb), kThreads, 0, s>>>(c->runStart.p, c->islandFirstRun.p, k + cLdsBatches,
+                                                       c->batchThreads / 2, c->batchKeys.p, c->batchOrder.p);
+        HIPCK(vxhost::radixSort(c->batchKeys.p, c->batchOrder.p, c->batchKeysTmp.p, c->batchOrderTmp.p, lb, 16,
+                                tempFor(c, vxhost::radixTempBytes(lb)), s));
+        HIPCK(hipGetLastError());
+        c->batchOrderReady = true;
+    }
 }
 
 // The LDS tail of the big-island colours: the last colours that each hold at
-// most kNarrowSweepColor contacts, kTailRows rows in all, their written
+// most colorLaunchRows contacts, kTailRows rows in all, their written
 // bodies given LDS slots.
 void prepareTail(vxhip_ctx* c, uint32_t K, SolveArrays sa) {
     c->tailColor = K;
@@ -765,7 +793,7 @@ void prepareTail(vxhip_ctx* c, uint32_t K, SolveArrays sa) {
     uint32_t t = K, tailRows = 0;
     while (t > 0) {
         const uint32_t cnt = cs[t] - cs[t - 1];
-        if (cnt > kNarrowSweepColor || tailRows + cnt > kTailRows || K - t >= kTailColors) break;
+        if (cnt > c->colorLaunchRows || tailRows + cnt > kTailRows || K - t >= kTailColors) break;
         tailRows += cnt;
         --t;
     }
@@ -812,7 +840,7 @@ hipStream_t sideStream(vxhip_ctx* c) {
 template <int kMode>
 void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
     hipStream_t s = c->stream;
-    const bool fork = c->batchCount && c->bigCount && !c->profile;
+    const bool fork = c->sideOn && c->batchCount && c->bigCount && !c->profile;
     const hipStream_t bs = fork ? sideStream(c) : s;
     if (fork) {
         HIPCK(hipEventRecord(c->forkEv, s));
@@ -825,9 +853,21 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
             const uint64_t visits = (uint64_t)c->ldsContactCount * sweeps;
             solveLaunchBegin(c);
             ++c->stats.launches[VXHIP_LAUNCH_BATCH];
-            kBatchSweeps<kMode><<<std::min<uint32_t>(lds, 4096), kThreads, 0, bs>>>(
-                sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p,
-                c->bodyList.p, c->counters.p + cRanges, sweeps, c->counters.p + cBatchError);
+            // Persistent: one workgroup per CU at most (a few CUs left free
+            // for the big-island chain beside it).
+            const uint32_t cap = c->persistentBatches
+                                     ? std::max<uint32_t>(1u, c->cuCount - (c->bigCount ? kChainCus : 0u))
+                                     : 65535u;
+            const uint32_t grid = std::min<uint32_t>(lds, cap);
+            const uint32_t* order = c->batchOrderReady ? c->batchOrder.p : nullptr;
+            if (c->batchThreads == 512)
+                kBatchSweeps<kMode, 512><<<grid, 512, 0, bs>>>(
+                    sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p, c->bodyList.p,
+                    c->counters.p + cRanges, order, c->batchQueue.p, sweeps, c->counters.p + cBatchError);
+            else
+                kBatchSweeps<kMode, 256><<<grid, 256, 0, bs>>>(
+                    sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p, c->bodyList.p,
+                    c->counters.p + cRanges, order, c->batchQueue.p, sweeps, c->counters.p + cBatchError);
             solveLaunchEnd(c, kMode == 2 ? VXHIP_KERNEL_BATCH_WARM : VXHIP_KERNEL_BATCH, visits);
         }
         if (global) {
@@ -853,7 +893,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                     solveLaunchEnd(c, VXHIP_KERNEL_COLOR, cs[K] - cs[col]);
                     break;
                 }
-                if (cnt > kNarrowSweepColor) {
+                if (cnt > c->colorLaunchRows) {
                     solveLaunchBegin(c);
                     ++c->stats.launches[VXHIP_LAUNCH_COLOR_PAIR];
                     kSweepColorPair<kMode><<<blocks(2 * cnt), kThreads, 0, s>>>(sa, c->smallCount, cs[col], cnt);
@@ -862,7 +902,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
                     continue;
                 }
                 uint32_t end = col + 1;
-                while (end < K && (kMode == 2 || end != c->tailColor) && cs[end + 1] - cs[end] <= kNarrowSweepColor)
+                while (end < K && (kMode == 2 || end != c->tailColor) && cs[end + 1] - cs[end] <= c->colorLaunchRows)
                     ++end;
                 solveLaunchBegin(c);
                 ++c->stats.launches[VXHIP_LAUNCH_COLOR_RUN];
@@ -1340,6 +1380,52 @@ void finishSleep(vxhip_ctx* c) {
 }
 
 // One full device step of the resident world.
+// Captures one later substep (integration, the velocity solve without the
+// impulse store, transforms) on the context's stream and turns it into the
+// context's executable graph: updated in place when the new capture has the
+// old one's shape, instantiated anew otherwise.  The capture only records:
+// every buffer it names was sized by the step's first substep, and no fill in
+// it is a memset node.
+void captureSubstep(vxhip_ctx* c, uint32_t K, V3 g, float h) {
+    hipStream_t s = c->stream;
+    const uint32_t n = c->n;
+    uint32_t before[VXHIP_LAUNCH_KINDS];
+    for (int l = 0; l < VXHIP_LAUNCH_KINDS; ++l) before[l] = c->stats.launches[l];
+    HIPCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
+        substepSolve(c, c->solve.p, K, h, c->cfg.options, false, false);
+        kTransforms<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, h);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCK(hipStreamEndCapture(s, &graph));
+    for (int l = 0; l < VXHIP_LAUNCH_KINDS; ++l) {
+        c->graphLaunches[l] = c->stats.launches[l] - before[l];
+        c->stats.launches[l] = before[l];
+    }
+    if (c->subExec) {
+        hipGraphNode_t bad = nullptr;
+        hipGraphExecUpdateResult why;
+        if (hipGraphExecUpdate(c->subExec, graph, &bad, &why) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(c->subExec);
+            c->subExec = nullptr;
+        }
+    }
+    if (!c->subExec) {
+        const hipError_t e = hipGraphInstantiate(&c->subExec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCK(e);
+    } else {
+        HIPCK(hipGraphDestroy(graph));
+    }
+    ++c->graphCaptures;
+}
+
 void worldStep(vxhip_ctx* c, float dt) {
     injectFailureForTesting();
     const vxhip_config& cfg = c->cfg;
@@ -1400,7 +1486,21 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->solveCount = m;
     markStage(c, sSolve);
     if (c->jointCount) kJointWake<<<1, 1, 0, s>>>(c->bodies.p, c->joints.p, c->jointCount);  // backend_cuda.cu:704-708
+    // Substeps 2..S launch the same kernels with the same arguments once the
+    // first has set up the step's islands and rows: with graphSubsteps they are
+    // captured once (as substep 2) and replayed from one executable graph,
+    // updated in place from step to step while its shape holds.
+    const bool graph = c->graphSubsteps && m && S > 1 && !c->profile && !cap && !c->bigCount && !c->jointCount;
     for (int st = 0; st < S; ++st) {
+        if (graph && st > 0) {
+            if (st == 1) captureSubstep(c, K, g, h);
+            else ++c->graphReplays;
+            HIPCK(hipGraphLaunch(c->subExec, s));
+            for (int l = 0; l < VXHIP_LAUNCH_KINDS; ++l) c->stats.launches[l] += c->graphLaunches[l];
+            if (st == S - 1)
+                kStoreImpulses<<<blocks(m), kThreads, 0, s>>>(c->solve.p, m, c->sImp.p, c->solvePos.p);
+            continue;
+        }
         if (st > 0) kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
         if (m) substepSolve(c, c->solve.p, K, h, cfg.options, st == 0, st == S - 1);
         substepJoints(c, h, st == 0);
@@ -1480,6 +1580,8 @@ void worldStep(vxhip_ctx* c, float dt) {
     c->stats.largestIsland = m ? c->largestIsland : 0;
     c->stats.maxBatchPasses = m ? c->maxBatchPasses : 0;
     c->stats.multiToiEvents = c->toiEventCount;
+    c->stats.graphCaptures = c->graphCaptures;
+    c->stats.graphReplays = c->graphReplays;
     if (cap) {
         cap->m = m;
         cap->ccdRescued = c->hostCounters[cCcdRescued];
@@ -1725,6 +1827,12 @@ vxhip_ctx* newContext() {
     vxhip_ctx* c = new vxhip_ctx();
     HIPCK(hipGetDevice(&c->device));
     HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        c->cuCount = (uint32_t)prop.multiProcessorCount;
+    c->batchQueue.reserve(2);
+    kFill32<<<1, 64, 0, c->stream>>>(c->batchQueue.p, 2, 0u);
+    HIPCK(hipGetLastError());
     c->counters.reserve(cCount);
     HIPCK(hipHostMalloc((void**)&c->hostCounters, cCount * sizeof(uint32_t), hipHostMallocDefault));
     return c;
@@ -1806,6 +1914,7 @@ void vxhip_destroy(vxhip_ctx* c) {
     // Device buffers are released by their destructors below.
     for (hipEvent_t e : c->stageEv) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->launchEv) (void)hipEventDestroy(e);
+    if (c->subExec) (void)hipGraphExecDestroy(c->subExec);
     if (c->forkEv) (void)hipEventDestroy(c->forkEv);
     if (c->joinEv) (void)hipEventDestroy(c->joinEv);
     if (c->hostCounters) (void)hipHostFree(c->hostCounters);
@@ -1868,6 +1977,37 @@ int vxhip_world_set_island_limit(vxhip_ctx* c, uint32_t limit) {
     return guarded([&] {
         if (!c || limit == 0) fail(VXHIP_ERR_INVALID, "vxhip: invalid island limit");
         c->islandLimit = limit;
+    });
+}
+
+int vxhip_world_set_tuning(vxhip_ctx* c, const vxhip_tuning* t) {
+    return guarded([&] {
+        if (!c || !t) fail(VXHIP_ERR_INVALID, "vxhip: invalid tuning arguments");
+        if (t->batchThreads && t->batchThreads != 256 && t->batchThreads != 512)
+            fail(VXHIP_ERR_INVALID, "vxhip: batchThreads must be 256 or 512");
+        if (t->persistentBatches > 2 || t->longestFirst > 2 || t->sideStream > 2 || t->graphSubsteps > 2)
+            fail(VXHIP_ERR_INVALID, "vxhip: switch fields take 0 (default), 1 (on) or 2 (off)");
+        c->islandLimit = t->islandLimit ? t->islandLimit : 4096u;
+        c->batchThreads = t->batchThreads ? t->batchThreads : 256u;
+        c->persistentBatches = t->persistentBatches != 2;
+        c->longestFirst = t->longestFirst != 2;
+        c->colorLaunchRows = t->colorLaunchRows ? t->colorLaunchRows : 1024u;
+        c->sideOn = t->sideStream != 2;
+        c->graphSubsteps = t->graphSubsteps == 1;
+    });
+}
+
+int vxhip_world_get_tuning(vxhip_ctx* c, vxhip_tuning* out) {
+    return guarded([&] {
+        if (!c || !out) fail(VXHIP_ERR_INVALID, "vxhip: invalid tuning arguments");
+        *out = vxhip_tuning{};
+        out->islandLimit = c->islandLimit;
+        out->batchThreads = c->batchThreads;
+        out->persistentBatches = c->persistentBatches ? 1u : 2u;
+        out->longestFirst = c->longestFirst ? 1u : 2u;
+        out->colorLaunchRows = c->colorLaunchRows;
+        out->sideStream = c->sideOn ? 1u : 2u;
+        out->graphSubsteps = c->graphSubsteps ? 1u : 2u;
     });
 }
 

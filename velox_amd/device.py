@@ -34,7 +34,7 @@ class Stats(C.Structure):
                 ("islandContacts", C.c_uint32), ("bigIslandContacts", C.c_uint32),
                 ("largestIsland", C.c_uint32), ("maxBatchPasses", C.c_uint32), ("multiToiEvents", C.c_uint32),
                 ("ccdFlagged", C.c_uint32), ("ccdRelevant", C.c_uint32), ("warmTies", C.c_uint32),
-                ("launches", C.c_uint32 * 6)]
+                ("launches", C.c_uint32 * 6), ("graphCaptures", C.c_uint32), ("graphReplays", C.c_uint32)]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_}
@@ -76,6 +76,30 @@ class MultiToiSettings(C.Structure):
         super().__init__(**defaults)
 
 
+class Tuning(C.Structure):
+    """vxhip_tuning (include/vxhip.h): scheduling knobs, none of which changes
+    a result.  Switch fields: 0 default, 1 on, 2 off."""
+    _fields_ = [("islandLimit", C.c_uint32), ("batchThreads", C.c_uint32), ("persistentBatches", C.c_uint32),
+                ("longestFirst", C.c_uint32), ("colorLaunchRows", C.c_uint32), ("sideStream", C.c_uint32),
+                ("graphSubsteps", C.c_uint32), ("reserved", C.c_uint32)]
+    SWITCHES = ("persistentBatches", "longestFirst", "sideStream", "graphSubsteps")
+
+    def as_dict(self):
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        for k in self.SWITCHES:
+            d[k] = d[k] == 1
+        return d
+
+    @classmethod
+    def from_dict(cls, d):
+        t = cls()
+        for name, _ in cls._fields_:
+            if name in d and name != "reserved":
+                v = d[name]
+                setattr(t, name, (1 if v else 2) if name in cls.SWITCHES else int(v))
+        return t
+
+
 class DiffStats(C.Structure):
     """vxhip_diff_stats (include/vxhip.h)."""
     _fields_ = [(name, C.c_uint32) for name in ("steps", "checkpoints", "worlds", "ccdRescued", "replayMismatch",
@@ -114,6 +138,8 @@ SIGNATURES = {
     "vxhip_world_step": (_i32, [_vp, _f32, _u32]),
     "vxhip_world_set_worlds": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_set_island_limit": (_i32, [_vp, _u32]),
+    "vxhip_world_set_tuning": (_i32, [_vp, _vp]),
+    "vxhip_world_get_tuning": (_i32, [_vp, _vp]),
     "vxhip_world_download_events": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_set_events": (_i32, [_vp, _i32]),
     "vxhip_world_set_sleep": (_i32, [_vp, _vp]),
@@ -214,6 +240,7 @@ class Scene:
         from .records import SoupArrays
         lib = library()
         h = C.c_void_p()
+        self.name, self.param = name, int(param)
         check(lib.vxhip_scene_build(name.encode(), int(param), C.byref(h)))
         try:
             n = lib.vxhip_scene_body_count(h)
@@ -255,6 +282,7 @@ class Scene:
         """World w of a batched scene as a scene of its own (same geometry
         arrays, the world's bodies in their batched order)."""
         sub = object.__new__(Scene)
+        sub.name, sub.param = self.name, self.param
         sub.bodies = np.ascontiguousarray(self.bodies[self.world_of == w])
         sub.worlds = 1
         sub.world_of = np.zeros(len(sub.bodies), np.uint32)
@@ -294,11 +322,28 @@ class Context:
             pass
 
     # -- resident world --------------------------------------------------
-    def load_scene(self, scene, options=None):
+    def load_scene(self, scene, options=None, tuned=True):
+        """Loads a synthetic scene; with `tuned`, the knobs the autotuner's
+        table (velox_amd/tuning.json) holds for the scene's workload are set
+        first (results are the same with any knobs)."""
+        if tuned:
+            from .tuning import knobs_for
+            self.set_tuning(**knobs_for(scene))
         self.load(scene.bodies, scene.soup, gravity=scene.gravity, substeps=scene.substeps, options=options)
         if getattr(scene, "worlds", 1) > 1:
             self.set_worlds(scene.world_of, scene.worlds)
         return self
+
+    def set_tuning(self, **knobs):
+        """Sets vxhip_tuning fields by name (unnamed fields take their defaults)."""
+        t = Tuning.from_dict(knobs)
+        check(library().vxhip_world_set_tuning(self._h, C.byref(t)))
+        return self
+
+    def tuning(self):
+        t = Tuning()
+        check(library().vxhip_world_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
 
     def set_island_limit(self, limit):
         """Islands above `limit` contacts are solved colour by colour over the device."""
