@@ -219,8 +219,9 @@ def batched_leg(worlds, steps, sample, plain_steps):
             "batched_ms_per_step": round(batched_s * 1e3 / steps, 3), "contacts_last_step": st["contacts"],
             "plain": f"{len(chosen)} of {worlds} worlds (every {stride}th), each loaded alone and stepped through its "
                      f"first {plain_steps} steps with a sync per world-step (timed), after an untimed warm-up step",
-            "workload": "log-uniform 8..512 bodies per world (box towers, sphere piles, capsule rows) on one plane "
-                        "each, dt=1/60, 4 substeps, 8 iterations"}
+            "joints": int(len(scene.joints)),
+            "workload": "log-uniform 8..512 bodies per world (box towers, sphere piles, ball-jointed capsule chains) "
+                        "on one plane each, dt=1/60, 4 substeps, 8 iterations"}
 
 
 def diff_leg(worlds, steps, every):

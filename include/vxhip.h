@@ -247,6 +247,13 @@ int vxhip_world_stats(vxhip_ctx* ctx, vxhip_stats* out);
  * World::finishBrokenJoints does (world.cpp:3765-3783): from the end, each
  * swapped with the last.  vxhip_world_load clears the joints. */
 int vxhip_world_set_joints(vxhip_ctx* ctx, const vx_joint* joints, uint32_t count);
+/* The same for any number of joints (no reference counterpart): every joint
+ * set, fewer than 64 included, is solved by the colour-ordered device path
+ * above, where the reference's World would hand a set of fewer than 64 to its
+ * host solver.  Batched worlds take their joints this way: the batch's joint
+ * set is large while each world's own is small, and a world stepped alone
+ * must step exactly as it does inside the batch. */
+int vxhip_world_set_joints_all(vxhip_ctx* ctx, const vx_joint* joints, uint32_t count);
 
 /* The joints after the last step (after removals). */
 int vxhip_world_download_joints(vxhip_ctx* ctx, vx_joint* out, uint32_t cap, uint32_t* count);
@@ -487,6 +494,9 @@ uint32_t vxhip_scene_body_count(const vxhip_scene* scene);
 const vx_body* vxhip_scene_bodies(const vxhip_scene* scene);
 const uint32_t* vxhip_scene_worlds(const vxhip_scene* scene, uint32_t* worldCount);
 int vxhip_scene_soup(const vxhip_scene* scene, vx_soup* out);
+/* The scene's joints (ball joints of the batched scene's capsule chains), or
+ * null with *count = 0. */
+const vx_joint* vxhip_scene_joints(const vxhip_scene* scene, uint32_t* count);
 int vxhip_scene_config(const vxhip_scene* scene, vxhip_config* config, float* dt);
 void vxhip_scene_free(vxhip_scene* scene);
 

@@ -28,13 +28,21 @@ def test_batched_scene_layout():
     counts = np.bincount(sc.world_of, minlength=sc.worlds)
     assert counts.min() >= 9 and counts.max() <= 513  # plane + 8..512 bodies
     assert (sc.bodies["shape"][sc.world_of == 3] == 1).sum() == 1  # one plane per world
+    # Capsule chains: ball joints between consecutive capsules of one world.
+    j = sc.joints
+    assert len(j) > 0 and (j["type"] == 0).all()
+    assert (sc.world_of[j["a"]] == sc.world_of[j["b"]]).all() and (j["b"] == j["a"] + 1).all()
+    assert (sc.bodies["shape"][j["a"]] == 3).all() and (sc.bodies["shape"][j["b"]] == 3).all()
+    sub = sc.world(int(sc.world_of[j["a"][0]]))
+    assert len(sub.joints) > 0 and (sub.bodies["shape"][sub.joints["a"]] == 3).all()
 
 
 @pytest.mark.parametrize("steps", [1, 25])
 def test_batched_equals_each_world_alone(ctx, steps):
     from velox_amd.device import Context
     sc = _worlds()
-    print(f"batched 12 worlds, {len(sc.bodies)} bodies, {steps} steps", flush=True)
+    assert len(sc.joints), "the scene has capsule chains"
+    print(f"batched 12 worlds, {len(sc.bodies)} bodies, {len(sc.joints)} joints, {steps} steps", flush=True)
     ctx.load_scene(sc)
     ctx.step(sc.dt, steps)
     batched = ctx.bodies()
@@ -74,12 +82,18 @@ def test_batched_large_equals_each_world_alone(ctx):
 
 
 def test_batched_world_matches_oracle(ctx):
+    """Worlds without capsule chains (the oracle has no joints; the chains'
+    ball joints follow the reference World's device joint path, pinned by the
+    joints fixtures)."""
     sc = _worlds()
     steps = 10
     ctx.load_scene(sc)
     ctx.step(sc.dt, steps)
     batched = ctx.bodies()
-    for w in (0, 5):
+    jointed = set(sc.world_of[sc.joints["a"]].tolist())
+    free = [w for w in range(sc.worlds) if w not in jointed][:2]
+    assert len(free) == 2
+    for w in free:
         sub = sc.world(w)
         ow = OracleWorld(sub.bodies, sub.soup, dt=sc.dt, substeps=sc.substeps, gravity=sc.gravity)
         ow.step(steps)

@@ -28,6 +28,7 @@ struct vxhip_scene {
     int substeps = 4;
     std::vector<uint32_t> worldOf;   // batched scenes: owning world of each body
     uint32_t worlds = 1;
+    std::vector<vx_joint> joints;
 };
 
 namespace {
@@ -358,9 +359,39 @@ void addBody(vxhip_scene& sc, const vxscene::Scene& s, const vxscene::BodySpec& 
 
 vxscene::Scene vxSceneByName(const std::string& name, uint32_t param, bool& ok);
 
+// A ball joint with the reference's Joint defaults (joint.h:92-190): axes
+// (0,1,0), references (1,0,0), unbreakable (FLT_MAX), no spring, motor or
+// limits, all limit axes masked in, linked bodies not colliding.
+vx_joint ballJoint(const vxscene::JointSpec& js) {
+    vx_joint j;
+    std::memset(&j, 0, sizeof(j));
+    j.type = VX_JOINT_BALL;
+    j.a = js.a;
+    j.b = js.b;
+    j.localAnchorA = {js.anchorA[0], js.anchorA[1], js.anchorA[2]};
+    j.localAnchorB = {js.anchorB[0], js.anchorB[1], js.anchorB[2]};
+    j.localAxisA = j.localAxisB = {0, 1, 0};
+    j.localRefA = j.localRefB = {1, 0, 0};
+    j.springDampingRatio = 1.0f;
+    j.breakForce = j.breakTorque = 3.402823466e+38F;
+    j.swingLimit = 3.14159265f;
+    j.lowerTwistLimit = -3.14159265f;
+    j.upperTwistLimit = 3.14159265f;
+    j.linearLimitMask = j.angularLimitMask = 7;
+    j.weldDampingRatio = 1.0f;
+    j.suspensionFrequencyHz = 2.0f;
+    j.suspensionDampingRatio = 0.7f;
+    j.wheelRadius = 0.5f;
+    j.maxSteeringAngle = 0.6f;
+    j.pulleyRatio = 1.0f;
+    j.gearRatio = 1.0f;
+    return j;
+}
+
 vxhip_scene* vxBuildScene(const vxscene::Scene& s) {
     vxhip_scene* sc = new vxhip_scene();
     for (const vxscene::BodySpec& b : s.bodies) addBody(*sc, s, b);
+    for (const vxscene::JointSpec& j : s.joints) sc->joints.push_back(ballJoint(j));
     std::memcpy(sc->gravity, s.gravity, sizeof(sc->gravity));
     sc->dt = s.dt;
     sc->substeps = s.substeps;
@@ -412,6 +443,11 @@ const vx_body* vxhip_scene_bodies(const vxhip_scene* s) { return s ? s->bodies.d
 const uint32_t* vxhip_scene_worlds(const vxhip_scene* s, uint32_t* worldCount) {
     if (worldCount) *worldCount = s ? s->worlds : 0;
     return s ? s->worldOf.data() : nullptr;
+}
+
+const vx_joint* vxhip_scene_joints(const vxhip_scene* s, uint32_t* count) {
+    if (count) *count = s ? (uint32_t)s->joints.size() : 0;
+    return s && !s->joints.empty() ? s->joints.data() : nullptr;
 }
 
 int vxhip_scene_soup(const vxhip_scene* s, vx_soup* out) {

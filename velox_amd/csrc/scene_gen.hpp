@@ -61,6 +61,13 @@ struct BodySpec {
     uint8_t quality = 1;          // BodyCcdTuning::quality (ccd.h:13): 1 Medium, 2 High
 };
 
+// A ball joint between bodies a and b (indices into Scene::bodies) at local
+// anchors; every other Joint field keeps the reference's default.
+struct JointSpec {
+    uint32_t a = 0, b = 0;
+    float anchorA[3] = {0, 0, 0}, anchorB[3] = {0, 0, 0};
+};
+
 struct HeightfieldSpec {
     uint32_t width = 2, depth = 2;
     float cell = 1.0f;
@@ -70,6 +77,7 @@ struct HeightfieldSpec {
 
 struct Scene {
     std::vector<BodySpec> bodies;
+    std::vector<JointSpec> joints;
     std::vector<float> hullPoints;      // xyz triples
     std::vector<HeightfieldSpec> heightfields;
     float gravity[3] = {0, -9.81f, 0};
@@ -395,8 +403,10 @@ inline Scene terrain(uint32_t side = 24, float cell = 0.5f, float amp = 1.0f, ui
 // n_w = round(exp(U(ln 8, ln 512))) dynamic bodies built from seeded
 // structures on a 4 m site grid: box towers of height U{3..12} (unit boxes,
 // small yaw jitter), sphere piles (r ~ U(0.2, 0.4), a loose cluster dropped
-// from 1-3 m) and capsule rows (r ~ U(0.15, 0.25), half-height U(0.3, 0.6),
-// lying side by side).  Density 1000, friction 0.6, restitution 0.
+// from 1-3 m) and capsule chains (3-5 capsules, r ~ U(0.1, 0.15), half-height
+// U(0.15, 0.3), lying end to end along x 2 cm apart, each joined to the next
+// by a ball joint midway between them).  Density 1000, friction 0.6,
+// restitution 0.
 inline Scene batchWorlds(uint32_t worlds = 4096, uint32_t seed = kSeedBatch) {
     Scene s;
     s.worlds = worlds;
@@ -447,19 +457,30 @@ inline Scene batchWorlds(uint32_t worlds = 4096, uint32_t seed = kSeedBatch) {
                     s.bodies.push_back(b);
                 }
                 placed += m;
-            } else {  // capsule row, axes along x, side by side along z
-                const uint32_t m = std::min<uint32_t>(left, 4u + (uint32_t)r.pick(4));
-                const float rad = r.range(0.15f, 0.25f), hh = r.range(0.3f, 0.6f);
+            } else {  // capsule chain, axes along x, end to end
+                const uint32_t m = std::min<uint32_t>(left, 3u + (uint32_t)r.pick(3));
+                const float rad = r.range(0.1f, 0.15f), hh = r.range(0.15f, 0.3f);
+                const float link = 2.0f * (hh + rad) + 0.02f;  // centre spacing
+                const uint32_t first = (uint32_t)s.bodies.size();
                 for (uint32_t k = 0; k < m; ++k) {
                     BodySpec b = spec(Capsule);
                     b.radius = rad;
                     b.halfHeight = hh;
                     b.mass = 1000.0f * (pi * rad * rad * 2.0f * hh + (4.0f / 3.0f) * pi * rad * rad * rad);
-                    b.pos[0] = sx;
+                    b.pos[0] = sx + link * ((float)k - 0.5f * (float)(m - 1));
                     b.pos[1] = rad + 0.01f;
-                    b.pos[2] = sz - 1.8f + (2.0f * rad + 0.002f) * (float)k;
+                    b.pos[2] = sz;
+                    // +90 degrees about z: the capsule's local y axis along world -x.
                     b.quat[0] = 0.0f; b.quat[1] = 0.0f; b.quat[2] = std::sin(0.25f * pi); b.quat[3] = std::cos(0.25f * pi);
                     s.bodies.push_back(b);
+                }
+                for (uint32_t k = 0; k + 1 < m; ++k) {
+                    JointSpec j;
+                    j.a = first + k;
+                    j.b = first + k + 1;
+                    j.anchorA[1] = -0.5f * link;  // world +x of link k
+                    j.anchorB[1] = 0.5f * link;   // world -x of link k + 1
+                    s.joints.push_back(j);
                 }
                 placed += m;
             }

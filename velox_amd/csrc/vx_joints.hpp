@@ -5,13 +5,14 @@
 //   kJointPairsOff    contacts of joint-connected pairs (collideConnected off)
 //                     are dropped, order kept, and their pairs leave the CCD net
 //   kJointWake        a joint between a sleeping and a waking body wakes both,
-//                     in joint order (one lane)
+//                     in joint order (one lane; only when a body sleeps)
 //   kJointReset       accumulated impulses cleared each substep (broken joints
 //                     kept after the first substep)
 //   kJointColor       the joints of one colour, one lane each (colours never
 //                     share a dynamic body), 8 passes over the colours
 //   kJointMarkBroken  reaction force / torque over the substep against the
 //                     joint's thresholds
+//   kJointBroken      World::finishBrokenJoints' break test, all joints at once
 //   kJointFinish      World::finishBrokenJoints: broken joints removed from the
 //                     end, each swapped with the last, with a break event
 #pragma once
@@ -97,14 +98,32 @@ __global__ __launch_bounds__(kThreads) void kJointMarkBroken(vx_joint* joints, u
     j.broken = force > j.breakForce || torque > j.breakTorque;
 }
 
-__global__ void kJointFinish(vx_joint* joints, uint32_t nj, float h, JointBreak* breaks, uint32_t* remaining,
-                             uint32_t* breakCount) {
-    if (threadIdx.x || blockIdx.x) return;
-    for (uint32_t k = 0; k < nj; ++k) {
+// The step-end break test of every joint (World::finishBrokenJoints' first
+// pass), in parallel; counts the broken joints.
+__global__ __launch_bounds__(kThreads) void kJointBroken(vx_joint* joints, uint32_t nj, float h,
+                                                         uint32_t* __restrict__ brokenCount) {
+    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    bool broken = false;
+    if (k < nj) {
         vx_joint& j = joints[k];
         const float force = length(v3(j.reactionLinearImpulse)) / h;
         const float torque = length(v3(j.reactionAngularImpulse)) / h;
-        j.broken = j.broken || force > j.breakForce || torque > j.breakTorque;
+        broken = j.broken || force > j.breakForce || torque > j.breakTorque;
+        j.broken = broken;
+    }
+    const uint64_t any = __ballot(broken);
+    if (any && (threadIdx.x & 63) == 0) atomicAdd(brokenCount, (uint32_t)__popcll(any));
+}
+
+// The removal pass, in the reference's order (from the end, each broken
+// joint swapped with the last); nothing to walk when none broke.
+__global__ void kJointFinish(vx_joint* joints, uint32_t nj, float h, JointBreak* breaks, uint32_t* remaining,
+                             uint32_t* breakCount, const uint32_t* __restrict__ brokenCount) {
+    if (threadIdx.x || blockIdx.x) return;
+    if (*brokenCount == 0) {
+        *remaining = nj;
+        *breakCount = 0;
+        return;
     }
     uint32_t count = nj, nb = 0;
     for (uint32_t i = nj; i-- > 0;) {

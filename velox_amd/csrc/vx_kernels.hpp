@@ -696,6 +696,79 @@ __global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __r
     }
 }
 
+// Batched worlds whose bodies lie world after world: the contacts of one
+// world are contiguous in contact order and none has a predecessor outside
+// it, so each world is coloured by one workgroup (the dataflow above over its
+// range [segStart[w], segStart[w + 1])), every world independently.  A lane
+// reads its contact and the two constrained-side flags before it waits, so a
+// hand-off costs only the predecessors' masks.
+__global__ __launch_bounds__(kThreads) void kGreedyColorWorlds(const vx_body* __restrict__ bodies,
+                                                               const vx_contact* __restrict__ cs,
+                                                               const uint32_t* __restrict__ segStart,
+                                                               const uint32_t* __restrict__ pred,
+                                                               unsigned long long* __restrict__ maskAfter,
+                                                               int* __restrict__ nextAfter, uint32_t* color,
+                                                               uint32_t* __restrict__ colorCount,
+                                                               uint32_t* __restrict__ numColors,
+                                                               uint32_t* __restrict__ error) {
+    const uint32_t begin = segStart[blockIdx.x], end = segStart[blockIdx.x + 1];
+    uint32_t k = begin + threadIdx.x;
+    uint32_t spins = 0;
+    bool loaded = false, sa = false, sb = false;
+    uint32_t pa = kNone, pb = kNone;
+    while (k < end) {
+        if (!loaded) {
+            pa = pred[2 * k];
+            pb = pred[2 * k + 1];
+            const vx_contact& c = cs[k];
+            sa = !constrainedSide(bodies, c, 0);
+            sb = !constrainedSide(bodies, c, 1);
+            loaded = true;
+        }
+        const bool readyA = pa == kNone || __hip_atomic_load(&color[pa >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        const bool readyB = pb == kNone || __hip_atomic_load(&color[pb >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        if (readyA && readyB) {
+            unsigned long long maskA = 0, maskB = 0;
+            int nextA = 64, nextB = 64;
+            if (!sa && pa != kNone) { maskA = maskAfter[pa]; nextA = nextAfter[pa]; }
+            if (!sb && pb != kNone) { maskB = maskAfter[pb]; nextB = nextAfter[pb]; }
+            const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
+            int col;
+            if (~used) {
+                col = __ffsll((long long)~used) - 1;
+            } else {
+                int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
+                col = na > nb2 ? na : nb2;
+                if (!sa) nextA = col + 1;
+                if (!sb) nextB = col + 1;
+            }
+            if (col < 64) {
+                if (!sa) maskA |= 1ull << col;
+                if (!sb) maskB |= 1ull << col;
+            }
+            maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
+            maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
+            atomicAdd(&colorCount[col], 1u);
+            atomicMax(numColors, (uint32_t)col + 1u);
+            __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            k += kThreads;
+            loaded = false;
+            spins = 0;
+        } else if (++spins > (1u << 26)) {
+            atomicOr(error, 1u);
+            break;
+        }
+    }
+}
+
+// Contacts per world (by the first body), for kGreedyColorWorlds' ranges.
+__global__ __launch_bounds__(kThreads) void kContactWorlds(const vx_contact* __restrict__ cs, uint32_t m,
+                                                           const uint32_t* __restrict__ worldOf,
+                                                           uint32_t* __restrict__ count) {
+    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m) atomicAdd(&count[worldOf[cs[k].a]], 1u);
+}
+
 // The same greedy colouring by dataflow across the whole device, for large
 // contact sets: lanes walk contacts with a grid-wide stride, wait for their
 // predecessors with relaxed device-scope loads, and take one device-scope

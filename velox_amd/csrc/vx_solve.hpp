@@ -576,13 +576,17 @@ __global__ __launch_bounds__(kThreads) void kRunTables(const uint64_t* __restric
 __global__ __launch_bounds__(kThreads) void kBatchPasses(const uint32_t* __restrict__ runStart,
                                                          const uint32_t* __restrict__ batchFirstRun,
                                                          const uint32_t* __restrict__ batches,
-                                                         uint32_t* __restrict__ maxPasses) {
+                                                         uint32_t* __restrict__ maxPasses,
+                                                         const uint2* __restrict__ batchBodies,
+                                                         uint32_t* __restrict__ maxBodies, uint32_t* __restrict__ maxRows) {
     uint32_t q = blockIdx.x * kThreads + threadIdx.x;
     if (q >= *batches) return;
     uint32_t passes = 0;
     for (uint32_t r = batchFirstRun[q]; r < batchFirstRun[q + 1]; ++r)
         passes += (runStart[r + 1] - runStart[r] + kThreads / 2 - 1) / (kThreads / 2);
     atomicMax(maxPasses, passes);
+    atomicMax(maxBodies, batchBodies[q].y - batchBodies[q].x);
+    atomicMax(maxRows, runStart[batchFirstRun[q + 1]] - runStart[batchFirstRun[q]]);
 }
 
 __global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t ms, uint32_t* batchFirstRun,
@@ -937,12 +941,16 @@ __global__ __launch_bounds__(kThreads) void kSweepColorPair(SolveArrays sa, uint
 constexpr uint32_t kBatchRows = kBatch + kLdsIsland;  // more than any batch's contact count
 
 // Persistent form: a grid of at most one workgroup per CU (the batch's LDS
-// allows no more) takes batches from a device queue, in the order `order`
+// allows no more; three with kSmall, whose LDS holds batches of up to
+// kSmallBodies bodies and kSmallRows contacts) takes batches from a device
+// queue, in the order `order`
 // lists them (longest first: the batches' pass counts, so the long chains
 // start early and the short ones fill in behind them), or in batch order
 // without one.  queue[0] hands out slots, queue[1] counts finished
 // workgroups; the last one to finish clears both for the next launch.
-template <int kMode, int kBlock = kThreads>
+constexpr uint32_t kSmallBodies = 1024, kSmallRows = 1024;
+
+template <int kMode, int kBlock = kThreads, bool kSmall = false>
 __global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uint32_t* __restrict__ slotPair,
                                                        const uint32_t* __restrict__ runStart,
                                                        const uint32_t* __restrict__ batchFirstRun,
@@ -951,9 +959,10 @@ __global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uin
                                                        const uint32_t* __restrict__ batchRange,
                                                        const uint32_t* __restrict__ order, uint32_t* queue, int sweeps,
                                                        uint32_t* __restrict__ error) {
-    __shared__ float lv[6 * kLdsBodies];
-    __shared__ float4 li4[kBatchRows];  // cN, cT1, cT2, cR1
-    __shared__ float2 li2[kBatchRows];  // cR2, cS
+    constexpr uint32_t kBodies = kSmall ? kSmallBodies : kLdsBodies, kRows = kSmall ? kSmallRows : kBatchRows;
+    __shared__ float lv[6 * kBodies];
+    __shared__ float4 li4[kRows];  // cN, cT1, cT2, cR1
+    __shared__ float2 li2[kRows];  // cR2, cS
     __shared__ uint32_t slotS;
     const uint32_t first = batchRange[0], last = batchRange[1];
     const uint32_t t = threadIdx.x;
@@ -969,7 +978,7 @@ __global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uin
         const uint32_t nb = bb.y - bb.x;
         const uint32_t r0 = batchFirstRun[q], r1 = batchFirstRun[q + 1];
         const uint32_t jb = runStart[r0], nc = runStart[r1] - jb;  // the batch's contacts, contiguous
-        if (nb > kLdsBodies || nc > kBatchRows) {  // excluded by the packing bounds; reported, never overrun
+        if (nb > kBodies || nc > kRows) {  // excluded by the packing bounds; reported, never overrun
             if (t == 0) atomicOr(error, 1u);
             __syncthreads();
             continue;
@@ -977,8 +986,8 @@ __global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uin
         for (uint32_t i = t; i < nb; i += kBlock) {
             const uint32_t body = bodyList[bb.x + i];
             const float4 v = sa.vel[body], w = sa.ang[body];
-            lv[i] = v.x; lv[kLdsBodies + i] = v.y; lv[2 * kLdsBodies + i] = v.z;
-            lv[3 * kLdsBodies + i] = w.x; lv[4 * kLdsBodies + i] = w.y; lv[5 * kLdsBodies + i] = w.z;
+            lv[i] = v.x; lv[kBodies + i] = v.y; lv[2 * kBodies + i] = v.z;
+            lv[3 * kBodies + i] = w.x; lv[4 * kBodies + i] = w.y; lv[5 * kBodies + i] = w.z;
         }
         for (uint32_t i = t; i < nc; i += kBlock) {
             const float4 a = sa.imp[2 * (jb + i)], b = sa.imp[2 * (jb + i) + 1];
@@ -1015,7 +1024,7 @@ __global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uin
                 const float2 e = li2[k];
                 cur.i0 = li4[k];
                 cur.i1 = make_float4(e.x, e.y, 0.0f, 0.0f);
-                if (pairRow<kMode, kLdsBodies, false, false>(sa, j, cur, sideB, lv)) {
+                if (pairRow<kMode, kBodies, false, false>(sa, j, cur, sideB, lv)) {
                     li4[k] = cur.i0;
                     li2[k] = make_float2(cur.i1.x, cur.i1.y);
                 }
@@ -1028,8 +1037,8 @@ __global__ __launch_bounds__(kBlock) void kBatchSweeps(SolveArrays sa, const uin
         }
         for (uint32_t i = t; i < nb; i += kBlock) {
             const uint32_t body = bodyList[bb.x + i];
-            sa.vel[body] = make_float4(lv[i], lv[kLdsBodies + i], lv[2 * kLdsBodies + i], 0.0f);
-            sa.ang[body] = make_float4(lv[3 * kLdsBodies + i], lv[4 * kLdsBodies + i], lv[5 * kLdsBodies + i], 0.0f);
+            sa.vel[body] = make_float4(lv[i], lv[kBodies + i], lv[2 * kBodies + i], 0.0f);
+            sa.ang[body] = make_float4(lv[3 * kBodies + i], lv[4 * kBodies + i], lv[5 * kBodies + i], 0.0f);
         }
         if (kMode != 2)
             for (uint32_t i = t; i < nc; i += kBlock) {

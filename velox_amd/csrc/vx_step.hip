@@ -160,6 +160,8 @@ struct vxhip_ctx {
     // interact) and the per-world oversize lists of the broad phase.
     Buf<uint32_t> worldOf, worldOversize, worldStart, worldFill, oversizeByWorld;
     uint32_t worldCount = 1, worldN = 0;
+    bool worldsContiguous = false;  // bodies lie world after world (non-decreasing world ids)
+    Buf<uint32_t> colorSeg, colorSegCount;
     Buf<uint2> cellRange;  // per-cell run of the sorted keys (small grids)
     // Contact events of the last step and the pair keys they leave for the next.
     Buf<uint64_t> evKeys, evKeysTmp, evPrevKeys, evCurKeys;
@@ -211,6 +213,7 @@ struct vxhip_ctx {
     Buf<uint32_t> batchQueue, batchOrder, batchOrderTmp;
     Buf<uint64_t> batchKeys, batchKeysTmp;
     bool batchOrderReady = false;
+    bool batchSmall = false;  // this step's LDS batches all fit kBatchSweeps' small layout
     // Substeps 2..S of a step replayed from one captured graph.
     hipGraphExec_t subExec = nullptr;
     uint32_t graphReplays = 0, graphCaptures = 0;
@@ -304,7 +307,7 @@ enum Counter : uint32_t {
     cOversize2 = 17, cSmall = 18, cBig = 19, cRuns = 20, cIslands = 21, cBatches = 22, cLdsBatches = 23, cRanges = 24, cLargest = 28, cPasses = 29, cBatchError = 30, cCcdRelevant = 33, cLdsContacts = 34, cEvActive = 35, cEvRuns = 36, cEvEnds = 37, cEvInert = 38, cEvCount = 39,
     cSlDynamic = 40 /* dynamic, awake, drowsy, asleep, to sleep, to drowsy */, cSlWoke = 46,
     cSlIslands = 47 /* islands, asleep, drowsy, awake */, cCarried = 51, cTailSlots = 52, cShiftBad = 53, cJointsLeft = 54, cJointBreaks = 55,
-    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cWarmTies = 64, cCount = 72
+    cCcdRound = 56 /* pairs each closure round added, 8 rounds */, cWarmTies = 64, cBatchMaxBodies = 72, cBatchMaxRows = 73, cJointsBroken = 74, cCount = 75
 };
 static_assert(cCount <= 128, "kResetCounters runs one 128-lane workgroup");
 
@@ -640,10 +643,22 @@ uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + cb,
                             tempFor(c, vxhost::radixTempBytes(e)), s));
     if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, cb, c->pred.p);
-    if (m <= kColorSingleLimit)
+    const bool batched = c->worldCount > 1 && c->worldN == c->n && c->worldsContiguous;
+    if (m <= kColorSingleLimit) {
         kGreedyColor<<<1, kColorThreads, 0, s>>>(c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p,
                                                  c->color.p, c->colorCount.p, k + cColors, k + cColorError);
-    else {
+    } else if (batched) {
+        // One workgroup per world (the worlds' contact ranges are contiguous).
+        const uint32_t W = c->worldCount;
+        c->colorSeg.reserve(W + 1); c->colorSegCount.reserve(W);
+        fill32(c, c->colorSegCount.p, W, 0);
+        kContactWorlds<<<blocks(m), kThreads, 0, s>>>(cs, m, c->worldOf.p, c->colorSegCount.p);
+        HIPCK(vxhost::exclusiveScan(c->colorSegCount.p, c->colorSeg.p, W, c->colorSeg.p + W,
+                                    tempFor(c, vxhost::scanTempBytes(W)), s));
+        kGreedyColorWorlds<<<W, kThreads, 0, s>>>(c->bodies.p, cs, c->colorSeg.p, c->pred.p, c->maskAfter.p,
+                                                       c->nextAfter.p, c->color.p, c->colorCount.p, k + cColors,
+                                                       k + cColorError);
+    } else {
         ++c->stats.launches[VXHIP_LAUNCH_GREEDY_DEVICE];
         kGreedyColorDevice<<<std::min<uint32_t>(blocks(m), kColorBlocks), kThreads, 0, s>>>(
             c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p, c->color.p, c->colorCount.p, k + cColors,
@@ -755,7 +770,8 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
                                                    c->runStart.p, c->islandFirstRun.p, c->batchBodies.p);
         kRunSentinels<<<1, 1, 0, s>>>(c->runStart.p, k + cRuns, ms, c->islandFirstRun.p, k + cBatches,
                                       k + cLdsBatches, k + cRanges, k + cLdsContacts);
-        kBatchPasses<<<blocks(ms), kThreads, 0, s>>>(c->runStart.p, c->islandFirstRun.p, k + cLdsBatches, k + cPasses);
+        kBatchPasses<<<blocks(ms), kThreads, 0, s>>>(c->runStart.p, c->islandFirstRun.p, k + cLdsBatches, k + cPasses,
+                                                     c->batchBodies.p, k + cBatchMaxBodies, k + cBatchMaxRows);
         HIPCK(hipGetLastError());
     }
     kSolvePositions<<<blocks(m), kThreads, 0, s>>>(c->smallIdx.p, ms, c->bigIdx.p, c->bigCount, c->solvePos.p);
@@ -764,10 +780,13 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
     c->largestIsland = c->hostCounters[cLargest];
     c->maxBatchPasses = ms ? c->hostCounters[cPasses] : 0;
     c->ldsContactCount = 0;
+    c->batchSmall = false;
     if (ms) {
         c->batchCount = c->hostCounters[cBatches];
         c->ldsBatchCount = c->hostCounters[cLdsBatches];
         c->ldsContactCount = c->hostCounters[cLdsContacts];
+        // Every LDS batch fits the small layout: three workgroups share a CU.
+        c->batchSmall = c->hostCounters[cBatchMaxBodies] <= kSmallBodies && c->hostCounters[cBatchMaxRows] <= kSmallRows;
     }
     // Longest-first hand-out order of the LDS batches.
     c->batchOrderReady = false;
@@ -853,21 +872,27 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
             const uint64_t visits = (uint64_t)c->ldsContactCount * sweeps;
             solveLaunchBegin(c);
             ++c->stats.launches[VXHIP_LAUNCH_BATCH];
-            // Persistent: one workgroup per CU at most (a few CUs left free
-            // for the big-island chain beside it).
-            const uint32_t cap = c->persistentBatches
-                                     ? std::max<uint32_t>(1u, c->cuCount - (c->bigCount ? kChainCus : 0u))
-                                     : 65535u;
+            // Persistent: as many workgroups as the CUs hold at once (one with
+            // the full LDS layout, two or three with the small one), a few CUs
+            // left free for the big-island chain beside them.
+            const bool small = c->batchSmall;
+            const uint32_t perCu = small ? (c->batchThreads == 512 ? 2u : 3u) : 1u;
+            const uint32_t cus = c->bigCount ? std::max<uint32_t>(1u, c->cuCount - kChainCus) : c->cuCount;
+            const uint32_t cap = c->persistentBatches ? cus * perCu : 65535u;
             const uint32_t grid = std::min<uint32_t>(lds, cap);
             const uint32_t* order = c->batchOrderReady ? c->batchOrder.p : nullptr;
-            if (c->batchThreads == 512)
-                kBatchSweeps<kMode, 512><<<grid, 512, 0, bs>>>(
-                    sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p, c->bodyList.p,
-                    c->counters.p + cRanges, order, c->batchQueue.p, sweeps, c->counters.p + cBatchError);
-            else
-                kBatchSweeps<kMode, 256><<<grid, 256, 0, bs>>>(
-                    sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p, c->batchBodies.p, c->bodyList.p,
-                    c->counters.p + cRanges, order, c->batchQueue.p, sweeps, c->counters.p + cBatchError);
+#define VX_BATCH_LAUNCH(T, SMALL)                                                                                   \
+    kBatchSweeps<kMode, T, SMALL><<<grid, T, 0, bs>>>(sa, c->slotPair.p, c->runStart.p, c->islandFirstRun.p,        \
+                                                      c->batchBodies.p, c->bodyList.p, c->counters.p + cRanges, order, \
+                                                      c->batchQueue.p, sweeps, c->counters.p + cBatchError)
+            if (c->batchThreads == 512) {
+                if (small) VX_BATCH_LAUNCH(512, true);
+                else VX_BATCH_LAUNCH(512, false);
+            } else {
+                if (small) VX_BATCH_LAUNCH(256, true);
+                else VX_BATCH_LAUNCH(256, false);
+            }
+#undef VX_BATCH_LAUNCH
             solveLaunchEnd(c, kMode == 2 ? VXHIP_KERNEL_BATCH_WARM : VXHIP_KERNEL_BATCH, visits);
         }
         if (global) {
@@ -1381,7 +1406,7 @@ void finishSleep(vxhip_ctx* c) {
 
 // One full device step of the resident world.
 // Captures one later substep (integration, the velocity solve without the
-// impulse store, transforms) on the context's stream and turns it into the
+// impulse store, the joint passes, transforms) on the context's stream and turns it into the
 // context's executable graph: updated in place when the new capture has the
 // old one's shape, instantiated anew otherwise.  The capture only records:
 // every buffer it names was sized by the step's first substep, and no fill in
@@ -1396,6 +1421,7 @@ void captureSubstep(vxhip_ctx* c, uint32_t K, V3 g, float h) {
     try {
         kIntegrate<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, g, h);
         substepSolve(c, c->solve.p, K, h, c->cfg.options, false, false);
+        substepJoints(c, h, false);
         kTransforms<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, h);
     } catch (...) {
         (void)hipStreamEndCapture(s, &graph);
@@ -1485,12 +1511,13 @@ void worldStep(vxhip_ctx* c, float dt) {
     }
     c->solveCount = m;
     markStage(c, sSolve);
-    if (c->jointCount) kJointWake<<<1, 1, 0, s>>>(c->bodies.p, c->joints.p, c->jointCount);  // backend_cuda.cu:704-708
+    if (c->jointCount && c->anyAsleep)  // backend_cuda.cu:704-708 (nothing to wake when no body sleeps)
+        kJointWake<<<1, 1, 0, s>>>(c->bodies.p, c->joints.p, c->jointCount);
     // Substeps 2..S launch the same kernels with the same arguments once the
     // first has set up the step's islands and rows: with graphSubsteps they are
     // captured once (as substep 2) and replayed from one executable graph,
     // updated in place from step to step while its shape holds.
-    const bool graph = c->graphSubsteps && m && S > 1 && !c->profile && !cap && !c->bigCount && !c->jointCount;
+    const bool graph = c->graphSubsteps && m && S > 1 && !c->profile && !cap && !c->bigCount;
     for (int st = 0; st < S; ++st) {
         if (graph && st > 0) {
             if (st == 1) captureSubstep(c, K, g, h);
@@ -1511,7 +1538,9 @@ void worldStep(vxhip_ctx* c, float dt) {
     kResetCounters<<<1, 128, 0, s>>>(k);
     if (c->jointCount) {  // World::finishBrokenJoints (world.cpp:3765-3783)
         c->jointBreaks.reserve(c->jointCount);
-        kJointFinish<<<1, 1, 0, s>>>(c->joints.p, c->jointCount, h, c->jointBreaks.p, k + cJointsLeft, k + cJointBreaks);
+        kJointBroken<<<blocks(c->jointCount), kThreads, 0, s>>>(c->joints.p, c->jointCount, h, k + cJointsBroken);
+        kJointFinish<<<1, 1, 0, s>>>(c->joints.p, c->jointCount, h, c->jointBreaks.p, k + cJointsLeft, k + cJointBreaks,
+                                     k + cJointsBroken);
     }
     highToi(c, m, dt);
     ccdNet(c, np, dt);
@@ -1970,6 +1999,9 @@ int vxhip_world_set_worlds(vxhip_ctx* c, const uint32_t* worldOf, uint32_t world
         }
         c->worldCount = worldCount;
         c->worldN = c->n;
+        c->worldsContiguous = worldCount > 1;
+        for (uint32_t i = 1; worldCount > 1 && i < c->n; ++i)
+            if (worldOf[i] < worldOf[i - 1]) c->worldsContiguous = false;
     });
 }
 
@@ -2063,19 +2095,27 @@ int vxhip_world_set_events(vxhip_ctx* c, int enable) {
     });
 }
 
+namespace {
+void setJoints(vxhip_ctx* c, const vx_joint* joints, uint32_t count, bool anyCount) {
+    if (!c || !c->loaded || (!joints && count)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+    if (!anyCount && count && count < kMinDeviceJoints)
+        fail(VXHIP_ERR_UNAVAILABLE,
+             "vxhip: fewer than 64 joints are solved by the host world (World::solveJoints), as with the CUDA backend");
+    for (uint32_t k = 0; k < count; ++k)
+        if (joints[k].type > VX_JOINT_GEAR || joints[k].a >= c->n || joints[k].b >= c->n)
+            fail(VXHIP_ERR_INVALID, "vxhip: joint with an unknown type or body");
+    c->hostJoints.assign(joints, joints + count);
+    prepareJoints(c);
+    c->jointBreakCount = 0;
+}
+}  // namespace
+
 int vxhip_world_set_joints(vxhip_ctx* c, const vx_joint* joints, uint32_t count) {
-    return guarded([&] {
-        if (!c || !c->loaded || (!joints && count)) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
-        if (count && count < kMinDeviceJoints)
-            fail(VXHIP_ERR_UNAVAILABLE,
-                 "vxhip: fewer than 64 joints are solved by the host world (World::solveJoints), as with the CUDA backend");
-        for (uint32_t k = 0; k < count; ++k)
-            if (joints[k].type > VX_JOINT_GEAR || joints[k].a >= c->n || joints[k].b >= c->n)
-                fail(VXHIP_ERR_INVALID, "vxhip: joint with an unknown type or body");
-        c->hostJoints.assign(joints, joints + count);
-        prepareJoints(c);
-        c->jointBreakCount = 0;
-    });
+    return guarded([&] { setJoints(c, joints, count, false); });
+}
+
+int vxhip_world_set_joints_all(vxhip_ctx* c, const vx_joint* joints, uint32_t count) {
+    return guarded([&] { setJoints(c, joints, count, true); });
 }
 
 int vxhip_world_download_joints(vxhip_ctx* c, vx_joint* out, uint32_t cap, uint32_t* count) {

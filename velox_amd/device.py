@@ -147,6 +147,7 @@ SIGNATURES = {
     "vxhip_world_set_multi_toi": (_i32, [_vp, _vp]),
     "vxhip_world_query_multi_toi": (_i32, [_vp, _u32, _f32, _vp, _u32, _vp]),
     "vxhip_world_set_joints": (_i32, [_vp, _vp, _u32]),
+    "vxhip_world_set_joints_all": (_i32, [_vp, _vp, _u32]),
     "vxhip_world_download_joints": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_download_joint_breaks": (_i32, [_vp, _vp, _u32, _vp]),
     "vxhip_world_state_size": (_i32, [_vp, _vp]),
@@ -174,6 +175,7 @@ SIGNATURES = {
     "vxhip_scene_bodies": (_vp, [_vp]),
     "vxhip_scene_worlds": (_vp, [_vp, _vp]),
     "vxhip_scene_soup": (_i32, [_vp, _vp]),
+    "vxhip_scene_joints": (_vp, [_vp, _vp]),
     "vxhip_scene_config": (_i32, [_vp, _vp, _vp]),
     "vxhip_scene_free": (None, [_vp]),
 }
@@ -251,6 +253,11 @@ class Scene:
             self.worlds = wc.value
             self.world_of = np.frombuffer(bytes((C.c_uint8 * (4 * n)).from_address(wp)), np.uint32).copy() \
                 if n else np.zeros(0, np.uint32)
+            from .records import JOINT
+            jc = C.c_uint32(0)
+            jp = lib.vxhip_scene_joints(h, C.byref(jc))
+            self.joints = np.frombuffer(bytes((C.c_uint8 * (JOINT.itemsize * jc.value)).from_address(jp)),
+                                        JOINT).copy() if jc.value else np.zeros(0, JOINT)
             view = SoupView()
             check(lib.vxhip_scene_soup(h, C.byref(view)))
             cfg = Config()
@@ -283,7 +290,14 @@ class Scene:
         arrays, the world's bodies in their batched order)."""
         sub = object.__new__(Scene)
         sub.name, sub.param = self.name, self.param
-        sub.bodies = np.ascontiguousarray(self.bodies[self.world_of == w])
+        mine = self.world_of == w
+        sub.bodies = np.ascontiguousarray(self.bodies[mine])
+        local = np.cumsum(mine, dtype=np.int64) - 1  # body index -> index within world w
+        j = self.joints[mine[self.joints["a"]]] if len(self.joints) else self.joints
+        sub.joints = j.copy()
+        if len(j):
+            sub.joints["a"] = local[j["a"]]
+            sub.joints["b"] = local[j["b"]]
         sub.worlds = 1
         sub.world_of = np.zeros(len(sub.bodies), np.uint32)
         sub.gravity, sub.substeps, sub.dt, sub.soup = self.gravity, self.substeps, self.dt, self.soup
@@ -332,6 +346,9 @@ class Context:
         self.load(scene.bodies, scene.soup, gravity=scene.gravity, substeps=scene.substeps, options=options)
         if getattr(scene, "worlds", 1) > 1:
             self.set_worlds(scene.world_of, scene.worlds)
+        joints = getattr(scene, "joints", None)
+        if joints is not None and len(joints):
+            self.set_joints(joints, any_count=True)
         return self
 
     def set_tuning(self, **knobs):
@@ -422,11 +439,14 @@ class Context:
         check(library().vxhip_world_set_sleep(self._h, C.byref(cfg)))
         return self
 
-    def set_joints(self, joints):
-        """Joint records (numpy array of the JOINT dtype, or raw 420-byte rows)."""
+    def set_joints(self, joints, any_count=False):
+        """Joint records (numpy array of the JOINT dtype, or raw 420-byte rows).
+        any_count: solve sets of fewer than 64 on the device too
+        (vxhip_world_set_joints_all)."""
         from .records import JOINT
         arr = np.ascontiguousarray(joints).view(np.uint8).reshape(-1, JOINT.itemsize).copy().view(JOINT).reshape(-1)
-        check(library().vxhip_world_set_joints(self._h, ptr(arr) if len(arr) else None, len(arr)))
+        fn = library().vxhip_world_set_joints_all if any_count else library().vxhip_world_set_joints
+        check(fn(self._h, ptr(arr) if len(arr) else None, len(arr)))
         return self
 
     def joints(self):
