@@ -642,12 +642,63 @@ __global__ __launch_bounds__(kThreads) void kColorPreds(const uint64_t* __restri
 constexpr int kColorThreads = 1024;
 
 // Greedy first-free-bit coloring (backend_cuda.cu:597-630) in contact order,
-// reproduced exactly by dataflow inside one workgroup: a contact's colour
-// depends only on the masks its two bodies carry after their previous
-// contacts, so each lane waits for those two predecessors and nothing else.
-// All waves share one CU, so workgroup-scope release/acquire orders the
-// hand-offs.  The lowest unfinished contact is always ready, hence progress;
-// a spin budget still bounds every wait.
+// reproduced exactly by dataflow: a contact's colour depends only on the
+// masks its two bodies carry after their previous contacts, so each lane
+// waits for those two predecessors and nothing else.  The lowest unfinished
+// contact is always ready, hence progress; a spin budget still bounds every
+// wait.  The kernels below differ in who takes which contacts and where the
+// hand-offs live; they share the colour rule (greedyPick) and count colours
+// in an LDS histogram flushed once per workgroup (one global atomic per
+// colour and workgroup instead of one per contact).
+constexpr uint32_t kColorHist = 256;  // colours counted in LDS; higher ones go to global atomics
+
+__device__ inline int greedyPick(bool sa, bool sb, uint32_t pa, uint32_t pb, unsigned long long maskA,
+                                 unsigned long long maskB, int& nextA, int& nextB, unsigned long long& outA,
+                                 unsigned long long& outB) {
+    if (sa || pa == kNone) { maskA = 0; nextA = 64; }
+    if (sb || pb == kNone) { maskB = 0; nextB = 64; }
+    const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
+    int col;
+    if (~used) {
+        col = __ffsll((long long)~used) - 1;
+    } else {
+        int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
+        col = na > nb2 ? na : nb2;
+        if (!sa) nextA = col + 1;
+        if (!sb) nextB = col + 1;
+    }
+    if (col < 64) {
+        if (!sa) maskA |= 1ull << col;
+        if (!sb) maskB |= 1ull << col;
+    }
+    outA = maskA;
+    outB = maskB;
+    return col;
+}
+
+struct ColorHist {
+    uint32_t* hist;  // LDS, kColorHist entries
+    uint32_t* top;   // LDS, largest colour + 1
+    __device__ void init() {
+        for (uint32_t i = threadIdx.x; i < kColorHist; i += blockDim.x) hist[i] = 0u;
+        if (threadIdx.x == 0) *top = 0u;
+        __syncthreads();
+    }
+    __device__ void add(int col, uint32_t* __restrict__ colorCount) {
+        if ((uint32_t)col < kColorHist) atomicAdd(&hist[col], 1u);
+        else atomicAdd(&colorCount[col], 1u);
+        atomicMax(top, (uint32_t)col + 1u);
+    }
+    __device__ void flush(uint32_t* __restrict__ colorCount, uint32_t* __restrict__ numColors) {
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < kColorHist; i += blockDim.x)
+            if (hist[i]) atomicAdd(&colorCount[i], hist[i]);
+        if (threadIdx.x == 0 && *top) atomicMax(numColors, *top);
+    }
+};
+
+// One workgroup over all contacts; workgroup-scope release/acquire orders
+// the hand-offs (all waves share one CU).
 __global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __restrict__ bodies,
                                                               const vx_contact* __restrict__ cs, uint32_t m,
                                                               const uint32_t* __restrict__ pred,
@@ -655,6 +706,9 @@ __global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __r
                                                               int* __restrict__ nextAfter, uint32_t* color,
                                                               uint32_t* __restrict__ colorCount,
                                                               uint32_t* __restrict__ numColors, uint32_t* __restrict__ error) {
+    __shared__ uint32_t hist[kColorHist], top;
+    ColorHist h{hist, &top};
+    h.init();
     uint32_t k = threadIdx.x;
     uint32_t spins = 0;
     while (k < m) {
@@ -664,28 +718,14 @@ __global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __r
         if (readyA && readyB) {
             const vx_contact& c = cs[k];
             const bool sa = !constrainedSide(bodies, c, 0), sb = !constrainedSide(bodies, c, 1);
-            unsigned long long maskA = 0, maskB = 0;
-            int nextA = 64, nextB = 64;
-            if (!sa && pa != kNone) { maskA = maskAfter[pa]; nextA = nextAfter[pa]; }
-            if (!sb && pb != kNone) { maskB = maskAfter[pb]; nextB = nextAfter[pb]; }
-            const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
-            int col;
-            if (~used) {
-                col = __ffsll((long long)~used) - 1;
-            } else {
-                int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
-                col = na > nb2 ? na : nb2;
-                if (!sa) nextA = col + 1;
-                if (!sb) nextB = col + 1;
-            }
-            if (col < 64) {
-                if (!sa) maskA |= 1ull << col;
-                if (!sb) maskB |= 1ull << col;
-            }
+            int nextA = 0, nextB = 0;
+            unsigned long long mA = 0, mB = 0, maskA = 0, maskB = 0;
+            if (!sa && pa != kNone) { mA = maskAfter[pa]; nextA = nextAfter[pa]; }
+            if (!sb && pb != kNone) { mB = maskAfter[pb]; nextB = nextAfter[pb]; }
+            const int col = greedyPick(sa, sb, pa, pb, mA, mB, nextA, nextB, maskA, maskB);
             maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
             maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
-            atomicAdd(&colorCount[col], 1u);
-            atomicMax(numColors, (uint32_t)col + 1u);
+            h.add(col, colorCount);
             __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             k += kColorThreads;
             spins = 0;
@@ -694,14 +734,19 @@ __global__ __launch_bounds__(kColorThreads) void kGreedyColor(const vx_body* __r
             break;
         }
     }
+    h.flush(colorCount, numColors);
 }
 
 // Batched worlds whose bodies lie world after world: the contacts of one
 // world are contiguous in contact order and none has a predecessor outside
-// it, so each world is coloured by one workgroup (the dataflow above over its
-// range [segStart[w], segStart[w + 1])), every world independently.  A lane
-// reads its contact and the two constrained-side flags before it waits, so a
-// hand-off costs only the predecessors' masks.
+// it, so each world is coloured by one workgroup over its range
+// [segStart[w], segStart[w + 1]), every world independently.  A world of up
+// to kWorldColorLds contacts keeps its colours and masks in LDS (hand-offs
+// through LDS, results copied out at the end); a larger one works on the
+// global arrays as kGreedyColor does.  A lane reads its contact's
+// predecessors and constrained-side flags before it waits.
+constexpr uint32_t kWorldColorLds = 2048;
+
 __global__ __launch_bounds__(kThreads) void kGreedyColorWorlds(const vx_body* __restrict__ bodies,
                                                                const vx_contact* __restrict__ cs,
                                                                const uint32_t* __restrict__ segStart,
@@ -711,7 +756,15 @@ __global__ __launch_bounds__(kThreads) void kGreedyColorWorlds(const vx_body* __
                                                                uint32_t* __restrict__ colorCount,
                                                                uint32_t* __restrict__ numColors,
                                                                uint32_t* __restrict__ error) {
+    __shared__ uint32_t hist[kColorHist], top;
+    __shared__ uint32_t lcol[kWorldColorLds];
+    __shared__ unsigned long long lmask[2 * kWorldColorLds];
+    __shared__ int lnext[2 * kWorldColorLds];
+    ColorHist h{hist, &top};
     const uint32_t begin = segStart[blockIdx.x], end = segStart[blockIdx.x + 1];
+    const bool inLds = end - begin <= kWorldColorLds;
+    for (uint32_t i = threadIdx.x; inLds && i < end - begin; i += kThreads) lcol[i] = kNone;
+    h.init();
     uint32_t k = begin + threadIdx.x;
     uint32_t spins = 0;
     bool loaded = false, sa = false, sb = false;
@@ -725,32 +778,37 @@ __global__ __launch_bounds__(kThreads) void kGreedyColorWorlds(const vx_body* __
             sb = !constrainedSide(bodies, c, 1);
             loaded = true;
         }
-        const bool readyA = pa == kNone || __hip_atomic_load(&color[pa >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
-        const bool readyB = pb == kNone || __hip_atomic_load(&color[pb >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        bool readyA, readyB;
+        if (inLds) {
+            readyA = pa == kNone || __hip_atomic_load(&lcol[(pa >> 1) - begin], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+            readyB = pb == kNone || __hip_atomic_load(&lcol[(pb >> 1) - begin], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        } else {
+            readyA = pa == kNone || __hip_atomic_load(&color[pa >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+            readyB = pb == kNone || __hip_atomic_load(&color[pb >> 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) != kNone;
+        }
         if (readyA && readyB) {
-            unsigned long long maskA = 0, maskB = 0;
-            int nextA = 64, nextB = 64;
-            if (!sa && pa != kNone) { maskA = maskAfter[pa]; nextA = nextAfter[pa]; }
-            if (!sb && pb != kNone) { maskB = maskAfter[pb]; nextB = nextAfter[pb]; }
-            const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
-            int col;
-            if (~used) {
-                col = __ffsll((long long)~used) - 1;
+            int nextA = 0, nextB = 0;
+            unsigned long long mA = 0, mB = 0, maskA = 0, maskB = 0;
+            if (inLds) {
+                if (!sa && pa != kNone) { mA = lmask[pa - 2 * begin]; nextA = lnext[pa - 2 * begin]; }
+                if (!sb && pb != kNone) { mB = lmask[pb - 2 * begin]; nextB = lnext[pb - 2 * begin]; }
             } else {
-                int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
-                col = na > nb2 ? na : nb2;
-                if (!sa) nextA = col + 1;
-                if (!sb) nextB = col + 1;
+                if (!sa && pa != kNone) { mA = maskAfter[pa]; nextA = nextAfter[pa]; }
+                if (!sb && pb != kNone) { mB = maskAfter[pb]; nextB = nextAfter[pb]; }
             }
-            if (col < 64) {
-                if (!sa) maskA |= 1ull << col;
-                if (!sb) maskB |= 1ull << col;
+            const int col = greedyPick(sa, sb, pa, pb, mA, mB, nextA, nextB, maskA, maskB);
+            if (inLds) {
+                const uint32_t j = 2 * (k - begin);
+                lmask[j] = maskA; lnext[j] = nextA;
+                lmask[j + 1] = maskB; lnext[j + 1] = nextB;
+                color[k] = (uint32_t)col;
+                __hip_atomic_store(&lcol[k - begin], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
+                maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
+                __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             }
-            maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
-            maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
-            atomicAdd(&colorCount[col], 1u);
-            atomicMax(numColors, (uint32_t)col + 1u);
-            __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            h.add(col, colorCount);
             k += kThreads;
             loaded = false;
             spins = 0;
@@ -759,6 +817,7 @@ __global__ __launch_bounds__(kThreads) void kGreedyColorWorlds(const vx_body* __
             break;
         }
     }
+    h.flush(colorCount, numColors);
 }
 
 // Contacts per world (by the first body), for kGreedyColorWorlds' ranges.
@@ -785,6 +844,9 @@ __global__ __launch_bounds__(kThreads) void kGreedyColorDevice(const vx_body* __
                                                                uint32_t* __restrict__ colorCount,
                                                                uint32_t* __restrict__ numColors,
                                                                uint32_t* __restrict__ error) {
+    __shared__ uint32_t hist[kColorHist], top;
+    ColorHist h{hist, &top};
+    h.init();
     const uint32_t stride = gridDim.x * kThreads;
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     uint32_t spins = 0;
@@ -796,28 +858,14 @@ __global__ __launch_bounds__(kThreads) void kGreedyColorDevice(const vx_body* __
             if (pa != kNone || pb != kNone) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             const vx_contact& c = cs[k];
             const bool sa = !constrainedSide(bodies, c, 0), sb = !constrainedSide(bodies, c, 1);
-            unsigned long long maskA = 0, maskB = 0;
-            int nextA = 64, nextB = 64;
-            if (!sa && pa != kNone) { maskA = maskAfter[pa]; nextA = nextAfter[pa]; }
-            if (!sb && pb != kNone) { maskB = maskAfter[pb]; nextB = nextAfter[pb]; }
-            const unsigned long long used = (sa ? 0ull : maskA) | (sb ? 0ull : maskB);
-            int col;
-            if (~used) {
-                col = __ffsll((long long)~used) - 1;
-            } else {
-                int na = sa ? 64 : nextA, nb2 = sb ? 64 : nextB;
-                col = na > nb2 ? na : nb2;
-                if (!sa) nextA = col + 1;
-                if (!sb) nextB = col + 1;
-            }
-            if (col < 64) {
-                if (!sa) maskA |= 1ull << col;
-                if (!sb) maskB |= 1ull << col;
-            }
+            int nextA = 0, nextB = 0;
+            unsigned long long mA = 0, mB = 0, maskA = 0, maskB = 0;
+            if (!sa && pa != kNone) { mA = maskAfter[pa]; nextA = nextAfter[pa]; }
+            if (!sb && pb != kNone) { mB = maskAfter[pb]; nextB = nextAfter[pb]; }
+            const int col = greedyPick(sa, sb, pa, pb, mA, mB, nextA, nextB, maskA, maskB);
             maskAfter[2 * k] = maskA; nextAfter[2 * k] = nextA;
             maskAfter[2 * k + 1] = maskB; nextAfter[2 * k + 1] = nextB;
-            atomicAdd(&colorCount[col], 1u);
-            atomicMax(numColors, (uint32_t)col + 1u);
+            h.add(col, colorCount);
             __hip_atomic_store(&color[k], (uint32_t)col, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             k += stride;
             spins = 0;
@@ -826,6 +874,7 @@ __global__ __launch_bounds__(kThreads) void kGreedyColorDevice(const vx_body* __
             break;
         }
     }
+    h.flush(colorCount, numColors);
 }
 
 // ----------------------------------------------------------------- solver --
