@@ -460,6 +460,12 @@ int vxhip_probe_color(vxhip_ctx* ctx, const vx_body* bodies, uint32_t n, const v
  * (world.cpp:4133-4184). */
 int vxhip_probe_warm_start(vxhip_ctx* ctx, vx_contact* cur, uint32_t m, const vx_contact* prev, uint32_t mp);
 
+/* The device step's stable LSD radix sort of (key, value) pairs by the low
+ * `bits` (0..64) of the key, in place (no reference counterpart: it stands
+ * where the reference calls Thrust's sort, and any stable sort gives its
+ * result, so it is checked against a host stable sort). */
+int vxhip_probe_radix_sort(vxhip_ctx* ctx, uint64_t* keys, uint32_t* vals, uint32_t n, int bits);
+
 /* std::sort (libstdc++) of (key, value) pairs by key, reproduced exactly on
  * the device, in place: the order equal keys are left in is std::sort's.
  * With segStart (segments + 1 non-decreasing offsets, 0 to n), each segment

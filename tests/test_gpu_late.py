@@ -95,3 +95,22 @@ def test_slab_stack_matches_reference(ctx):
     # 4,096 contacts at step 1, kIslandSweeps' range).
     assert all(ran[k] > 0 for k in ("batch", "island", "color_pair", "color_run", "tail")), ran
     assert ties > 0, "no warm-start tie was reached: the fixture no longer exercises the cache order"
+
+
+@pytest.mark.parametrize("n,bits,spread,seed", [(2, 8, 4, 1), (1000, 8, 256, 2), (4096, 16, 3, 3), (4097, 16, 70000, 4),
+                                                (20000, 21, 1 << 21, 5), (70000, 24, 50, 6), (185000, 42, 1 << 42, 7),
+                                                (300001, 64, 1 << 63, 8), (100000, 3, 8, 9)])
+def test_device_radix_sort_is_a_stable_sort(ctx, n, bits, spread, seed):
+    """The step's radix sort (one launch per 8-bit pass, tile offsets by
+    look-back) against numpy's stable argsort on the low `bits` of the key:
+    single and many tiles, few and many distinct keys, keys with bits above
+    `bits` set (ignored, order kept)."""
+    rng = np.random.default_rng(seed)
+    keys = rng.integers(0, spread, n, dtype=np.uint64) | (rng.integers(0, 2, n, dtype=np.uint64) << np.uint64(63))
+    vals = np.arange(n, dtype=np.uint32)
+    print(f"[gpu] radix sort n={n} bits={bits}", flush=True)
+    got_k, got_v = ctx.radix_sort(keys, vals, bits)
+    mask = np.uint64((1 << bits) - 1) if bits < 64 else np.uint64(0xFFFFFFFFFFFFFFFF)
+    order = np.argsort(keys & mask, kind="stable")
+    assert np.array_equal(got_v, vals[order])
+    assert np.array_equal(got_k, keys[order])

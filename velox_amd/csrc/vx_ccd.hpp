@@ -287,9 +287,11 @@ __global__ __launch_bounds__(kThreads) void kCcdRelList(const uint32_t* __restri
 // sorted, they give each pair its predecessor on each body.
 __global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restrict__ bodies,
                                                         const uint64_t* __restrict__ pairs,
-                                                        const uint32_t* __restrict__ list, uint32_t nr, int nb,
+                                                        const uint32_t* __restrict__ list,
+                                                        const uint32_t* __restrict__ nrDev, int nb,
                                                         int pbits, uint64_t* __restrict__ keys,
                                                         uint32_t* __restrict__ vals, uint32_t* __restrict__ counter) {
+    const uint32_t nr = *nrDev;
     uint32_t t = blockIdx.x * kThreads + threadIdx.x;
     bool s0 = false, s1 = false;
     uint32_t i = 0, j = 0, p = 0;
@@ -322,13 +324,15 @@ constexpr int kCoopRescues = 2;  // at most this many ready rescues share the wa
 __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const vx_vec3* __restrict__ prevPos,
                                                          const vx_quat* __restrict__ prevQ, Soup soup, float dt,
                                                          const uint64_t* __restrict__ pairs,
-                                                         const uint32_t* __restrict__ list, uint32_t nr, int nb,
+                                                         const uint32_t* __restrict__ list,
+                                                         const uint32_t* __restrict__ nrDev, int nb,
                                                          const uint8_t* __restrict__ flag,
                                                          const float4* __restrict__ endGap,
                                                          const uint32_t* __restrict__ pred, uint32_t* word,
                                                          uint32_t* __restrict__ examined,
                                                          uint32_t* __restrict__ rescued, uint32_t* __restrict__ error,
                                                          const uint8_t* __restrict__ toiDone) {
+    const uint32_t nr = *nrDev;
     const uint32_t waves = gridDim.x * (kThreads / 64);
     const uint32_t wave = blockIdx.x * (kThreads / 64) + threadIdx.x / 64, lane = threadIdx.x % 64;
     const uint32_t stride = 64 * waves;

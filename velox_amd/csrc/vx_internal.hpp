@@ -7,8 +7,9 @@
 namespace vxhost {
 
 size_t radixTempBytes(uint32_t n);
+// With nDev, the count is read on the device and n is only the capacity.
 hipError_t radixSort(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t* valsTmp, uint32_t n, int bits,
-                     void* temp, hipStream_t s);
+                     void* temp, hipStream_t s, const uint32_t* nDev = nullptr);
 size_t scanTempBytes(uint32_t n);
 hipError_t exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, void* temp, hipStream_t s);
 

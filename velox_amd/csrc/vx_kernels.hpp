@@ -629,9 +629,12 @@ __global__ __launch_bounds__(kThreads) void kColorEntries(const vx_body* __restr
 }
 
 // For each constrained side, the previous contact side on the same body.
+// (`eDev`: the entry count on the device, e then only bounds the grid.)
 __global__ __launch_bounds__(kThreads) void kColorPreds(const uint64_t* __restrict__ keys, const uint32_t* __restrict__ vals,
-                                                        uint32_t e, int cb, uint32_t* __restrict__ pred) {
+                                                        uint32_t e, int cb, uint32_t* __restrict__ pred,
+                                                        const uint32_t* __restrict__ eDev = nullptr) {
     uint32_t t = blockIdx.x * kThreads + threadIdx.x;
+    if (eDev) e = *eDev;
     if (t >= e) return;
     uint32_t side = vals[t];
     uint32_t p = kNone;

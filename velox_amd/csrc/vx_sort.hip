@@ -1,11 +1,11 @@
 // Hand-written device sort and scan primitives for the HIP device step.
 //
 //  * LSD radix sort of (u64 key, u32 value) pairs over the low `bits` of the
-//    key, 8-bit digits.  Each pass: per-tile digit histograms (LDS), one scan
-//    over the (digit-major, tile-minor) histogram table, then a stable scatter
-//    in which every lane ranks itself among equal digits of its wavefront with
-//    eight ballots (a 64-lane match mask) and waves are chained through LDS.
-//    Stability per pass is what makes the multi-pass LSD sort correct.
+//    key, 8-bit digits, one launch per pass (tile offsets by decoupled
+//    look-back) after one launch that counts every pass's digits.  The pass's
+//    scatter ranks every lane among equal digits of its wavefront with eight
+//    ballots (a 64-lane match mask) and chains waves through LDS.  Stability
+//    per pass is what makes the multi-pass LSD sort correct.
 //  * Exclusive scan of u32 counts: one workgroup up to 16384 counts, else
 //    tile sums, scan of tile sums, add-back.
 #include <algorithm>
@@ -21,63 +21,138 @@ constexpr int kSortTile = kSortThreads * kSortItems;
 
 __device__ inline uint32_t digitOf(uint64_t key, int shift) { return (uint32_t)(key >> shift) & 0xffu; }
 
-__global__ __launch_bounds__(kSortThreads) void kRadixHistogram(const uint64_t* __restrict__ keys, uint32_t n,
-                                                                int shift, uint32_t* __restrict__ hist,
-                                                                uint32_t tiles) {
-    __shared__ uint32_t counts[256];
-    const uint32_t tile = blockIdx.x;
-    counts[threadIdx.x] = 0;
+// One launch per pass.  Before the passes, one read of the keys counts the
+// digits of every pass (kRadixDigitCounts).  A pass's workgroup takes the
+// next tile from a counter (so every lower tile has started), counts its
+// tile's digits, publishes them for its successors, and finds the tile's
+// offset per digit by looking back over its predecessors' published words
+// (decoupled look-back: a word holds either the tile's own count, or the
+// inclusive prefix through that tile; the walk adds own counts until it
+// meets a prefix).  Tile 0 publishes its prefix at once, so the walk always
+// ends, and a tile waits only on tiles that started before it.  The scatter
+// then ranks each lane among equal digits of its wavefront with eight
+// ballots and chains the wavefronts through LDS, so a pass is stable.
+constexpr uint32_t kLookAggregate = 1u << 30, kLookPrefix = 2u << 30, kLookCount = (1u << 30) - 1;
+constexpr int kMaxPasses = 8;
+
+// `nDev`: the element count on the device (then `n` is only the capacity the
+// grid was sized for).
+__global__ __launch_bounds__(kSortThreads) void kRadixDigitCounts(const uint64_t* __restrict__ keys, uint32_t n,
+                                                                  const uint32_t* __restrict__ nDev, int passes,
+                                                                  uint32_t* __restrict__ digitCounts) {
+    __shared__ uint32_t counts[kMaxPasses][256];
+    if (nDev) n = *nDev;
+    if (blockIdx.x * kSortTile >= n) return;
+    for (int p = 0; p < kMaxPasses; ++p) counts[p][threadIdx.x] = 0;
     __syncthreads();
-    const uint32_t base = tile * kSortTile;
+    const uint32_t base = blockIdx.x * kSortTile;
     for (int r = 0; r < kSortItems; ++r) {
-        uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&counts[digitOf(keys[i], shift)], 1u);
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) {
+            const uint64_t k = keys[i];
+            for (int p = 0; p < passes; ++p) atomicAdd(&counts[p][digitOf(k, 8 * p)], 1u);
+        }
     }
     __syncthreads();
-    hist[threadIdx.x * tiles + tile] = counts[threadIdx.x];
+    for (int p = 0; p < passes; ++p)
+        if (counts[p][threadIdx.x]) atomicAdd(&digitCounts[p * 256 + threadIdx.x], counts[p][threadIdx.x]);
 }
 
-__global__ __launch_bounds__(kSortThreads) void kRadixScatter(const uint64_t* __restrict__ keysIn,
-                                                              const uint32_t* __restrict__ valsIn, uint32_t n,
-                                                              int shift, const uint32_t* __restrict__ offsets,
-                                                              uint32_t tiles, uint64_t* __restrict__ keysOut,
-                                                              uint32_t* __restrict__ valsOut) {
+__global__ __launch_bounds__(kSortThreads) void kRadixPass(const uint64_t* __restrict__ keysIn,
+                                                           const uint32_t* __restrict__ valsIn, uint32_t n,
+                                                           const uint32_t* __restrict__ nDev, int shift,
+                                                           const uint32_t* __restrict__ digitCounts,
+                                                           uint32_t* __restrict__ tileCounter, uint32_t* status,
+                                                           uint64_t* __restrict__ keysOut,
+                                                           uint32_t* __restrict__ valsOut) {
     __shared__ uint32_t running[256];
     __shared__ uint32_t waveCount[kSortWaves][256];
-    const uint32_t tile = blockIdx.x;
+    __shared__ uint32_t mine[256];
+    __shared__ uint32_t waveSums[kSortWaves];
+    __shared__ uint32_t tileS;
+    const uint32_t d = threadIdx.x;
+    if (nDev) n = *nDev;
+    if (d == 0) tileS = atomicAdd(tileCounter, 1u);
+    mine[d] = 0;
+    for (int w = 0; w < kSortWaves; ++w) waveCount[w][d] = 0;
+    __syncthreads();
+    const uint32_t tile = tileS;
+    const uint32_t base = tile * kSortTile;
+    if (base >= n) return;  // past the last tile: no tile waits on it
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&mine[digitOf(keysIn[i], shift)], 1u);
+    }
+    __syncthreads();
+    const uint32_t own = mine[d];
+    uint32_t* word = status + (size_t)tile * 256 + d;
+    __hip_atomic_store(word, (tile ? kLookAggregate : kLookPrefix) | own, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // The digit's first position in the output: exclusive scan of the pass's
+    // digit counts over the 256 digits.
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    running[threadIdx.x] = offsets[threadIdx.x * tiles + tile];
-    for (int w = 0; w < kSortWaves; ++w) waveCount[w][threadIdx.x] = 0;
+    const uint32_t c = digitCounts[d];
+    uint32_t x = c;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) waveSums[wave] = x;
+    // Look-back: the counts of this digit in all earlier tiles.
+    uint32_t before = 0;
+    if (tile) {
+        uint32_t pred = tile - 1, spins = 0;
+        for (;;) {
+            const uint32_t w = __hip_atomic_load(status + (size_t)pred * 256 + d, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if ((w & ~kLookCount) == 0) {
+                if (++spins > (1u << 26)) break;  // unreachable by construction; never hang
+                continue;
+            }
+            before += w & kLookCount;
+            if ((w & ~kLookCount) == kLookPrefix || pred == 0) break;
+            --pred;
+        }
+        __hip_atomic_store(word, kLookPrefix | (before + own), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    uint32_t waveOffset = 0;
+    for (int w = 0; w < wave; ++w) waveOffset += waveSums[w];
+    running[d] = waveOffset + x - c + before;
     __syncthreads();
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
-    const uint32_t base = tile * kSortTile;
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
         const bool valid = i < n;
         uint64_t key = valid ? keysIn[i] : 0;
         uint32_t val = valid ? valsIn[i] : 0;
-        uint32_t d = valid ? digitOf(key, shift) : 0;
+        uint32_t dg = valid ? digitOf(key, shift) : 0;
         // Lanes holding the same digit: intersect eight bit-plane ballots.
         uint64_t match = __ballot(valid);
         for (int b = 0; b < 8; ++b) {
-            uint64_t ones = __ballot(valid && ((d >> b) & 1u));
-            match &= ((d >> b) & 1u) ? ones : ~ones;
+            uint64_t ones = __ballot(valid && ((dg >> b) & 1u));
+            match &= ((dg >> b) & 1u) ? ones : ~ones;
         }
         const uint32_t rankInWave = (uint32_t)__popcll(match & below);
-        if (valid && rankInWave == 0) waveCount[wave][d] = (uint32_t)__popcll(match);
+        if (valid && rankInWave == 0) waveCount[wave][dg] = (uint32_t)__popcll(match);
         __syncthreads();
         if (valid) {
-            uint32_t pos = running[d] + rankInWave;
-            for (int w = 0; w < wave; ++w) pos += waveCount[w][d];
+            uint32_t pos = running[dg] + rankInWave;
+            for (int w = 0; w < wave; ++w) pos += waveCount[w][dg];
             keysOut[pos] = key;
             valsOut[pos] = val;
         }
         __syncthreads();
         uint32_t add = 0;
-        for (int w = 0; w < kSortWaves; ++w) { add += waveCount[w][threadIdx.x]; waveCount[w][threadIdx.x] = 0; }
-        running[threadIdx.x] += add;
+        for (int w = 0; w < kSortWaves; ++w) { add += waveCount[w][d]; waveCount[w][d] = 0; }
+        running[d] += add;
         __syncthreads();
     }
+}
+
+__global__ void kSortClear(uint32_t* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ----------------------------------------------------------------- scan --
@@ -381,35 +456,34 @@ namespace vxhost {
 
 using namespace vxk;
 
+// Working memory of a sort: per pass the 256 digit counts, the tile counter
+// and the tiles' look-back words.
 size_t radixTempBytes(uint32_t n) {
-    uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    return (size_t)256 * (tiles ? tiles : 1) * sizeof(uint32_t) * 2 + 64;
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    return (size_t)kMaxPasses * (256 + 1 + 256 * (size_t)(tiles ? tiles : 1)) * sizeof(uint32_t) + 64;
 }
 
 // Sorts (keys, vals) in place by the low `bits` of the key.  `keysTmp`,
-// `valsTmp` hold n elements; `temp` holds radixTempBytes(n) bytes.
+// `valsTmp` hold n elements; `temp` holds radixTempBytes(n) bytes.  With
+// `nDev` the count is read on the device and n is the capacity: the grids
+// are sized for n and the workgroups past the count return at once, so no
+// host round trip is needed to learn the count.
 hipError_t radixSort(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t* valsTmp, uint32_t n, int bits,
-                     void* temp, hipStream_t s) {
-    if (n <= 1 || bits <= 0) return hipSuccess;
+                     void* temp, hipStream_t s, const uint32_t* nDev) {
+    if ((n <= 1 && !nDev) || n == 0 || bits <= 0) return hipSuccess;
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    uint32_t* hist = (uint32_t*)temp;
-    uint32_t* sums = hist + (size_t)256 * tiles;
+    const int passes = std::min((bits + 7) / 8, kMaxPasses);
+    uint32_t* digitCounts = (uint32_t*)temp;                            // passes x 256
+    uint32_t* tileCounter = digitCounts + (size_t)passes * 256;         // passes
+    uint32_t* status = tileCounter + passes;                            // passes x tiles x 256
+    const uint32_t words = (uint32_t)((size_t)passes * (257 + 256 * (size_t)tiles));
+    kSortClear<<<(words + 255) / 256, 256, 0, s>>>(digitCounts, words);
+    kRadixDigitCounts<<<tiles, kSortThreads, 0, s>>>(keys, n, nDev, passes, digitCounts);
     uint64_t* ka = keys; uint32_t* va = vals;
     uint64_t* kb = keysTmp; uint32_t* vb = valsTmp;
-    const uint32_t histN = 256 * tiles;
-    const uint32_t scanTiles = (histN + kScanTile - 1) / kScanTile;
-    int passes = (bits + 7) / 8;
     for (int p = 0; p < passes; ++p) {
-        const int shift = 8 * p;
-        kRadixHistogram<<<tiles, kSortThreads, 0, s>>>(ka, n, shift, hist, tiles);
-        if (histN <= kScanOneMax) {
-            kScanOne<<<1, kScanOneThreads, 0, s>>>(hist, histN, hist, nullptr);
-        } else {
-            kScanTiles<<<scanTiles, kScanThreads, 0, s>>>(hist, histN, hist, sums);
-            kScanSums<<<1, kScanThreads, 0, s>>>(sums, scanTiles, nullptr);
-            kScanAdd<<<scanTiles, kScanThreads, 0, s>>>(hist, histN, sums);
-        }
-        kRadixScatter<<<tiles, kSortThreads, 0, s>>>(ka, va, n, shift, hist, tiles, kb, vb);
+        kRadixPass<<<tiles, kSortThreads, 0, s>>>(ka, va, n, nDev, 8 * p, digitCounts + 256 * p, tileCounter + p,
+                                                  status + (size_t)p * tiles * 256, kb, vb);
         uint64_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
     }

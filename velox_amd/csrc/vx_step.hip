@@ -363,7 +363,10 @@ void collectProfile(vxhip_ctx* c) {
         const int kind = c->launchKind[i / 2];
         c->timings.kernelMs[kind] += ms;
         c->timings.kernelLaunches[kind] += 1;
-        c->timings.kernelVisits[kind] += c->launchContacts[i / 2];
+        // The dataflow's relevant pairs are counted on the device (read with
+        // the step's last counters).
+        c->timings.kernelVisits[kind] +=
+            kind == VXHIP_KERNEL_CCD_DATAFLOW ? c->hostCounters[cCcdRelevant] : c->launchContacts[i / 2];
     }
     c->launchUsed = 0;
     c->launchContacts.clear();
@@ -638,11 +641,10 @@ uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     fill32(c, c->colorCount.p, m + 65, 0);
     kColorEntries<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, cs, m, cb, c->keys.p, c->colorVals.p, k + cEntries);
     HIPCK(hipGetLastError());
-    readCounters(c);
-    const uint32_t e = c->hostCounters[cEntries];
-    HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + cb,
-                            tempFor(c, vxhost::radixTempBytes(e)), s));
-    if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, cb, c->pred.p);
+    // At most two entries per contact; the count stays on the device.
+    HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, 2 * m, nb + cb,
+                            tempFor(c, vxhost::radixTempBytes(2 * m)), s, k + cEntries));
+    kColorPreds<<<blocks(2 * m), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, 2 * m, cb, c->pred.p, k + cEntries);
     const bool batched = c->worldCount > 1 && c->worldN == c->n && c->worldsContiguous;
     if (m <= kColorSingleLimit) {
         kGreedyColor<<<1, kColorThreads, 0, s>>>(c->bodies.p, cs, m, c->pred.p, c->maskAfter.p, c->nextAfter.p,
@@ -1185,30 +1187,27 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     HIPCK(vxhost::exclusiveScan(c->ccdRel.p, c->ccdRelPos.p, np, k + cCcdRelevant, tempFor(c, vxhost::scanTempBytes(np)), s));
     kCcdRelList<<<blocks(np), kThreads, 0, s>>>(c->ccdRel.p, c->ccdRelPos.p, np, c->ccdList.p);
     HIPCK(hipGetLastError());
-    readCounters(c);
-    const uint32_t nr = c->hostCounters[cCcdRelevant];
-    c->keys.reserve(2 * (size_t)nr); c->keysTmp.reserve(2 * (size_t)nr);
-    c->colorVals.reserve(2 * (size_t)nr); c->colorVals2.reserve(2 * (size_t)nr);
+    // The relevant pairs' count stays on the device: every buffer is sized
+    // for all np pairs and the kernels read the count.
+    c->keys.reserve(2 * (size_t)np); c->keysTmp.reserve(2 * (size_t)np);
+    c->colorVals.reserve(2 * (size_t)np); c->colorVals2.reserve(2 * (size_t)np);
     c->pred.reserve(2 * (size_t)np);
     c->ccdDone.reserve(np);
     fill32(c, c->pred.p, 2 * np, kNone);
     fill32(c, c->ccdDone.p, np, 0);
-    kCcdEntries<<<blocks(nr), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdList.p, nr, nb, pbits, c->keys.p,
-                                                c->colorVals.p, k + cEntries);
+    kCcdEntries<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdList.p, k + cCcdRelevant, nb, pbits,
+                                                c->keys.p, c->colorVals.p, k + cEntries);
     HIPCK(hipGetLastError());
-    readCounters(c);
-    const uint32_t e = c->hostCounters[cEntries];
-    HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, e, nb + pbits,
-                            tempFor(c, vxhost::radixTempBytes(e)), s));
-    if (e) kColorPreds<<<blocks(e), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, e, pbits, c->pred.p);
-    const uint32_t grid = std::min<uint32_t>(blocks(nr, kThreads / 64), kCcdBlocks);  // one entry per wavefront first
-    if (nr) {
-        solveLaunchBegin(c);
-        kCcdDataflow<<<grid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p,
-                                               c->ccdList.p, nr, nb, c->ccdFlag.p, c->ccdGap.p, c->pred.p, c->ccdDone.p,
-                                               k + cCcdExamined, k + cCcdRescued, k + cColorError, toiDone);
-        solveLaunchEnd(c, VXHIP_KERNEL_CCD_DATAFLOW, nr);
-    }
+    HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, 2 * np, nb + pbits,
+                            tempFor(c, vxhost::radixTempBytes(2 * np)), s, k + cEntries));
+    kColorPreds<<<blocks(2 * np), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, 2 * np, pbits, c->pred.p, k + cEntries);
+    // A fixed co-resident grid; entry t goes to wavefront t mod (grid x 4).
+    solveLaunchBegin(c);
+    kCcdDataflow<<<kCcdBlocks, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p,
+                                                 c->ccdList.p, k + cCcdRelevant, nb, c->ccdFlag.p, c->ccdGap.p,
+                                                 c->pred.p, c->ccdDone.p, k + cCcdExamined, k + cCcdRescued,
+                                                 k + cColorError, toiDone);
+    solveLaunchEnd(c, VXHIP_KERNEL_CCD_DATAFLOW, 0);
     HIPCK(hipGetLastError());
 }
 
@@ -2663,6 +2662,22 @@ int vxhip_std_sort_noalloc(vxhip_ctx* c, uint64_t* keys, uint32_t* vals, uint32_
         HIPCK(hipStreamSynchronize(s));
         if (c->hostCounters[kNoAllocFlagSlot])
             fail(VXHIP_ERR_INVALID, "vxhip: segment offsets must rise from 0 to n without falling");
+    });
+}
+
+int vxhip_probe_radix_sort(vxhip_ctx* c, uint64_t* keys, uint32_t* vals, uint32_t n, int bits) {
+    return guarded([&] {
+        if (!c || (n && (!keys || !vals)) || bits < 0 || bits > 64) fail(VXHIP_ERR_INVALID, "vxhip: invalid arguments");
+        if (n <= 1) return;
+        hipStream_t s = c->stream;
+        c->exKeys.reserve(n); c->exVals.reserve(n); c->exWorld.reserve(n); c->exIdx.reserve(n);
+        HIPCK(hipMemcpyAsync(c->exKeys.p, keys, (size_t)n * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+        HIPCK(hipMemcpyAsync(c->exVals.p, vals, (size_t)n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        HIPCK(vxhost::radixSort(c->exKeys.p, c->exVals.p, c->exWorld.p, c->exIdx.p, n, bits,
+                                tempFor(c, vxhost::radixTempBytes(n)), s));
+        HIPCK(hipMemcpyAsync(keys, c->exKeys.p, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipMemcpyAsync(vals, c->exVals.p, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        HIPCK(hipStreamSynchronize(s));
     });
 }
 

@@ -163,6 +163,7 @@ SIGNATURES = {
     "vxhip_probe_color": (_i32, [_vp, _vp, _u32, _vp, _u32, _vp, _vp]),
     "vxhip_probe_warm_start": (_i32, [_vp, _vp, _u32, _vp, _u32]),
     "vxhip_probe_std_sort": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
+    "vxhip_probe_radix_sort": (_i32, [_vp, _vp, _vp, _u32, _i32]),
     "vxhip_std_sort_noalloc": (_i32, [_vp, _vp, _vp, _u32, _vp, _u32]),
     "vxhip_world_record": (_i32, [_vp, _i32]),
     "vxhip_world_download_record": (_i32, [_vp, _vp, _vp, _u32, _vp]),
@@ -586,6 +587,13 @@ class Context:
         check(library().vxhip_probe_color(self._h, ptr(bodies), len(bodies), ptr(contacts), len(contacts),
                                           ptr(colors), C.byref(k)))
         return colors[:len(contacts)], k.value
+
+    def radix_sort(self, keys, vals, bits):
+        """(keys, vals) stably sorted by the low `bits` of the key on the device."""
+        keys = np.ascontiguousarray(keys, np.uint64).copy()
+        vals = np.ascontiguousarray(vals, np.uint32).copy()
+        check(library().vxhip_probe_radix_sort(self._h, ptr(keys), ptr(vals), len(keys), int(bits)))
+        return keys, vals
 
     def std_sort(self, keys, vals, seg_start=None):
         """(keys, vals) in the order libstdc++'s std::sort leaves them (per
