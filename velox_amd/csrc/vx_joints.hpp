@@ -8,6 +8,9 @@
 //                     in joint order (one lane; only when a body sleeps)
 //   kJointReset       accumulated impulses cleared each substep (broken joints
 //                     kept after the first substep)
+//   kJointPacks       packs of whole joint-graph components, one workgroup
+//                     each: 8 passes over the pack's colours, a barrier
+//                     between colours
 //   kJointColor       the joints of one colour, one lane each (colours never
 //                     share a dynamic body), 8 passes over the colours
 //   kJointMarkBroken  reaction force / torque over the substep against the
@@ -73,12 +76,9 @@ __global__ __launch_bounds__(kThreads) void kJointReset(vx_joint* joints, uint32
     jointReset(joints[k]);
 }
 
-__global__ __launch_bounds__(kThreads) void kJointColor(vx_body* bodies, vx_joint* joints,
-                                                        const uint32_t* __restrict__ order, uint32_t first,
-                                                        uint32_t count, float h) {
-    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= count) return;
-    vx_joint& j = joints[order[first + k]];
+__device__ __attribute__((always_inline)) inline void solveJointAt(vx_body* bodies, vx_joint* joints, uint32_t idx,
+                                                                   float h) {
+    vx_joint& j = joints[idx];
     if (j.broken) return;
     vx_body a = bodies[j.a], b = bodies[j.b];
     if (asleepFully(a) && asleepFully(b)) return;
@@ -87,6 +87,37 @@ __global__ __launch_bounds__(kThreads) void kJointColor(vx_body* bodies, vx_join
     j = jj;
     bodies[j.a].velocity = a.velocity; bodies[j.a].angularVelocity = a.angularVelocity;
     bodies[j.b].velocity = b.velocity; bodies[j.b].angularVelocity = b.angularVelocity;
+}
+
+__global__ __launch_bounds__(kThreads) void kJointColor(vx_body* bodies, vx_joint* joints,
+                                                        const uint32_t* __restrict__ order, uint32_t first,
+                                                        uint32_t count, float h) {
+    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= count) return;
+    solveJointAt(bodies, joints, order[first + k], h);
+}
+
+// Joint packs: one workgroup per pack runs all kJointIterations passes over
+// the pack's colours, a barrier between colours.  A pack is a union of whole
+// components of the joint graph, so no other workgroup writes its dynamic
+// bodies, and its joints of one colour share none; each joint therefore sees
+// the state the per-colour launches give it.  Static and kinematic endpoints
+// are read by several packs and written back unchanged.
+constexpr uint32_t kJointPack = 256;        // a pack is closed at this many joints
+constexpr uint32_t kJointPackMax = 4096;    // larger packs take the per-colour launches
+constexpr int kJointPackThreads = 256;
+
+__global__ __launch_bounds__(kJointPackThreads) void kJointPacks(vx_body* bodies, vx_joint* joints,
+                                                                 const uint32_t* __restrict__ order,
+                                                                 const uint32_t* __restrict__ runStart,
+                                                                 const uint32_t* __restrict__ packFirstRun, float h) {
+    const uint32_t r0 = packFirstRun[blockIdx.x], r1 = packFirstRun[blockIdx.x + 1];
+    for (int it = 0; it < vxd::kJointIterations; ++it)
+        for (uint32_t r = r0; r < r1; ++r) {
+            const uint32_t b = runStart[r], e = runStart[r + 1];
+            for (uint32_t k = b + threadIdx.x; k < e; k += kJointPackThreads) solveJointAt(bodies, joints, order[k], h);
+            __syncthreads();
+        }
 }
 
 __global__ __launch_bounds__(kThreads) void kJointMarkBroken(vx_joint* joints, uint32_t nj, float h) {

@@ -231,6 +231,12 @@ struct vxhip_ctx {
     std::vector<uint8_t> hostDynamic;
     std::vector<uint32_t> jointColorStartHost;
     Buf<uint32_t> jointOrder, jointKeep, jointKeepPos;
+    // Joint packs: joints whose dynamic bodies connect them (the components of
+    // the joint graph), concatenated into packs of about kJointPack joints,
+    // each solved colour by colour in one workgroup.  Packs above
+    // kJointPackMax joints go through the per-colour launches (jointOrder).
+    uint32_t jointPacks = 0;
+    Buf<uint32_t> packOrder, packRunStart, packFirstRun;
     Buf<uint64_t> jointExcluded;
     Buf<vx_contact> contactsTmp;
     Buf<vxk::JointBreak> jointBreaks;
@@ -1030,12 +1036,93 @@ void prepareJoints(vxhip_ctx* c) {
         colorOf[k] = color;
         if (color + 1 > numColors) numColors = color + 1;
     }
+    // Components of the joint graph over dynamic bodies: joints of different
+    // components share no body they write, so each component may run its
+    // colours apart from the others, and the per-colour order of the
+    // reference holds within it.
+    std::vector<uint32_t> parent(c->n);
+    for (uint32_t i = 0; i < c->n; ++i) parent[i] = i;
+    auto find = [&](uint32_t x) {
+        while (parent[x] != x) x = parent[x] = parent[parent[x]];
+        return x;
+    };
+    for (uint32_t k = 0; k < nj; ++k)
+        if (c->hostDynamic[js[k].a] && c->hostDynamic[js[k].b]) {
+            const uint32_t ra = find(js[k].a), rb = find(js[k].b);
+            if (ra != rb) parent[ra < rb ? rb : ra] = ra < rb ? ra : rb;
+        }
+    // Component of each joint (its first joint's index), components in the
+    // order of their first joint.
+    std::vector<uint32_t> compOf(nj), compFirst(c->n, kNone);
+    std::vector<uint32_t> comps;  // first joint of each component
+    for (uint32_t k = 0; k < nj; ++k) {
+        const uint32_t body = c->hostDynamic[js[k].a] ? js[k].a : js[k].b;
+        if (!c->hostDynamic[body]) { compOf[k] = k; comps.push_back(k); continue; }  // no dynamic body
+        const uint32_t r = find(body);
+        if (compFirst[r] == kNone) { compFirst[r] = k; comps.push_back(k); }
+        compOf[k] = compFirst[r];
+    }
+    std::vector<uint32_t> compSize(nj, 0), compRank(nj, 0);
+    for (uint32_t k = 0; k < nj; ++k) ++compSize[compOf[k]];
+    // Packs: whole components in order, closed once they reach kJointPack.
+    std::vector<uint32_t> packOf(nj);
+    std::vector<uint32_t> packSize;
+    {
+        std::vector<uint32_t> packOfComp(nj, 0);
+        uint32_t cur = 0;
+        for (uint32_t f : comps) {
+            if (packSize.empty() || cur >= kJointPack) { packSize.push_back(0); cur = 0; }
+            packOfComp[f] = (uint32_t)packSize.size() - 1;
+            packSize.back() += compSize[f];
+            cur += compSize[f];
+        }
+        for (uint32_t k = 0; k < nj; ++k) packOf[k] = packOfComp[compOf[k]];
+    }
+    const uint32_t np = (uint32_t)packSize.size();
+    // Pack runs (pack, colour) in joint order; oversize packs' joints go to
+    // the per-colour list.
+    std::vector<std::vector<uint32_t>> runs;  // per (pack, colour): joints
+    std::vector<uint32_t> packRunsHost, packOrderHost, packFirst;
+    std::vector<uint8_t> bigJoint(nj, 0);
+    {
+        std::vector<std::vector<uint32_t>> byPack(np);
+        for (uint32_t k = 0; k < nj; ++k) byPack[packOf[k]].push_back(k);
+        for (uint32_t p = 0; p < np; ++p) {
+            if (packSize[p] > kJointPackMax) {
+                for (uint32_t k : byPack[p]) bigJoint[k] = 1;
+                continue;
+            }
+            int colours = 0;
+            for (uint32_t k : byPack[p]) colours = colorOf[k] + 1 > colours ? colorOf[k] + 1 : colours;
+            std::vector<uint32_t> start(colours + 1, 0);
+            for (uint32_t k : byPack[p]) ++start[colorOf[k] + 1];
+            for (int q = 0; q < colours; ++q) start[q + 1] += start[q];
+            std::vector<uint32_t> fillP(start.begin(), start.end() - 1), ordP(byPack[p].size());
+            for (uint32_t k : byPack[p]) ordP[fillP[colorOf[k]]++] = k;
+            packFirst.push_back((uint32_t)packRunsHost.size());
+            const uint32_t base = (uint32_t)packOrderHost.size();
+            for (int q = 0; q < colours; ++q)
+                if (start[q + 1] > start[q]) packRunsHost.push_back(base + start[q]);
+            packOrderHost.insert(packOrderHost.end(), ordP.begin(), ordP.end());
+        }
+        packFirst.push_back((uint32_t)packRunsHost.size());
+        packRunsHost.push_back((uint32_t)packOrderHost.size());
+    }
+    c->jointPacks = (uint32_t)packFirst.size() - 1;
+    if (c->jointPacks) {
+        upload(c, c->packOrder, packOrderHost.data(), (uint32_t)packOrderHost.size());
+        upload(c, c->packRunStart, packRunsHost.data(), (uint32_t)packRunsHost.size());
+        upload(c, c->packFirstRun, packFirst.data(), (uint32_t)packFirst.size());
+    }
+    // The per-colour list holds the oversize packs' joints only.
     c->jointColors = (uint32_t)numColors;
     c->jointColorStartHost.assign(numColors + 1, 0);
-    for (int col : colorOf) ++c->jointColorStartHost[col + 1];
+    for (uint32_t k = 0; k < nj; ++k)
+        if (bigJoint[k]) ++c->jointColorStartHost[colorOf[k] + 1];
     for (int k = 0; k < numColors; ++k) c->jointColorStartHost[k + 1] += c->jointColorStartHost[k];
     std::vector<uint32_t> fill(c->jointColorStartHost.begin(), c->jointColorStartHost.end() - 1), order(nj);
-    for (uint32_t k = 0; k < nj; ++k) order[fill[colorOf[k]]++] = k;
+    for (uint32_t k = 0; k < nj; ++k)
+        if (bigJoint[k]) order[fill[colorOf[k]]++] = k;
     std::vector<uint64_t> excluded;
     for (const vx_joint& j : js)
         if (!j.collideConnected) {
@@ -1081,7 +1168,11 @@ void substepJoints(vxhip_ctx* c, float h, bool first) {
     if (!nj) return;
     hipStream_t s = c->stream;
     kJointReset<<<blocks(nj), kThreads, 0, s>>>(c->joints.p, nj, first ? 0 : 1);
-    for (int it = 0; it < vxd::kJointIterations; ++it)
+    if (c->jointPacks)
+        kJointPacks<<<c->jointPacks, kJointPackThreads, 0, s>>>(c->bodies.p, c->joints.p, c->packOrder.p,
+                                                                c->packRunStart.p, c->packFirstRun.p, h);
+    if (!c->jointColorStartHost.empty() && c->jointColorStartHost.back())
+        for (int it = 0; it < vxd::kJointIterations; ++it)
         for (uint32_t col = 0; col < c->jointColors; ++col) {
             const uint32_t first0 = c->jointColorStartHost[col], cnt = c->jointColorStartHost[col + 1] - first0;
             if (cnt) kJointColor<<<blocks(cnt), kThreads, 0, s>>>(c->bodies.p, c->joints.p, c->jointOrder.p, first0, cnt, h);

@@ -97,10 +97,20 @@ VXD Q4 qnormalize(Q4 q) {
     return {x * inv, y * inv, z * inv, w * inv};
 }
 
-// math.h:113-122
+// cross() without the canonical zero sign.  Only for an operand of another
+// cross(): cross() reads its operands through products, whose zero results it
+// canonicalises, and comparisons, which see -0 == +0, so its result does not
+// depend on the signs of zero operand components.  A zero component of u x v
+// that is added to v * w only gives a zero of either sign where the sum is
+// zero, so the sum feeds the outer cross() the same operand up to such signs.
+VXD V3 crossRaw(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// math.h:113-122 (the inner product through crossRaw: same result bit for bit)
 VXD V3 rotate(Q4 q, V3 v) {
     V3 u{q.x, q.y, q.z};
-    V3 t = cross(u, cross(u, v) + v * q.w);
+    V3 t = cross(u, crossRaw(u, v) + v * q.w);
     return v + t * 2.0f;
 }
 VXD V3 rotateInv(Q4 q, V3 v) { return rotate(Q4{-q.x, -q.y, -q.z, q.w}, v); }
