@@ -219,49 +219,50 @@ constexpr int kCcdThreads = 1024;
 // Pairs that can see a rewound body.  A rescue only happens at a flagged
 // pair or at a pair one of whose dynamic bodies an earlier rescue moved, so
 // every pair that can do anything is flagged or follows (in pair order) such
-// a pair on a shared dynamic body.  The closure is built by alternating two
-// passes until nothing changes: each relevant pair lowers firstRel of its
-// dynamic bodies, and a pair behind a body's firstRel becomes relevant.
-// Every other pair sees exactly the state the parallel check saw.
-__global__ __launch_bounds__(kThreads) void kCcdFlagsToRel(const uint8_t* __restrict__ flag, uint32_t np,
-                                                           uint32_t* __restrict__ rel) {
+// a pair on a shared dynamic body.  Each relevant pair lowers firstRel of its
+// dynamic bodies to its index, and a pair behind a body's firstRel becomes
+// relevant, until nothing changes.  A pair that becomes relevant lowers its
+// bodies' firstRel at once, within the same round: firstRel only ever holds
+// indices of relevant pairs, so any pair it puts behind one is in the
+// closure, and a round that adds nothing has seen every earlier round's
+// marks.  Every other pair sees exactly the state the parallel check saw.
+__global__ __launch_bounds__(kThreads) void kCcdFlagsToRel(const vx_body* __restrict__ bodies,
+                                                           const uint64_t* __restrict__ pairs,
+                                                           const uint8_t* __restrict__ flag, uint32_t np, int nb,
+                                                           uint32_t* __restrict__ rel, uint32_t* __restrict__ firstRel) {
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
-    if (p < np) rel[p] = flag[p] ? 1u : 0u;
-}
-
-constexpr int kCcdRounds = 8;
-
-// `prevAdded`: pairs the previous round added (null in the first round); a
-// round after one that added none has nothing to do.
-__global__ __launch_bounds__(kThreads) void kCcdRelMark(const vx_body* __restrict__ bodies,
-                                                        const uint64_t* __restrict__ pairs,
-                                                        const uint32_t* __restrict__ rel, uint32_t np, int nb,
-                                                        uint32_t* __restrict__ firstRel,
-                                                        const uint32_t* __restrict__ prevAdded) {
-    if (prevAdded && *prevAdded == 0) return;
-    uint32_t p = blockIdx.x * kThreads + threadIdx.x;
-    if (p >= np || !rel[p]) return;
+    if (p >= np) return;
+    rel[p] = flag[p] ? 1u : 0u;
+    if (!flag[p]) return;
     uint32_t i, j;
     ccdBodies(bodies, pairs[p], nb, i, j);
     if (isDynamic(bodies[i])) atomicMin(&firstRel[i], p);
     if (isDynamic(bodies[j])) atomicMin(&firstRel[j], p);
 }
 
-__global__ __launch_bounds__(kThreads) void kCcdRelGrow(const vx_body* __restrict__ bodies,
-                                                        const uint64_t* __restrict__ pairs,
-                                                        const uint32_t* __restrict__ counts, uint32_t np, int nb,
-                                                        const uint32_t* __restrict__ firstRel, uint32_t* __restrict__ rel,
-                                                        uint32_t* __restrict__ changed,
-                                                        const uint32_t* __restrict__ prevAdded) {
+constexpr int kCcdRounds = 8;
+
+// One closure round.  `prevAdded`: pairs the previous round added (null in
+// the first round); a round after one that added none has nothing to do.
+__global__ __launch_bounds__(kThreads) void kCcdRelRound(const vx_body* __restrict__ bodies,
+                                                         const uint64_t* __restrict__ pairs,
+                                                         const uint32_t* __restrict__ counts, uint32_t np, int nb,
+                                                         uint32_t* firstRel, uint32_t* __restrict__ rel,
+                                                         uint32_t* __restrict__ changed,
+                                                         const uint32_t* __restrict__ prevAdded) {
     if (prevAdded && *prevAdded == 0) return;
     uint32_t p = blockIdx.x * kThreads + threadIdx.x;
     if (p >= np || rel[p] || !counts[p]) return;
     uint32_t i, j;
     ccdBodies(bodies, pairs[p], nb, i, j);
-    const bool behind = (isDynamic(bodies[i]) && firstRel[i] < p) || (isDynamic(bodies[j]) && firstRel[j] < p);
+    const bool di = isDynamic(bodies[i]), dj = isDynamic(bodies[j]);
+    const bool behind = (di && __hip_atomic_load(&firstRel[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < p) ||
+                        (dj && __hip_atomic_load(&firstRel[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < p);
     if (behind) {
         rel[p] = 1u;
         atomicAdd(changed, 1u);
+        if (di) atomicMin(&firstRel[i], p);
+        if (dj) atomicMin(&firstRel[j], p);
     }
 }
 

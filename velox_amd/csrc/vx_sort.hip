@@ -98,20 +98,34 @@ __global__ __launch_bounds__(kSortThreads) void kRadixPass(const uint64_t* __res
         if (lane >= off) x += y;
     }
     if (lane == 63) waveSums[wave] = x;
-    // Look-back: the counts of this digit in all earlier tiles.
+    // Look-back: the counts of this digit in all earlier tiles.  The words of
+    // kLookBatch predecessors are loaded together and consumed in tile order
+    // (down to the first prefix, or to the first one not yet published, which
+    // the next round reloads), so a walk over t tiles waits on about
+    // t / kLookBatch round trips instead of t.
     uint32_t before = 0;
     if (tile) {
+        constexpr uint32_t kLookBatch = 8;
         uint32_t pred = tile - 1, spins = 0;
         for (;;) {
-            const uint32_t w = __hip_atomic_load(status + (size_t)pred * 256 + d, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            if ((w & ~kLookCount) == 0) {
-                if (++spins > (1u << 26)) break;  // unreachable by construction; never hang
-                continue;
+            uint32_t w[kLookBatch];
+#pragma unroll
+            for (uint32_t q = 0; q < kLookBatch; ++q)
+                w[q] = q <= pred ? __hip_atomic_load(status + (size_t)(pred - q) * 256 + d, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)
+                                 : 0u;
+            uint32_t used = 0;
+            bool done = false;
+#pragma unroll
+            for (uint32_t q = 0; q < kLookBatch; ++q) {
+                if (done || used < q || q > pred || (w[q] & ~kLookCount) == 0) continue;
+                before += w[q] & kLookCount;
+                used = q + 1;
+                done = (w[q] & ~kLookCount) == kLookPrefix || pred == q;
             }
-            before += w & kLookCount;
-            if ((w & ~kLookCount) == kLookPrefix || pred == 0) break;
-            --pred;
+            if (done) break;
+            if (used == 0 && ++spins > (1u << 26)) break;  // unreachable by construction; never hang
+            pred -= used;
         }
         __hip_atomic_store(word, kLookPrefix | (before + own), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }

@@ -1260,18 +1260,17 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     // Pairs that can see a rewound body (flagged pairs and what follows them
     // on shared dynamic bodies).
     c->ccdRel.reserve(np); c->ccdRelPos.reserve(np); c->ccdList.reserve(np); c->ccdFirstRel.reserve(n);
-    kCcdFlagsToRel<<<blocks(np), kThreads, 0, s>>>(c->ccdFlag.p, np, c->ccdRel.p);
     fill32(c, c->ccdFirstRel.p, n, kNone);
+    kCcdFlagsToRel<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdFlag.p, np, nb, c->ccdRel.p,
+                                                   c->ccdFirstRel.p);
     // Eight closure rounds queued at once: a round after one that added
     // nothing returns at once, and the all-pairs fallback runs only when the
     // eighth still added pairs (no host round trip per round).
     fill32(c, k + cCcdRound, kCcdRounds, 0);
     for (int round = 0; round < kCcdRounds; ++round) {
         const uint32_t* prev = round ? k + cCcdRound + round - 1 : nullptr;
-        kCcdRelMark<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdRel.p, np, nb, c->ccdFirstRel.p,
-                                                    prev);
-        kCcdRelGrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, c->ccdFirstRel.p,
-                                                    c->ccdRel.p, k + cCcdRound + round, prev);
+        kCcdRelRound<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, c->ccdFirstRel.p,
+                                                     c->ccdRel.p, k + cCcdRound + round, prev);
     }
     kCcdRelAll<<<blocks(np), kThreads, 0, s>>>(c->counts.p, np, c->ccdRel.p, k + cCcdRound + kCcdRounds - 1);
     HIPCK(hipGetLastError());
