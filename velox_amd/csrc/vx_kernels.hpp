@@ -379,7 +379,11 @@ __global__ __launch_bounds__(kThreads) void kPairClass(const vx_body* __restrict
     const uint64_t key = pairs[p];
     const uint32_t ca = (uint32_t)(key >> nb), cb = (uint32_t)(key & ((1ull << nb) - 1));
     const uint32_t i = ca < cb ? ca : cb, j = ca < cb ? cb : ca;
-    keys[p] = (uint64_t)(bodies[i].shape & 15u) << 4 | (bodies[j].shape & 15u);
+    // Classes in descending shape order (compounds, hulls, meshes, capsules,
+    // boxes, planes, spheres), so the longest wavefronts are dispatched first
+    // and the ones that wait for a free SIMD are the short ones.  Each pair's results land at its
+    // own index, so the order is a schedule only.
+    keys[p] = 255u - ((bodies[i].shape & 15u) << 4 | (bodies[j].shape & 15u));
     vals[p] = p;
 }
 constexpr int kPairClassBits = 8;

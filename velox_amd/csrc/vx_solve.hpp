@@ -609,8 +609,10 @@ __global__ void kRunSentinels(uint32_t* runStart, const uint32_t* runs, uint32_t
 // solveContact (narrowphase.h:183-365).  b -= x is computed as b + (-x),
 // which IEEE defines to be the same operation.
 
+// Every lane reads a lane of its own quad, so no lane keeps its old value:
+// the move takes no initial value (no register to clear first).
 __device__ inline float swapLane(float x) {  // lanes 2i <-> 2i+1
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0xB1, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, true));
 }
 __device__ inline V3 swapLane(V3 v) { return {swapLane(v.x), swapLane(v.y), swapLane(v.z)}; }
 
