@@ -867,8 +867,14 @@ hipStream_t sideStream(vxhip_ctx* c) {
 template <int kMode>
 void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
     hipStream_t s = c->stream;
-    const bool fork = c->sideOn && c->batchCount && c->bigCount && !c->profile;
+    // Independent groups go side by side: the island batches on the side
+    // stream, and on the library's stream the big-island chain or, without
+    // one, the medium islands (they share no body with the batches).
+    const uint32_t globalIslands = c->batchCount - c->ldsBatchCount;
+    const bool fork = c->sideOn && c->batchCount && !c->profile &&
+                      (c->bigCount || (c->ldsBatchCount && globalIslands));
     const hipStream_t bs = fork ? sideStream(c) : s;
+    const hipStream_t is = fork && !c->bigCount ? s : bs;  // medium islands
     if (fork) {
         HIPCK(hipEventRecord(c->forkEv, s));
         HIPCK(hipStreamWaitEvent(bs, c->forkEv, 0));
@@ -906,7 +912,7 @@ void solvePass(vxhip_ctx* c, SolveArrays sa, uint32_t K, int sweeps) {
         if (global) {
             solveLaunchBegin(c);
             ++c->stats.launches[VXHIP_LAUNCH_ISLAND];
-            kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, bs>>>(
+            kIslandSweeps<kMode, kThreads><<<std::min<uint32_t>(global, 4096), kThreads, 0, is>>>(
                 sa, c->runStart.p, c->islandFirstRun.p, c->counters.p + cRanges + 2, sweeps);
             solveLaunchEnd(c, VXHIP_KERNEL_ISLAND, (uint64_t)(c->smallCount - c->ldsContactCount) * sweeps);
         }
