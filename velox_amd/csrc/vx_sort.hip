@@ -16,7 +16,7 @@ namespace vxk {
 
 constexpr int kSortThreads = 256;          // 4 wavefronts
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = 16;             // rounds per tile
+constexpr int kSortItems = 8;              // rounds per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
 
 __device__ inline uint32_t digitOf(uint64_t key, int shift) { return (uint32_t)(key >> shift) & 0xffu; }
@@ -46,12 +46,18 @@ __global__ __launch_bounds__(kSortThreads) void kRadixDigitCounts(const uint64_t
     for (int p = 0; p < kMaxPasses; ++p) counts[p][threadIdx.x] = 0;
     __syncthreads();
     const uint32_t base = blockIdx.x * kSortTile;
+    // The tile's keys are loaded together (one memory round trip), then counted.
+    uint64_t kr[kSortItems];
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) {
-            const uint64_t k = keys[i];
-            for (int p = 0; p < passes; ++p) atomicAdd(&counts[p][digitOf(k, 8 * p)], 1u);
-        }
+        kr[r] = i < n ? keys[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n)
+            for (int p = 0; p < passes; ++p) atomicAdd(&counts[p][digitOf(kr[r], 8 * p)], 1u);
     }
     __syncthreads();
     for (int p = 0; p < passes; ++p)
@@ -79,9 +85,20 @@ __global__ __launch_bounds__(kSortThreads) void kRadixPass(const uint64_t* __res
     const uint32_t tile = tileS;
     const uint32_t base = tile * kSortTile;
     if (base >= n) return;  // past the last tile: no tile waits on it
+    // The tile's keys and values stay in registers from one load (issued
+    // together) to the scatter.
+    uint64_t kr[kSortItems];
+    uint32_t vr[kSortItems];
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n) atomicAdd(&mine[digitOf(keysIn[i], shift)], 1u);
+        kr[r] = i < n ? keysIn[i] : 0;
+        vr[r] = i < n ? valsIn[i] : 0;
+    }
+#pragma unroll
+    for (int r = 0; r < kSortItems; ++r) {
+        const uint32_t i = base + r * kSortThreads + threadIdx.x;
+        if (i < n) atomicAdd(&mine[digitOf(kr[r], shift)], 1u);
     }
     __syncthreads();
     const uint32_t own = mine[d];
@@ -135,11 +152,12 @@ __global__ __launch_bounds__(kSortThreads) void kRadixPass(const uint64_t* __res
     running[d] = waveOffset + x - c + before;
     __syncthreads();
     const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+#pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
         const bool valid = i < n;
-        uint64_t key = valid ? keysIn[i] : 0;
-        uint32_t val = valid ? valsIn[i] : 0;
+        const uint64_t key = kr[r];
+        const uint32_t val = vr[r];
         uint32_t dg = valid ? digitOf(key, shift) : 0;
         // Lanes holding the same digit: intersect eight bit-plane ballots.
         uint64_t match = __ballot(valid);
