@@ -16,7 +16,7 @@ namespace vxk {
 
 constexpr int kSortThreads = 256;          // 4 wavefronts
 constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = 8;              // rounds per tile
+constexpr int kSortItems = 4;              // rounds per tile
 constexpr int kSortTile = kSortThreads * kSortItems;
 
 __device__ inline uint32_t digitOf(uint64_t key, int shift) { return (uint32_t)(key >> shift) & 0xffu; }
