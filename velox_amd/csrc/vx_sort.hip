@@ -37,9 +37,15 @@ constexpr int kMaxPasses = 8;
 
 // `nDev`: the element count on the device (then `n` is only the capacity the
 // grid was sized for).
-__global__ __launch_bounds__(kSortThreads) void kRadixDigitCounts(const uint64_t* __restrict__ keys, uint32_t n,
+// With `keysCopy` it also copies (keys, vals) there: a sort of an odd number
+// of passes then runs from the copy, so its last pass writes the input
+// buffers and needs no copy back.
+__global__ __launch_bounds__(kSortThreads) void kRadixDigitCounts(const uint64_t* __restrict__ keys,
+                                                                  const uint32_t* __restrict__ vals, uint32_t n,
                                                                   const uint32_t* __restrict__ nDev, int passes,
-                                                                  uint32_t* __restrict__ digitCounts) {
+                                                                  uint32_t* __restrict__ digitCounts,
+                                                                  uint64_t* __restrict__ keysCopy,
+                                                                  uint32_t* __restrict__ valsCopy) {
     __shared__ uint32_t counts[kMaxPasses][256];
     if (nDev) n = *nDev;
     if (blockIdx.x * kSortTile >= n) return;
@@ -56,8 +62,13 @@ __global__ __launch_bounds__(kSortThreads) void kRadixDigitCounts(const uint64_t
 #pragma unroll
     for (int r = 0; r < kSortItems; ++r) {
         const uint32_t i = base + r * kSortThreads + threadIdx.x;
-        if (i < n)
+        if (i < n) {
             for (int p = 0; p < passes; ++p) atomicAdd(&counts[p][digitOf(kr[r], 8 * p)], 1u);
+            if (keysCopy) {
+                keysCopy[i] = kr[r];
+                valsCopy[i] = vals[i];
+            }
+        }
     }
     __syncthreads();
     for (int p = 0; p < passes; ++p)
@@ -510,18 +521,18 @@ hipError_t radixSort(uint64_t* keys, uint32_t* vals, uint64_t* keysTmp, uint32_t
     uint32_t* status = tileCounter + passes;                            // passes x tiles x 256
     const uint32_t words = (uint32_t)((size_t)passes * (257 + 256 * (size_t)tiles));
     kSortClear<<<(words + 255) / 256, 256, 0, s>>>(digitCounts, words);
-    kRadixDigitCounts<<<tiles, kSortThreads, 0, s>>>(keys, n, nDev, passes, digitCounts);
-    uint64_t* ka = keys; uint32_t* va = vals;
-    uint64_t* kb = keysTmp; uint32_t* vb = valsTmp;
+    // An odd pass count starts from a copy in the second buffer (made by the
+    // counting launch), so the passes end in (keys, vals).
+    const bool odd = passes & 1;
+    kRadixDigitCounts<<<tiles, kSortThreads, 0, s>>>(keys, vals, n, nDev, passes, digitCounts,
+                                                     odd ? keysTmp : nullptr, odd ? valsTmp : nullptr);
+    uint64_t* ka = odd ? keysTmp : keys; uint32_t* va = odd ? valsTmp : vals;
+    uint64_t* kb = odd ? keys : keysTmp; uint32_t* vb = odd ? vals : valsTmp;
     for (int p = 0; p < passes; ++p) {
         kRadixPass<<<tiles, kSortThreads, 0, s>>>(ka, va, n, nDev, 8 * p, digitCounts + 256 * p, tileCounter + p,
                                                   status + (size_t)p * tiles * 256, kb, vb);
         uint64_t* tk = ka; ka = kb; kb = tk;
         uint32_t* tv = va; va = vb; vb = tv;
-    }
-    if (ka != keys) {
-        (void)hipMemcpyAsync(keys, ka, (size_t)n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
-        (void)hipMemcpyAsync(vals, va, (size_t)n * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     }
     return hipGetLastError();
 }
