@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -413,6 +414,35 @@ void fill32(vxhip_ctx* c, uint32_t* p, uint32_t n, uint32_t v) {
     if (n) kFill32<<<blocks(n), kThreads, 0, c->stream>>>(p, n, v);
 }
 
+// Up to kFillMax buffers filled by one launch (each fill is otherwise a
+// launch of its own, and the step's fills come in runs).
+constexpr int kFillMax = 6;
+struct FillSet {
+    uint32_t* p[kFillMax];
+    uint32_t n[kFillMax];
+    uint32_t v[kFillMax];
+    int count;
+};
+__global__ void kFillMany(FillSet f) {
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+#pragma unroll
+    for (int q = 0; q < kFillMax; ++q)
+        if (q < f.count && i < f.n[q]) f.p[q][i] = f.v[q];
+}
+struct Fill { uint32_t* p; uint32_t n; uint32_t v; };
+void fillMany(vxhip_ctx* c, std::initializer_list<Fill> fills) {
+    FillSet f{};
+    uint32_t most = 0;
+    for (const Fill& x : fills) {
+        if (!x.n) continue;
+        if (f.count == kFillMax) fail(VXHIP_ERR_INVALID, "vxhip: too many fills in one launch");
+        f.p[f.count] = x.p; f.n[f.count] = x.n; f.v[f.count] = x.v;
+        ++f.count;
+        most = std::max(most, x.n);
+    }
+    if (f.count) kFillMany<<<blocks(most), kThreads, 0, c->stream>>>(f);
+}
+
 template <class T>
 void upload(vxhip_ctx* c, Buf<T>& b, const T* host, uint32_t count) {
     b.reserve(count ? count : 1);
@@ -642,9 +672,7 @@ uint32_t colorContacts(vxhip_ctx* c, const vx_contact* cs, uint32_t m) {
     c->maskAfter.reserve(2 * (size_t)m);
     c->nextAfter.reserve(2 * (size_t)m);
     kResetCounters<<<1, 128, 0, s>>>(k);
-    fill32(c, c->pred.p, 2 * m, kNone);
-    fill32(c, c->color.p, m, kNone);
-    fill32(c, c->colorCount.p, m + 65, 0);
+    fillMany(c, {{c->pred.p, 2 * m, kNone}, {c->color.p, m, kNone}, {c->colorCount.p, m + 65, 0}});
     kColorEntries<<<blocks(m), kThreads, 0, s>>>(c->bodies.p, cs, m, cb, c->keys.p, c->colorVals.p, k + cEntries);
     HIPCK(hipGetLastError());
     // At most two entries per contact; the count stays on the device.
@@ -723,11 +751,8 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
     c->runStart.reserve(m + 1); c->islandFirstRun.reserve(m + 1); c->slotPair.reserve(m); c->batchBodies.reserve(m);
     kResetCounters<<<1, 128, 0, s>>>(k);
     kIota<<<blocks(n), kThreads, 0, s>>>(c->uf.p, n);
-    fill32(c, c->islandSize.p, n, 0);
-    fill32(c, c->bodyCount.p, n, 0);
-    fill32(c, c->rangeStart.p, globalBase, kNone);
-    fill32(c, c->rangeEnd.p, globalBase, 0);
-    fill32(c, c->bigColorCount.p, K + 1, 0);
+    fillMany(c, {{c->islandSize.p, n, 0}, {c->bodyCount.p, n, 0}, {c->rangeStart.p, globalBase, kNone},
+                 {c->rangeEnd.p, globalBase, 0}, {c->bigColorCount.p, K + 1, 0}});
     kColorOfSolve<<<blocks(m), kThreads, 0, s>>>(c->color.p, c->order.p, m, c->colorSolve.p);
     kIslandUnion<<<blocks(m), kThreads, 0, s>>>(meta, m, c->uf.p);
     kIslandOf<<<blocks(m), kThreads, 0, s>>>(meta, m, c->uf.p, c->islandOf.p, c->islandSize.p);
@@ -1266,13 +1291,12 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     // Pairs that can see a rewound body (flagged pairs and what follows them
     // on shared dynamic bodies).
     c->ccdRel.reserve(np); c->ccdRelPos.reserve(np); c->ccdList.reserve(np); c->ccdFirstRel.reserve(n);
-    fill32(c, c->ccdFirstRel.p, n, kNone);
-    kCcdFlagsToRel<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdFlag.p, np, nb, c->ccdRel.p,
-                                                   c->ccdFirstRel.p);
     // Eight closure rounds queued at once: a round after one that added
     // nothing returns at once, and the all-pairs fallback runs only when the
     // eighth still added pairs (no host round trip per round).
-    fill32(c, k + cCcdRound, kCcdRounds, 0);
+    fillMany(c, {{c->ccdFirstRel.p, n, kNone}, {k + cCcdRound, kCcdRounds, 0}});
+    kCcdFlagsToRel<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdFlag.p, np, nb, c->ccdRel.p,
+                                                   c->ccdFirstRel.p);
     for (int round = 0; round < kCcdRounds; ++round) {
         const uint32_t* prev = round ? k + cCcdRound + round - 1 : nullptr;
         kCcdRelRound<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->counts.p, np, nb, c->ccdFirstRel.p,
@@ -1289,8 +1313,7 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     c->colorVals.reserve(2 * (size_t)np); c->colorVals2.reserve(2 * (size_t)np);
     c->pred.reserve(2 * (size_t)np);
     c->ccdDone.reserve(np);
-    fill32(c, c->pred.p, 2 * np, kNone);
-    fill32(c, c->ccdDone.p, np, 0);
+    fillMany(c, {{c->pred.p, 2 * np, kNone}, {c->ccdDone.p, np, 0}});
     kCcdEntries<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->pairs.p, c->ccdList.p, k + cCcdRelevant, nb, pbits,
                                                 c->keys.p, c->colorVals.p, k + cEntries);
     HIPCK(hipGetLastError());
