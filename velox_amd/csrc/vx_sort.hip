@@ -7,7 +7,8 @@
 //    ballots (a 64-lane match mask) and chains waves through LDS.  Stability
 //    per pass is what makes the multi-pass LSD sort correct.
 //  * Exclusive scan of u32 counts: one workgroup up to 16384 counts, else
-//    tile sums, scan of tile sums, add-back.
+//    tile scans and an add-back whose workgroups sum the earlier tiles'
+//    totals themselves.
 #include <algorithm>
 #include "vx_internal.hpp"
 #include "vx_stdsort.hpp"
@@ -240,30 +241,29 @@ __global__ __launch_bounds__(kScanThreads) void kScanTiles(const uint32_t* __res
     if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
 }
 
-// Single-block scan of the tile sums (any count) and total.
-__global__ __launch_bounds__(kScanThreads) void kScanSums(uint32_t* __restrict__ sums, uint32_t count,
-                                                          uint32_t* __restrict__ totalOut) {
-    __shared__ uint32_t smem[kScanThreads];
-    uint32_t carry = 0;
-    for (uint32_t start = 0; start < count; start += kScanThreads) {
-        uint32_t i = start + threadIdx.x;
-        uint32_t v = i < count ? sums[i] : 0u;
-        uint32_t total;
-        uint32_t ex = blockExclusiveScan(v, smem, total);
-        if (i < count) sums[i] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0 && totalOut) *totalOut = carry;
-}
-
+// Add-back with each tile's offset summed from the raw tile sums by the
+// tile's own workgroup (tile sums of earlier tiles, reduced in LDS), so the
+// scan of the tile sums needs no launch of its own; the last tile writes the
+// total.
 __global__ __launch_bounds__(kScanThreads) void kScanAdd(uint32_t* __restrict__ out, uint32_t n,
-                                                         const uint32_t* __restrict__ tileSums) {
+                                                         const uint32_t* __restrict__ tileSums, uint32_t tiles,
+                                                         uint32_t* __restrict__ total) {
+    __shared__ uint32_t red[kScanThreads];
+    uint32_t acc = 0;
+    for (uint32_t t = threadIdx.x; t < blockIdx.x; t += kScanThreads) acc += tileSums[t];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = kScanThreads / 2; off > 0; off >>= 1) {
+        if (threadIdx.x < (uint32_t)off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    const uint32_t add = red[0];
     const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    const uint32_t add = tileSums[blockIdx.x];
     for (int k = 0; k < kScanItems; ++k) {
         uint32_t i = base + k;
         if (i < n) out[i] += add;
     }
+    if (total && threadIdx.x == 0 && blockIdx.x == tiles - 1) *total = add + tileSums[tiles - 1];
 }
 
 // Single-workgroup exclusive scan of up to kScanOneMax counts in one launch
@@ -552,8 +552,7 @@ hipError_t exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t
     const uint32_t tiles = (n + kScanTile - 1) / kScanTile;
     uint32_t* sums = (uint32_t*)temp;
     kScanTiles<<<tiles, kScanThreads, 0, s>>>(in, n, out, sums);
-    kScanSums<<<1, kScanThreads, 0, s>>>(sums, tiles, total);
-    kScanAdd<<<tiles, kScanThreads, 0, s>>>(out, n, sums);
+    kScanAdd<<<tiles, kScanThreads, 0, s>>>(out, n, sums, tiles, total);
     return hipGetLastError();
 }
 
