@@ -96,7 +96,7 @@ __device__ inline float ccdSearch(const vx_body& a, float dt) {
 
 // Flags pairs (that produced contacts) whose end state penetrates deeper than
 // 8 mm and records the first flagged index.
-__global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict__ bodies, Soup soup, float dt,
+__global__ __launch_bounds__(kCollideThreads) void kCcdCheck(const vx_body* __restrict__ bodies, Soup soup, float dt,
                                                       const uint64_t* __restrict__ pairs,
                                                       const uint32_t* __restrict__ counts, uint32_t np, int nb,
                                                       const uint32_t* __restrict__ order,
@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void kCcdCheck(const vx_body* __restrict_
                                                       uint32_t* __restrict__ flagged,
                                                       uint32_t* __restrict__ firstFlagged,
                                                       const uint8_t* __restrict__ toiDone) {
-    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t q = blockIdx.x * kCollideThreads + threadIdx.x;
     if (q >= np) return;
     const uint32_t p = order[q];  // pairs grouped by shape class
     uint8_t f = 0;

@@ -388,11 +388,17 @@ __global__ __launch_bounds__(kThreads) void kPairClass(const vx_body* __restrict
 }
 constexpr int kPairClassBits = 8;
 
-__global__ __launch_bounds__(kThreads) void kNarrow(const vx_body* __restrict__ bodies, Soup soup, float dt,
+// The collision kernels run one wavefront per workgroup: at one wavefront
+// per SIMD, a workgroup of several would hold its CU's slots until its
+// slowest wavefront ends, and the second round of wavefronts could not start
+// on a SIMD as soon as it frees.
+constexpr int kCollideThreads = 64;
+
+__global__ __launch_bounds__(kCollideThreads) void kNarrow(const vx_body* __restrict__ bodies, Soup soup, float dt,
                                                     const uint64_t* __restrict__ pairs, uint32_t np, int nb,
                                                     const uint32_t* __restrict__ order,
                                                     vx_contact* __restrict__ slots, uint32_t* __restrict__ counts) {
-    const uint32_t q = blockIdx.x * kThreads + threadIdx.x;
+    const uint32_t q = blockIdx.x * kCollideThreads + threadIdx.x;
     if (q >= np) return;
     const uint32_t p = order[q];
     uint64_t key = pairs[p];

@@ -637,7 +637,7 @@ uint32_t narrowPhase(vxhip_ctx* c, uint32_t np, float dt) {
         HIPCK(vxhost::radixSort(c->pairClass.p, c->pairOrder.p, c->pairClassTmp.p, c->pairOrderTmp.p, np,
                                 kPairClassBits, tempFor(c, vxhost::radixTempBytes(np)), s));
         solveLaunchBegin(c);
-        kNarrow<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, np, nb, c->pairOrder.p,
+        kNarrow<<<blocks(np, kCollideThreads), kCollideThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, np, nb, c->pairOrder.p,
                                                 c->slots.p, c->counts.p);
         solveLaunchEnd(c, VXHIP_KERNEL_NARROW, np);
         HIPCK(hipGetLastError());
@@ -1281,7 +1281,7 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     c->ccdGap.reserve(np);
     const uint8_t* toiDone = c->toiProcessed ? c->toiEligible.p : nullptr;
     solveLaunchBegin(c);
-    kCcdCheck<<<blocks(np), kThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb,
+    kCcdCheck<<<blocks(np, kCollideThreads), kCollideThreads, 0, s>>>(c->bodies.p, c->soup, dt, c->pairs.p, c->counts.p, np, nb,
                                               c->pairOrder.p, c->ccdFlag.p,
                                               c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst, toiDone);
     solveLaunchEnd(c, VXHIP_KERNEL_CCD_CHECK, np);
