@@ -113,21 +113,40 @@ __device__ inline void cellOf(float4 lo, float4 hi, float inv, int& cx, int& cy,
     cz = (int)floorf((lo.z + hi.z) * 0.5f * inv);
 }
 
+// Reduced over the wavefront first: one set of atomics per wavefront (the
+// bounds are six words every body would otherwise hit).
 __global__ __launch_bounds__(kThreads) void kCellBounds(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
                                                         const float4* __restrict__ hi, uint32_t n,
                                                         const uint32_t* __restrict__ extentBits, int* __restrict__ bounds,
                                                         uint32_t* __restrict__ oversizeCount) {
-    uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n) return;
-    const float cell = gridCellOf(extentBits);
-    float4 l = lo[i], h = hi[i];
-    float ext = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
-    const uint8_t shape = bodies[i].shape;
-    if (ext > cell || shape == VX_SHAPE_PLANE || shape == VX_SHAPE_MESH) { atomicAdd(oversizeCount, 1u); return; }
-    int cx, cy, cz;
-    cellOf(l, h, 1.0f / cell, cx, cy, cz);
-    atomicMin(&bounds[0], cx); atomicMin(&bounds[1], cy); atomicMin(&bounds[2], cz);
-    atomicMax(&bounds[3], cx); atomicMax(&bounds[4], cy); atomicMax(&bounds[5], cz);
+    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+    bool over = false, in = false;
+    int cx = 0x7fffffff, cy = 0x7fffffff, cz = 0x7fffffff, mx = (int)0x80000000, my = (int)0x80000000,
+        mz = (int)0x80000000;
+    if (i < n) {
+        const float cell = gridCellOf(extentBits);
+        float4 l = lo[i], h = hi[i];
+        float ext = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
+        const uint8_t shape = bodies[i].shape;
+        over = ext > cell || shape == VX_SHAPE_PLANE || shape == VX_SHAPE_MESH;
+        if (!over) {
+            cellOf(l, h, 1.0f / cell, cx, cy, cz);
+            mx = cx; my = cy; mz = cz;
+            in = true;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        cx = min(cx, __shfl_xor(cx, off)); cy = min(cy, __shfl_xor(cy, off)); cz = min(cz, __shfl_xor(cz, off));
+        mx = max(mx, __shfl_xor(mx, off)); my = max(my, __shfl_xor(my, off)); mz = max(mz, __shfl_xor(mz, off));
+    }
+    const uint64_t overMask = __ballot(over), inMask = __ballot(in);
+    if ((threadIdx.x & 63) == 0) {
+        if (overMask) atomicAdd(oversizeCount, (uint32_t)__popcll(overMask));
+        if (inMask) {
+            atomicMin(&bounds[0], cx); atomicMin(&bounds[1], cy); atomicMin(&bounds[2], cz);
+            atomicMax(&bounds[3], mx); atomicMax(&bounds[4], my); atomicMax(&bounds[5], mz);
+        }
+    }
 }
 
 // Batched worlds share one grid: a body's key is world * ncells + cell, so

@@ -386,13 +386,25 @@ __global__ __launch_bounds__(kThreads) void kIslandUnion(const uint4* __restrict
 __global__ __launch_bounds__(kThreads) void kIslandOf(const uint4* __restrict__ meta, uint32_t m, uint32_t* parent,
                                                       uint32_t* __restrict__ islandOf, uint32_t* __restrict__ size) {
     uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= m) return;
-    const uint4 mt = meta[k];
     uint32_t root = kNone;
-    if (mt.z & kDynA) root = ufFind(parent, mt.x);
-    else if (mt.z & kDynB) root = ufFind(parent, mt.y);
-    islandOf[k] = root;
-    if (root != kNone) atomicAdd(&size[root], 1u);
+    if (k < m) {
+        const uint4 mt = meta[k];
+        if (mt.z & kDynA) root = ufFind(parent, mt.x);
+        else if (mt.z & kDynB) root = ufFind(parent, mt.y);
+        islandOf[k] = root;
+    }
+    // The lanes in the first active lane's island add together (one atomic
+    // for a wavefront inside one big island); the others add their own.
+    const uint64_t act = __ballot(root != kNone);
+    if (!act) return;
+    const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)root, __ffsll((unsigned long long)act) - 1);
+    const uint64_t same = __ballot(root == lead);
+    if (root == lead) {
+        if ((threadIdx.x & 63) == (uint32_t)(__ffsll((unsigned long long)same) - 1))
+            atomicAdd(&size[lead], (uint32_t)__popcll(same));
+    } else if (root != kNone) {
+        atomicAdd(&size[root], 1u);
+    }
 }
 
 // Islands of up to kLdsIsland contacts are packed, in root order, into
@@ -474,10 +486,28 @@ __global__ __launch_bounds__(kThreads) void kBatchBodyRange(const uint32_t* __re
                                                             uint32_t* __restrict__ rangeStart,
                                                             uint32_t* __restrict__ rangeEnd) {
     uint32_t r = blockIdx.x * kThreads + threadIdx.x;
-    if (r >= n || bodyCount[r] == 0) return;
-    const uint32_t b = ldsOffset[r] / kBatch;
-    atomicMin(&rangeStart[b], bodyOffset[r]);
-    atomicMax(&rangeEnd[b], bodyOffset[r] + bodyCount[r]);
+    const bool has = r < n && bodyCount[r] != 0;
+    const uint32_t b = has ? ldsOffset[r] / kBatch : kNone;
+    uint32_t lo = has ? bodyOffset[r] : 0xffffffffu, hi = has ? bodyOffset[r] + bodyCount[r] : 0u;
+    // Roots are in batch order, so a wavefront's roots mostly share the first
+    // one's batch: those combine into one atomic pair, the others go alone.
+    const uint64_t act = __ballot(has);
+    if (!act) return;
+    const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)b, __ffsll((unsigned long long)act) - 1);
+    const bool same = has && b == lead;
+    uint32_t slo = same ? lo : 0xffffffffu, shi = same ? hi : 0u;
+    for (int off = 32; off > 0; off >>= 1) {
+        slo = min(slo, (uint32_t)__shfl_xor((int)slo, off));
+        shi = max(shi, (uint32_t)__shfl_xor((int)shi, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMin(&rangeStart[lead], slo);
+        atomicMax(&rangeEnd[lead], shi);
+    }
+    if (has && !same) {
+        atomicMin(&rangeStart[b], lo);
+        atomicMax(&rangeEnd[b], hi);
+    }
 }
 
 // Body list of the batches and each body's LDS slot within its batch.
@@ -516,14 +546,28 @@ __global__ __launch_bounds__(kThreads) void kBatchContactSlots(const uint64_t* _
     slotPair[j] = sa | (sb << 16);
 }
 
+// Colour counts go through an LDS histogram when the K colours fit it: one
+// global atomic per colour and workgroup instead of one per contact.
+constexpr uint32_t kBigHist = 2048;
 __global__ __launch_bounds__(kThreads) void kBigList(const uint32_t* __restrict__ bigFlag,
                                                      const uint32_t* __restrict__ bigPos, uint32_t m,
                                                      const uint32_t* __restrict__ color, uint32_t* __restrict__ bigIdx,
-                                                     uint32_t* __restrict__ bigColorCount) {
-    uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= m || !bigFlag[k]) return;
-    bigIdx[bigPos[k]] = k;
-    atomicAdd(&bigColorCount[color[k]], 1u);
+                                                     uint32_t* __restrict__ bigColorCount, uint32_t K) {
+    __shared__ uint32_t hist[kBigHist];
+    const bool lds = K <= kBigHist;
+    if (lds)
+        for (uint32_t q = threadIdx.x; q < K; q += kThreads) hist[q] = 0;
+    __syncthreads();
+    const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
+    if (k < m && bigFlag[k]) {
+        bigIdx[bigPos[k]] = k;
+        if (lds) atomicAdd(&hist[color[k]], 1u);
+        else atomicAdd(&bigColorCount[color[k]], 1u);
+    }
+    if (!lds) return;
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < K; q += kThreads)
+        if (hist[q]) atomicAdd(&bigColorCount[q], hist[q]);
 }
 
 // Solve position of every contact: batched contacts at their place in the

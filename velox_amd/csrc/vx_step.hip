@@ -774,7 +774,7 @@ void prepareIslands(vxhip_ctx* c, uint32_t m, uint32_t K, const uint4* meta) {
                                                    c->smallIdx.p, k + cSmall, mb);
     HIPCK(vxhost::exclusiveScan(c->bigFlag.p, c->bigPos.p, m, k + cBig, tempFor(c, vxhost::scanTempBytes(m)), s));
     kBigList<<<blocks(m), kThreads, 0, s>>>(c->bigFlag.p, c->bigPos.p, m, c->colorSolve.p, c->bigIdx.p,
-                                            c->bigColorCount.p);
+                                            c->bigColorCount.p, K);
     HIPCK(vxhost::exclusiveScan(c->bigColorCount.p, c->bigColorStart.p, K, c->bigColorStart.p + K,
                                 tempFor(c, vxhost::scanTempBytes(K)), s));
     HIPCK(hipGetLastError());
