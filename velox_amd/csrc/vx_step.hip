@@ -720,9 +720,10 @@ void colorOrder(vxhip_ctx* c, uint32_t m, uint32_t K) {
                             vxhost::bitsFor(K ? K - 1 : 0), tempFor(c, vxhost::radixTempBytes(m)), s));
     HIPCK(vxhost::exclusiveScan(c->colorCount.p, c->colorStart.p, K, c->colorStart.p + K,
                                 tempFor(c, vxhost::scanTempBytes(K)), s));
+    // Read on the host only by the positional correction, after the island
+    // set-up's counter readback has synchronised the stream.
     c->colorStartHost.resize(K + 1);
     HIPCK(hipMemcpyAsync(c->colorStartHost.data(), c->colorStart.p, (K + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIPCK(hipStreamSynchronize(s));
 }
 
 uint32_t maxColorSize(const std::vector<uint32_t>& start) {
@@ -1286,8 +1287,8 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
                                               c->ccdGap.p, k + cCcdFlagged, k + cCcdFirst, toiDone);
     solveLaunchEnd(c, VXHIP_KERNEL_CCD_CHECK, np);
     HIPCK(hipGetLastError());
-    readCounters(c);
-    if (c->hostCounters[cCcdFlagged] == 0) return;
+    // No host round trip for the flagged count: with none flagged, the closure
+    // finds no relevant pair and every later launch of the net returns at once.
     // Pairs that can see a rewound body (flagged pairs and what follows them
     // on shared dynamic bodies).
     c->ccdRel.reserve(np); c->ccdRelPos.reserve(np); c->ccdList.reserve(np); c->ccdFirstRel.reserve(n);
