@@ -75,9 +75,22 @@ __device__ inline bool oversized(const vx_body& b, float extent, float cell) {
 
 // narrowphase.h:369-398; planes unbounded, meshes use their root box
 // (backend_cuda.cu:103-129).  The largest finite extent sets the cell edge.
+// Extent histogram of the grid-cell choice: 1/8-octave buckets of the float
+// bits, 2^-16 to 2^16 (the ends clamped).
+constexpr uint32_t kExtBuckets = 256, kExtBucketBase = (127u - 16u) << 3;
+__device__ inline uint32_t extBucket(float e) {
+    const int b = (int)(__float_as_uint(e) >> 20) - (int)kExtBucketBase;
+    return (uint32_t)(b < 0 ? 0 : b > (int)kExtBuckets - 1 ? (int)kExtBuckets - 1 : b);
+}
+__device__ inline float extBucketEnd(uint32_t b) { return __uint_as_float((b + kExtBucketBase + 1) << 20); }
+static_assert(kThreads == (int)kExtBuckets, "kAabb keeps one histogram bucket per thread");
+
 __global__ __launch_bounds__(kThreads) void kAabb(const vx_body* __restrict__ bodies, uint32_t n, Soup soup, float dt,
                                                   float4* __restrict__ lo, float4* __restrict__ hi,
-                                                  uint32_t* __restrict__ extentBits) {
+                                                  uint32_t* __restrict__ extentBits, uint32_t* __restrict__ extHist) {
+    __shared__ uint32_t hist[kExtBuckets];
+    hist[threadIdx.x] = 0;
+    __syncthreads();
     uint32_t i = blockIdx.x * kThreads + threadIdx.x;
     float ext = 0.0f;
     if (i < n) {
@@ -95,9 +108,45 @@ __global__ __launch_bounds__(kThreads) void kAabb(const vx_body* __restrict__ bo
         lo[i] = f4(l);
         hi[i] = f4(h);
     }
+    if (ext > 0.0f) atomicAdd(&hist[extBucket(ext)], 1u);
     // One atomic per wavefront: positive floats order like their bit patterns.
     for (int off = 32; off > 0; off >>= 1) ext = fmaxf(ext, __shfl_xor(ext, off));
     if ((threadIdx.x & 63) == 0 && ext > 0.0f) atomicMax(extentBits, __float_as_uint(ext));
+    __syncthreads();
+    if (hist[threadIdx.x]) atomicAdd(&extHist[threadIdx.x], hist[threadIdx.x]);
+}
+
+// The grid cell's extent: the largest body extent, unless a few bodies
+// (spinning ones, whose swept boxes grow with their angular speed) are far
+// larger than the rest.  Then the cell is the end of the smallest extent
+// bucket with at most 32 + n/2048 bodies above it, and those bodies take the
+// oversize path (each tested against every body): the pair set is the same
+// whatever the cell, since every body larger than the cell is oversize.
+// Clears the histogram for the next step.  One workgroup of kExtBuckets.
+__global__ __launch_bounds__(kExtBuckets) void kCellSize(uint32_t* __restrict__ extHist, uint32_t n,
+                                                         uint32_t* __restrict__ extentBits) {
+    __shared__ uint32_t above[kExtBuckets + 1];  // above[b]: bodies in buckets b and later
+    __shared__ uint32_t pick;
+    const uint32_t t = threadIdx.x;
+    above[t] = extHist[t];
+    extHist[t] = 0;
+    if (t == 0) { above[kExtBuckets] = 0; pick = kExtBuckets; }
+    __syncthreads();
+    for (uint32_t off = 1; off < kExtBuckets; off <<= 1) {
+        const uint32_t v = t + off < kExtBuckets ? above[t + off] : 0u;
+        __syncthreads();
+        above[t] += v;
+        __syncthreads();
+    }
+    // The smallest bucket with at most `limit` bodies in later buckets.
+    const uint32_t limit = 32u + n / 2048u;
+    const bool ok = above[t + 1] <= limit, okPrev = t > 0 && above[t] <= limit;
+    if (ok && !okPrev) pick = t;
+    __syncthreads();
+    if (t != 0 || pick >= kExtBuckets) return;
+    const float maxExt = __uint_as_float(*extentBits);
+    const float edge = extBucketEnd(pick);
+    if (maxExt > 1.5f * edge) *extentBits = __float_as_uint(edge);
 }
 
 __device__ inline float gridCellOf(const uint32_t* extentBits) {

@@ -256,6 +256,7 @@ struct vxhip_ctx {
     Buf<float4> ccdGap;
     Buf<uint8_t> temp;
     Buf<uint32_t> counters;  // device counters, see Counter
+    Buf<uint32_t> extHist;   // body extents by bucket (kAabb), cleared by kCellSize
     uint32_t* hostCounters = nullptr;
     uint32_t pairCap = 0;
     uint32_t pairsCount = 0;
@@ -535,7 +536,8 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
     c->keys.reserve(n); c->keysTmp.reserve(n); c->idx.reserve(n); c->idxTmp.reserve(n); c->oversize.reserve(n);
     uint32_t* k = c->counters.p;
     kResetCounters<<<1, 128, 0, s>>>(k);
-    kAabb<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, dt, c->lo.p, c->hi.p, k + cExtent);
+    kAabb<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, dt, c->lo.p, c->hi.p, k + cExtent, c->extHist.p);
+    kCellSize<<<1, kExtBuckets, 0, s>>>(c->extHist.p, n, k + cExtent);
     kCellBounds<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, k + cExtent, (int*)(k + cBounds),
                                                k + cOversize2);
     HIPCK(hipGetLastError());
@@ -1982,6 +1984,9 @@ vxhip_ctx* newContext() {
     kFill32<<<1, 64, 0, c->stream>>>(c->batchQueue.p, 2, 0u);
     HIPCK(hipGetLastError());
     c->counters.reserve(cCount);
+    c->extHist.reserve(kExtBuckets);
+    kFill32<<<1, kExtBuckets, 0, c->stream>>>(c->extHist.p, kExtBuckets, 0u);
+    HIPCK(hipGetLastError());
     HIPCK(hipHostMalloc((void**)&c->hostCounters, cCount * sizeof(uint32_t), hipHostMallocDefault));
     return c;
 }
