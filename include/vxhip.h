@@ -194,7 +194,8 @@ typedef struct vxhip_tuning {
                                     2: one stream (1) */
     uint32_t graphSubsteps;      /* 1: substeps 2..S captured once per step in a hipGraph and replayed (when the
                                     step has no big island, joints or recording); 2: launched one by one (2) */
-    uint32_t reserved;
+    uint32_t jointPackMax;       /* joint packs (whole components of the joint graph, solved by one workgroup
+                                    each) above this many joints take a launch per colour instead (4096) */
 } vxhip_tuning;
 
 int vxhip_world_set_tuning(vxhip_ctx* ctx, const vxhip_tuning* tuning);

@@ -1,7 +1,8 @@
 """Scheduling knobs (vxhip_tuning): every setting must step a world bit for
 bit the same.  Covers the persistent longest-first island-batch sweeps, the
 512-thread batch workgroups, the colour-launch threshold, the island limit,
-the side stream and the hipGraph-captured substeps (each graph replay checked
+the joint-pack limit (packs of one workgroup against a launch per colour), the
+side stream and the hipGraph-captured substeps (each graph replay checked
 against eagerly launched substeps, step by step, with the program's own syncs
 and downloads between the replays).  Needs an MI355X."""
 import numpy as np
@@ -12,7 +13,7 @@ from vxfix import OracleWorld, as_bodies, bits_equal, body_state, load_case
 pytestmark = pytest.mark.gpu
 
 DEFAULTS = dict(batchThreads=256, persistentBatches=True, longestFirst=True, colorLaunchRows=1024,
-                islandLimit=4096, sideStream=True, graphSubsteps=False)
+                islandLimit=4096, sideStream=True, graphSubsteps=False, jointPackMax=4096)
 
 
 @pytest.fixture(scope="module")
@@ -51,6 +52,7 @@ def test_tuning_round_trip(ctx):
     dict(longestFirst=False),
     dict(graphSubsteps=True),
     dict(graphSubsteps=True, batchThreads=512, persistentBatches=False),
+    dict(jointPackMax=1),
 ])
 def test_batched_worlds_any_knobs_bitwise(ctx, knobs):
     """256 batched worlds, 12 steps: each knob setting against the defaults,

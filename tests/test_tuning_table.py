@@ -5,7 +5,7 @@ import json
 from velox_amd import tuning
 from velox_amd.device import Tuning
 
-KNOWN = {name for name, _ in Tuning._fields_} - {"reserved"}
+KNOWN = {name for name, _ in Tuning._fields_}
 
 
 def test_table_is_well_formed():

@@ -22,6 +22,8 @@ Knobs (include/vxhip.h, vxhip_tuning):
   islandLimit        islands above this many contacts go colour by colour over the device
   sideStream         island batches beside the big-island chain on a second stream
   graphSubsteps      substeps 2..S replayed from a captured hipGraph
+  jointPackMax       joint packs above this size take a launch per colour (not
+                     tuned here: the workloads' joint components are small)
 """
 import argparse
 import json

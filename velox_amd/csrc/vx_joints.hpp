@@ -104,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void kJointColor(vx_body* bodies, vx_join
 // the state the per-colour launches give it.  Static and kinematic endpoints
 // are read by several packs and written back unchanged.
 constexpr uint32_t kJointPack = 256;        // a pack is closed at this many joints
-constexpr uint32_t kJointPackMax = 4096;    // larger packs take the per-colour launches
+constexpr uint32_t kJointPackMax = 4096;    // larger packs take the per-colour launches (default)
 constexpr int kJointPackThreads = 256;
 
 __global__ __launch_bounds__(kJointPackThreads) void kJointPacks(vx_body* bodies, vx_joint* joints,

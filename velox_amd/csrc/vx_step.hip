@@ -235,8 +235,9 @@ struct vxhip_ctx {
     // Joint packs: joints whose dynamic bodies connect them (the components of
     // the joint graph), concatenated into packs of about kJointPack joints,
     // each solved colour by colour in one workgroup.  Packs above
-    // kJointPackMax joints go through the per-colour launches (jointOrder).
-    uint32_t jointPacks = 0;
+    // jointPackMax joints (a tuning knob) go through the per-colour launches
+    // (jointOrder).
+    uint32_t jointPacks = 0, jointPackMax = kJointPackMax;
     Buf<uint32_t> packOrder, packRunStart, packFirstRun;
     Buf<uint64_t> jointExcluded;
     Buf<vx_contact> contactsTmp;
@@ -1122,7 +1123,7 @@ void prepareJoints(vxhip_ctx* c) {
         std::vector<std::vector<uint32_t>> byPack(np);
         for (uint32_t k = 0; k < nj; ++k) byPack[packOf[k]].push_back(k);
         for (uint32_t p = 0; p < np; ++p) {
-            if (packSize[p] > kJointPackMax) {
+            if (packSize[p] > c->jointPackMax) {
                 for (uint32_t k : byPack[p]) bigJoint[k] = 1;
                 continue;
             }
@@ -2150,6 +2151,11 @@ int vxhip_world_set_tuning(vxhip_ctx* c, const vxhip_tuning* t) {
         c->colorLaunchRows = t->colorLaunchRows ? t->colorLaunchRows : 1024u;
         c->sideOn = t->sideStream != 2;
         c->graphSubsteps = t->graphSubsteps == 1;
+        const uint32_t packMax = t->jointPackMax ? t->jointPackMax : kJointPackMax;
+        if (packMax != c->jointPackMax) {
+            c->jointPackMax = packMax;
+            if (c->jointCount) prepareJoints(c);  // the packs follow the limit
+        }
     });
 }
 
@@ -2164,6 +2170,7 @@ int vxhip_world_get_tuning(vxhip_ctx* c, vxhip_tuning* out) {
         out->colorLaunchRows = c->colorLaunchRows;
         out->sideStream = c->sideOn ? 1u : 2u;
         out->graphSubsteps = c->graphSubsteps ? 1u : 2u;
+        out->jointPackMax = c->jointPackMax;
     });
 }
 

@@ -81,11 +81,11 @@ class Tuning(C.Structure):
     a result.  Switch fields: 0 default, 1 on, 2 off."""
     _fields_ = [("islandLimit", C.c_uint32), ("batchThreads", C.c_uint32), ("persistentBatches", C.c_uint32),
                 ("longestFirst", C.c_uint32), ("colorLaunchRows", C.c_uint32), ("sideStream", C.c_uint32),
-                ("graphSubsteps", C.c_uint32), ("reserved", C.c_uint32)]
+                ("graphSubsteps", C.c_uint32), ("jointPackMax", C.c_uint32)]
     SWITCHES = ("persistentBatches", "longestFirst", "sideStream", "graphSubsteps")
 
     def as_dict(self):
-        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
         for k in self.SWITCHES:
             d[k] = d[k] == 1
         return d
@@ -94,7 +94,7 @@ class Tuning(C.Structure):
     def from_dict(cls, d):
         t = cls()
         for name, _ in cls._fields_:
-            if name in d and name != "reserved":
+            if name in d:
                 v = d[name]
                 setattr(t, name, (1 if v else 2) if name in cls.SWITCHES else int(v))
         return t
