@@ -318,8 +318,11 @@ __global__ __launch_bounds__(kThreads) void kCcdEntries(const vx_body* __restric
 // divergent GJK work and the flagged pairs sit close together in the list;
 // each lane walks its entries in increasing order and the grid is sized to be
 // co-resident, so the lowest unfinished pair is always running and ready; a
-// spin budget bounds waits.
-constexpr int kCcdBlocks = 128;
+// spin budget bounds waits.  The kernel runs one wavefront per SIMD, so one
+// workgroup of four fits a CU: the grid takes three CUs in four (more
+// wavefronts share fewer entries each: 128 workgroups took 2.27 ms per step
+// of debris, 192 on 256 CUs 2.14 ms), keeping a quarter free.
+constexpr uint32_t kCcdGridPerCu4 = 3;  // workgroups per four CUs
 constexpr int kCoopRescues = 2;  // at most this many ready rescues share the wavefront
 
 __global__ __launch_bounds__(kThreads) void kCcdDataflow(vx_body* bodies, const vx_vec3* __restrict__ prevPos,

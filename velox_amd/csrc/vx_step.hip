@@ -1324,12 +1324,13 @@ void ccdNet(vxhip_ctx* c, uint32_t np, float dt) {
     HIPCK(vxhost::radixSort(c->keys.p, c->colorVals.p, c->keysTmp.p, c->colorVals2.p, 2 * np, nb + pbits,
                             tempFor(c, vxhost::radixTempBytes(2 * np)), s, k + cEntries));
     kColorPreds<<<blocks(2 * np), kThreads, 0, s>>>(c->keys.p, c->colorVals.p, 2 * np, pbits, c->pred.p, k + cEntries);
-    // A fixed co-resident grid; entry t goes to wavefront t mod (grid x 4).
+    // A co-resident grid; entry t goes to wavefront t mod (grid x 4).
     solveLaunchBegin(c);
-    kCcdDataflow<<<kCcdBlocks, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p,
-                                                 c->ccdList.p, k + cCcdRelevant, nb, c->ccdFlag.p, c->ccdGap.p,
-                                                 c->pred.p, c->ccdDone.p, k + cCcdExamined, k + cCcdRescued,
-                                                 k + cColorError, toiDone);
+    const uint32_t ccdGrid = std::max<uint32_t>(1u, c->cuCount * kCcdGridPerCu4 / 4);
+    kCcdDataflow<<<ccdGrid, kThreads, 0, s>>>(c->bodies.p, c->prevPos.p, c->prevQ.p, c->soup, dt, c->pairs.p,
+                                              c->ccdList.p, k + cCcdRelevant, nb, c->ccdFlag.p, c->ccdGap.p,
+                                              c->pred.p, c->ccdDone.p, k + cCcdExamined, k + cCcdRescued,
+                                              k + cColorError, toiDone);
     solveLaunchEnd(c, VXHIP_KERNEL_CCD_DATAFLOW, 0);
     HIPCK(hipGetLastError());
 }
