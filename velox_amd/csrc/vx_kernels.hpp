@@ -105,7 +105,8 @@ __global__ __launch_bounds__(kThreads) void kAabb(const vx_body* __restrict__ bo
             float e = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
             if (e < 1e6f) ext = e;
         }
-        lo[i] = f4(l);
+        // lo.w marks the shapes that are always oversize (planes, meshes).
+        lo[i] = f4(l, b.shape == VX_SHAPE_PLANE || b.shape == VX_SHAPE_MESH ? 1.0f : 0.0f);
         hi[i] = f4(h);
     }
     if (ext > 0.0f) atomicAdd(&hist[extBucket(ext)], 1u);
@@ -164,7 +165,7 @@ __device__ inline void cellOf(float4 lo, float4 hi, float inv, int& cx, int& cy,
 
 // Reduced over the wavefront first: one set of atomics per wavefront (the
 // bounds are six words every body would otherwise hit).
-__global__ __launch_bounds__(kThreads) void kCellBounds(const vx_body* __restrict__ bodies, const float4* __restrict__ lo,
+__global__ __launch_bounds__(kThreads) void kCellBounds(const float4* __restrict__ lo,
                                                         const float4* __restrict__ hi, uint32_t n,
                                                         const uint32_t* __restrict__ extentBits, int* __restrict__ bounds,
                                                         uint32_t* __restrict__ oversizeCount) {
@@ -176,8 +177,7 @@ __global__ __launch_bounds__(kThreads) void kCellBounds(const vx_body* __restric
         const float cell = gridCellOf(extentBits);
         float4 l = lo[i], h = hi[i];
         float ext = fmaxf(h.x - l.x, fmaxf(h.y - l.y, h.z - l.z));
-        const uint8_t shape = bodies[i].shape;
-        over = ext > cell || shape == VX_SHAPE_PLANE || shape == VX_SHAPE_MESH;
+        over = ext > cell || l.w != 0.0f;
         if (!over) {
             cellOf(l, h, 1.0f / cell, cx, cy, cz);
             mx = cx; my = cy; mz = cz;

@@ -539,7 +539,7 @@ uint32_t broadPhase(vxhip_ctx* c, float dt) {
     kResetCounters<<<1, 128, 0, s>>>(k);
     kAabb<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, n, c->soup, dt, c->lo.p, c->hi.p, k + cExtent, c->extHist.p);
     kCellSize<<<1, kExtBuckets, 0, s>>>(c->extHist.p, n, k + cExtent);
-    kCellBounds<<<blocks(n), kThreads, 0, s>>>(c->bodies.p, c->lo.p, c->hi.p, n, k + cExtent, (int*)(k + cBounds),
+    kCellBounds<<<blocks(n), kThreads, 0, s>>>(c->lo.p, c->hi.p, n, k + cExtent, (int*)(k + cBounds),
                                                k + cOversize2);
     HIPCK(hipGetLastError());
     readCounters(c);
